@@ -1,0 +1,100 @@
+/* matcha_amd: C ABI of the MI355X (gfx950) orientation-search library.
+ *
+ * The reference (swing-research/Matcha) has no FFI: its hot path is a set of
+ * duck-typed Python objects (Matcha, CrossCorrelationMatcher, ShiftMatcher)
+ * over torch, numba.cuda kernels, cufinufft and torch-harmonics.  Each entry
+ * point below replaces one reference function or kernel group, cited as
+ * path:line in the reference repository.  The Python classes in
+ * matcha_amd/core bind these through ctypes.
+ *
+ * Conventions
+ *  - every array argument is device memory owned by the caller, except the
+ *    plan, which owns its tables and workspaces (allocated at create time:
+ *    hot calls allocate nothing);
+ *  - complex arrays are interleaved float2 (torch complex64);
+ *  - launches go on the caller's stream; no call synchronises the host;
+ *  - return 0 on success, an MCH_ERR_* or HIP code otherwise; the message is
+ *    in mch_last_error() (per host thread);
+ *  - a plan is not re-entrant: calls that share a plan must be serialised.
+ */
+#ifndef MATCHA_AMD_H
+#define MATCHA_AMD_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCH_OK 0
+#define MCH_ERR_ARG 1001
+#define MCH_ERR_SIZE 1002
+#define MCH_ERR_HIP 1003
+#define MCH_ERR_UNSUPPORTED 1004
+
+/* Last error message of this host thread. */
+const char* mch_last_error(void);
+
+/* ---------------------------------------------------------------- SO(3) --- */
+
+/* Plan for the SO(3) search: SOFFT tables (d^l(beta_j), l < L0, FP64->FP32),
+ * DFT twiddles, and workspaces for max_batch particles x max_candidates.
+ * Replaces SOFFT._precompute (src/matcha/core/SOFFT.py:30-64) and the
+ * WignerDMatrices/WignerDGradients constructors (WignerDMatrices.py:1076-1176,
+ * WignerDGradients.py:81-152). */
+int mch_so3_plan_create(int L_max, int L0, int K, int max_batch, int max_candidates,
+                        void** out_plan);
+int mch_so3_plan_destroy(void* plan);
+
+/* Full score grid f[b, beta, alpha, gamma] (float32, (B, n, n, n), n = 2*L0*K).
+ * Replaces SOFFT.eval (src/matcha/core/SOFFT.py:67-98). sigma: (B, >= compute_size(L0))
+ * complex64 rows of stride ld_sigma (elements). */
+int mch_sofft_eval(void* plan, const void* sigma, long long ld_sigma, int B, float* grid,
+                   hipStream_t stream);
+
+/* Top-C grid points per particle as Euler angles and scores, each (B, C).
+ * Replaces SOFFT.eval + sparse_topk_per_batch_jit + SOFFT.ids_to_angles
+ * (src/matcha/core/Matcha.py:27-58,207-236; SOFFT.py:100-124). Ties: lower
+ * flat grid index first. */
+int mch_sofft_topk(void* plan, const void* sigma, long long ld_sigma, int B, int C, float* alpha,
+                   float* beta, float* gamma, float* score, hipStream_t stream);
+
+/* Score, gradient and Hessian of Re sum_{l<=L} D(alpha,beta,gamma) sigma at
+ * (B, C) given angles; out is (B*C, 10): s, g_a, g_b, g_g, H_aa, H_ab, H_ag,
+ * H_bb, H_bg, H_gg.  Replaces WignerDMatrices.D + the score einsum +
+ * WignerDGradients.set_so3_coeffs/get_derivatives
+ * (WignerDMatrices.py:1291-1375; Matcha.py:184,288; WignerDGradients.py:155-302). */
+int mch_so3_evaluate(void* plan, const void* sigma, long long ld_sigma, int B, int C, int L,
+                     const float* alpha, const float* beta, const float* gamma, float* out,
+                     hipStream_t stream);
+
+/* Newton (newton=1) or gradient (newton=0) march of (B, C) candidates over the
+ * level schedule; angles in/out, score out.  Replaces Matcha.optimize and
+ * Matcha._take_step with the singular-Hessian skip and the lr rule
+ * (src/matcha/core/Matcha.py:11-15,150-205,239-367). */
+int mch_newton_march(void* plan, const void* sigma, long long ld_sigma, int B, int C,
+                     const int* levels, int n_levels, int num_steps, int newton, float base_lr,
+                     float* alpha, float* beta, float* gamma, float* score, hipStream_t stream);
+
+/* Best candidate per particle (first maximum).  Replaces reduce_over_candidates
+ * (src/matcha/core/Matcha.py:19-24). */
+int mch_reduce_candidates(const float* alpha, const float* beta, const float* gamma,
+                          const float* score, int B, int C, float* out_alpha, float* out_beta,
+                          float* out_gamma, float* out_score, hipStream_t stream);
+
+/* Matcha.search_orientations: grid top-C, march, reduction; outputs (B,).
+ * Replaces src/matcha/core/Matcha.py:369-403. */
+int mch_search_orientations(void* plan, const void* sigma, long long ld_sigma, int B, int C,
+                            const int* levels, int n_levels, int num_steps, int newton,
+                            float base_lr, float* out_alpha, float* out_beta, float* out_gamma,
+                            float* out_score, hipStream_t stream);
+
+/* Host FP64 Wigner d^l_{m'm}(beta), 0 <= l <= L, D layout (host memory).
+ * Replaces spherical.Wigner(L).d (used by src/matcha/core/SOFFT.py:31-39). */
+int mch_wigner_d_host(double beta, int L, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MATCHA_AMD_H */

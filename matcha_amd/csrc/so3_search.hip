@@ -1,0 +1,863 @@
+// SO(3) orientation search on gfx950: SOFFT grid + top-C, and the fused
+// frequency-marched Wigner-D Newton refinement.
+//
+// Reference behaviour (swing-research/Matcha):
+//   grid search      src/matcha/core/SOFFT.py:30-124, Matcha.py:27-58,207-236
+//   Newton march     src/matcha/core/Matcha.py:11-24,150-205,239-403
+//   derivatives      src/matcha/core/WignerDGradients.py:155-302
+//   Wigner D         src/matcha/core/WignerDMatrices.py:1291-1375
+//
+// Design: the reference materialises D (B*C x Dsize) and a 15-column
+// coefficient bank every step.  Here one workgroup owns one candidate for the
+// whole march.  For every fundamental index pair (m' >= |m|) a lane marches l
+// upward with the three-term l-recurrence of d^l_{m'm}(beta) and of its first
+// and second beta-derivatives, accumulating U = sum_l sigma d (and U', U'')
+// for the four symmetry images (m',m), (-m',-m), (m,m'), (-m,-m').  Score,
+// gradient and Hessian are then closed-form contractions of U with the
+// alpha/gamma phases; nothing of size Dsize is ever written.
+#include "common.hpp"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdlib.h>
+#include <vector>
+
+namespace mch {
+
+static thread_local char g_err[512];
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+const char* get_error() { return g_err; }
+
+// ---------------------------------------------------------------------------
+// host FP64 Wigner d table (l-recurrence, textbook convention)
+// ---------------------------------------------------------------------------
+// d^l_{m'm}(beta) for all 0 <= l <= L, D layout.  Seeds at l0 = max(|m'|,|m|)
+// come from the closed forms of the four edge cases of Wigner's sum.
+static void host_wigner_d(double beta, int L, double* out /* dsize(L) */) {
+  const double c = cos(0.5 * beta), s = sin(0.5 * beta);
+  const double x = cos(beta);
+  auto lf = [](int n) { return lgamma((double)n + 1.0); };
+  auto ipow = [](double v, int p) {
+    double r = 1.0;
+    for (int i = 0; i < p; ++i) r *= v;
+    return r;
+  };
+  for (int mp = -L; mp <= L; ++mp) {
+    for (int m = -L; m <= L; ++m) {
+      const int j = abs(mp) > abs(m) ? abs(mp) : abs(m);
+      int a, b, k;
+      double sign;
+      if (mp == j) {
+        a = mp + m; b = mp - m; k = mp + m; sign = ((mp - m) & 1) ? -1.0 : 1.0;
+      } else if (mp == -j) {
+        a = j - m; b = j + m; k = j + m; sign = 1.0;
+      } else if (m == j) {
+        a = j + mp; b = j - mp; k = j + mp; sign = 1.0;
+      } else {  // m == -j
+        a = j - mp; b = j + mp; k = j - mp; sign = ((j + mp) & 1) ? -1.0 : 1.0;
+      }
+      const double K = sign * exp(0.5 * (lf(2 * j) - lf(k) - lf(2 * j - k)));
+      double d = K * ipow(c, a) * ipow(s, b);
+      double pd = 0.0, Cprev = 0.0;
+      out[dindex(j, mp, m)] = d;
+      for (int l = j + 1; l <= L; ++l) {
+        const double P = (double)(l * l - m * m) * (double)(l * l - mp * mp);
+        const double A = (double)l * (2 * l - 1) / sqrt(P);
+        const double Bc = (l > 1) ? (double)(m * mp) / ((double)(l - 1) * l) : 0.0;
+        const double nd = A * ((x - Bc) * d - Cprev * pd);
+        pd = d;
+        d = nd;
+        Cprev = sqrt(P) / ((double)l * (2 * l + 1));
+        out[dindex(l, mp, m)] = d;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+struct So3Plan {
+  int Lmax, L0, K, n, max_batch, max_cand;
+  int nsz;               // compute_size(L0) = dsize(L0-1): entries used by the grid
+  float* d_grid;         // (n, nsz)
+  float2* twiddle;       // (n) e^{2 pi i k / n}
+  float2* sig_pk;        // (max_batch, pk_size(Lmax))
+  float* part_val;       // (max_batch, n, max_cand)
+  int* part_idx;         // (max_batch, n, max_cand)
+  float* cand;           // (max_batch, max_cand, 4)
+  float* lgf;            // log2 factorial table, size 2*Lmax+2
+};
+
+}  // namespace mch
+
+using mch::So3Plan;
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+// sigma (reference D layout, row stride ldsig) -> 4-image repacked layout.
+__global__ void so3_repack_kernel(const float2* __restrict__ sig, long long ldsig,
+                                  float2* __restrict__ pk, int Lmax, int B) {
+  const int per = mch::pk_size(Lmax) / 4;  // number of (l, pair) slots
+  const long long total = (long long)per * B;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int b = (int)(t / per);
+    int r = (int)(t - (long long)b * per);
+    // find shell l: slots of shell l are (l+1)^2, offset sum_{k<=l} k^2
+    int l = 0;
+    while (mch::pk_offset(l + 1) / 4 <= r) ++l;
+    const int p = r - mch::pk_offset(l) / 4;
+    int mp = (int)sqrtf((float)p);
+    while (mp * mp > p) --mp;
+    while ((mp + 1) * (mp + 1) <= p) ++mp;
+    const int m = p - mp * mp - mp;
+    const float2* s = sig + (long long)b * ldsig;
+    const float sg = ((m - mp) & 1) ? -1.f : 1.f;
+    float2 v0 = s[mch::dindex(l, mp, m)];
+    float2 v1 = s[mch::dindex(l, -mp, -m)];
+    float2 v2 = s[mch::dindex(l, m, mp)];
+    float2 v3 = s[mch::dindex(l, -m, -mp)];
+    v1.x *= sg; v1.y *= sg;
+    v2.x *= sg; v2.y *= sg;
+    if (m == mp || m == -mp) { v2 = make_float2(0.f, 0.f); v3 = v2; }
+    if (mp == 0) v1 = make_float2(0.f, 0.f);
+    float4* o = reinterpret_cast<float4*>(pk + (long long)b * mch::pk_size(Lmax) +
+                                          mch::pk_offset(l) + 4 * p);
+    o[0] = make_float4(v0.x, v0.y, v1.x, v1.y);
+    o[1] = make_float4(v2.x, v2.y, v3.x, v3.y);
+  }
+}
+
+// One workgroup per (particle, beta row): F(u,v) = sum_l d^l_{uv}(beta_j) sigma,
+// 2-D inverse DFT over (alpha, gamma), local top-C of the n*n slice.
+template <int MAXPER>
+__global__ __launch_bounds__(256) void sofft_slice_kernel(
+    const float2* __restrict__ sig, long long ldsig, const float* __restrict__ dgrid,
+    const float2* __restrict__ twiddle, int L0, int n, int nsz, int C,
+    float* __restrict__ part_val, int* __restrict__ part_idx, float* __restrict__ grid_out) {
+  extern __shared__ float4 smem4[];
+  float2* smem = reinterpret_cast<float2*>(smem4);
+  const int j = blockIdx.x;
+  const int b = blockIdx.y;
+  const int W = 2 * L0 - 1;
+  float2* F = smem;            // W*W
+  float2* G = F + W * W;       // W*n
+  float2* tw = G + W * n;      // n
+  __shared__ float red_v[4];
+  __shared__ int red_i[4];
+  const int tid = threadIdx.x;
+  for (int k = tid; k < n; k += blockDim.x) tw[k] = twiddle[k];
+  const float2* s = sig + (long long)b * ldsig;
+  const float* dj = dgrid + (long long)j * nsz;
+  for (int q = tid; q < W * W; q += blockDim.x) {
+    const int u = q / W - (L0 - 1), v = q % W - (L0 - 1);
+    const int l0 = abs(u) > abs(v) ? abs(u) : abs(v);
+    float ax = 0.f, ay = 0.f;
+    for (int l = l0; l < L0; ++l) {
+      const int id = mch::dindex(l, u, v);
+      const float dv = dj[id];
+      const float2 sv = s[id];
+      ax += dv * sv.x;
+      ay += dv * sv.y;
+    }
+    F[q] = make_float2(ax, ay);
+  }
+  __syncthreads();
+  // stage A: G(u, g) = sum_v F(u, v) e^{2 pi i v g / n}
+  for (int q = tid; q < W * n; q += blockDim.x) {
+    const int ui = q / n, g = q % n;
+    float ax = 0.f, ay = 0.f;
+    for (int vi = 0; vi < W; ++vi) {
+      const int v = vi - (L0 - 1);
+      int k = (v * g) % n;
+      if (k < 0) k += n;
+      const float2 f = F[ui * W + vi], t = tw[k];
+      ax += f.x * t.x - f.y * t.y;
+      ay += f.x * t.y + f.y * t.x;
+    }
+    G[q] = make_float2(ax, ay);
+  }
+  __syncthreads();
+  // stage B: f(a, g) = Re sum_u G(u, g) e^{2 pi i u a / n} / n^2 ; values stay
+  // in registers (static indexing: MAXPER >= ceil(n*n / 256))
+  const float inv = 1.0f / ((float)n * (float)n);
+  const int nn = n * n;
+  float vals[MAXPER];
+#pragma unroll
+  for (int r = 0; r < MAXPER; ++r) {
+    const int q = tid + r * 256;
+    float acc = -INFINITY;
+    if (q < nn) {
+      const int a = q / n, g = q % n;
+      acc = 0.f;
+      for (int ui = 0; ui < W; ++ui) {
+        const int u = ui - (L0 - 1);
+        int k = (u * a) % n;
+        if (k < 0) k += n;
+        const float2 gv = G[ui * n + g], t = tw[k];
+        acc += gv.x * t.x - gv.y * t.y;
+      }
+      acc *= inv;
+    }
+    vals[r] = acc;
+    if (grid_out != nullptr && q < nn) grid_out[((long long)b * n + j) * nn + q] = acc;
+  }
+  // C rounds of block argmax (value desc, flat index asc)
+  const int lane = tid & 63, wave = tid >> 6;
+  for (int c = 0; c < C; ++c) {
+    float bv = -INFINITY;
+    int bi = 0x7fffffff;
+#pragma unroll
+    for (int r = 0; r < MAXPER; ++r) {
+      const int q = tid + r * 256;
+      if (q < nn && (vals[r] > bv || (vals[r] == bv && q < bi))) { bv = vals[r]; bi = q; }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const float ov = __shfl_xor(bv, off, 64);
+      const int oi = __shfl_xor(bi, off, 64);
+      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
+    __syncthreads();
+    bv = red_v[0]; bi = red_i[0];
+    for (int w = 1; w < 4; ++w) {
+      if (red_v[w] > bv || (red_v[w] == bv && red_i[w] < bi)) { bv = red_v[w]; bi = red_i[w]; }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < MAXPER; ++r)
+      if (tid + r * 256 == bi) vals[r] = -INFINITY;  // taken
+    if (tid == 0) {
+      const long long o = ((long long)b * n + j) * C + c;
+      part_val[o] = bv;
+      part_idx[o] = (bi == 0x7fffffff) ? -1 : j * nn + bi;
+    }
+  }
+}
+
+// Merge the per-row candidates into the global top-C per particle and convert
+// flat ids to Euler angles (SOFFT.ids_to_angles).
+__global__ __launch_bounds__(256) void sofft_merge_kernel(
+    const float* __restrict__ part_val, const int* __restrict__ part_idx, int n, int C,
+    float* __restrict__ out_a, float* __restrict__ out_b, float* __restrict__ out_g,
+    float* __restrict__ out_s, int out_ld) {
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int tot = n * C;
+  __shared__ float sv[4];
+  __shared__ int si[4];
+  __shared__ int taken_slot;
+  extern __shared__ unsigned char used[];  // tot bytes
+  for (int q = tid; q < tot; q += blockDim.x) used[q] = 0;
+  __syncthreads();
+  const float* pv = part_val + (long long)b * tot;
+  const int* pi = part_idx + (long long)b * tot;
+  const int lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+  const float bstep = (float)(M_PI / (double)(n - 1));
+  const float astep = (float)(2.0 * M_PI / (double)n);
+  for (int c = 0; c < C; ++c) {
+    float bv = -INFINITY;
+    int bi = 0x7fffffff, bslot = -1;
+    for (int q = tid; q < tot; q += blockDim.x) {
+      if (used[q] || pi[q] < 0) continue;
+      const float v = pv[q];
+      const int id = pi[q];
+      if (v > bv || (v == bv && id < bi)) { bv = v; bi = id; bslot = q; }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const float ov = __shfl_xor(bv, off, 64);
+      const int oi = __shfl_xor(bi, off, 64);
+      const int os = __shfl_xor(bslot, off, 64);
+      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; bslot = os; }
+    }
+    __shared__ int sslot[4];
+    if (lane == 0) { sv[wave] = bv; si[wave] = bi; sslot[wave] = bslot; }
+    __syncthreads();
+    if (tid == 0) {
+      float v = sv[0];
+      int id = si[0], sl = sslot[0];
+      for (int w = 1; w < nw; ++w)
+        if (sv[w] > v || (sv[w] == v && si[w] < id)) { v = sv[w]; id = si[w]; sl = sslot[w]; }
+      taken_slot = sl;
+      const long long o = (long long)b * out_ld + c;
+      if (id == 0x7fffffff) {
+        // fewer than C grid points: mirror the reference's -1 fill
+        out_b[o] = -bstep; out_a[o] = -astep; out_g[o] = -astep; out_s[o] = -1.f;
+      } else {
+        const int nn = n * n;
+        const int ib = id / nn, ia = (id / n) % n, ig = id % n;
+        out_b[o] = (float)ib * bstep;
+        out_a[o] = (float)ia * astep;
+        out_g[o] = (float)ig * astep;
+        out_s[o] = v;
+      }
+    }
+    __syncthreads();
+    if (tid == 0 && taken_slot >= 0) used[taken_slot] = 1;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fused Newton march
+// ---------------------------------------------------------------------------
+struct MarchArgs {
+  const float2* sig_pk;  // (B, pk_stride)
+  int pk_stride;
+  int B, C;
+  float* alpha;  // (B*C) in/out
+  float* beta;
+  float* gamma;
+  float* score;  // (B*C) out
+  int n_levels;
+  int levels[16];
+  int num_steps;
+  int newton;    // 1: Newton step, 0: gradient step
+  float base_lr;
+  int Lmax;
+};
+
+constexpr int kMarchThreads = 256;
+constexpr int kMarchWaves = kMarchThreads / 64;
+constexpr int kMaxL = 96;
+
+struct EvalOut {
+  float v[10];  // s, ga, gb, gg, Haa, Hab, Hag, Hbb, Hbg, Hgg
+};
+
+// Evaluate score, gradient and Hessian of Re sum_{l<=Li} D(a,b,g) sigma.
+__device__ void march_eval(const float2* __restrict__ sp, int Li, float beta,
+                           const float2* __restrict__ za, const float2* __restrict__ zg,
+                           const float* __restrict__ lgf, float* red, float out[10]) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  float acc[10];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) acc[i] = 0.f;
+  const float x = cosf(beta), sx = sinf(beta);
+  const float hc = cosf(0.5f * beta), hs = sinf(0.5f * beta);
+  const float lc = log2f(fabsf(hc)), ls = log2f(fabsf(hs));
+  const bool cneg = hc < 0.f, sneg = hs < 0.f;
+  const int P = (Li + 1) * (Li + 1);
+  const int ntasks = (P + 63) / 64;
+  for (int t = wave; t < ntasks; t += kMarchWaves) {
+    const int p = t * 64 + lane;
+    const bool valid = p < P;
+    const int pc = valid ? p : P - 1;
+    int mp = (int)sqrtf((float)pc);
+    if (mp * mp > pc) --mp;
+    if ((mp + 1) * (mp + 1) <= pc) ++mp;
+    const int m = pc - mp * mp - mp;
+    int lstart = (int)sqrtf((float)(t * 64));
+    if (lstart * lstart > t * 64) --lstart;
+    if ((lstart + 1) * (lstart + 1) <= t * 64) ++lstart;
+    // seed at l0 = mp: d = (-1)^{mp-m} sqrt(C(2mp, mp+m)) c^{mp+m} s^{mp-m}
+    const int a = mp + m, bb = mp - m;
+    const float l2K = 0.5f * (lgf[2 * mp] - lgf[a] - lgf[2 * mp - a]);
+    auto term = [&](int pa, int pb) -> float {
+      if (pa < 0 || pb < 0) return 0.f;
+      float e = l2K;
+      if (pa) e += (float)pa * lc;
+      if (pb) e += (float)pb * ls;
+      float v = exp2f(e);
+      if ((cneg && (pa & 1)) != (sneg && (pb & 1))) v = -v;
+      return v;
+    };
+    const float sgn = (bb & 1) ? -1.f : 1.f;
+    float d = sgn * term(a, bb);
+    float dp = sgn * 0.5f * ((float)bb * term(a + 1, bb - 1) - (float)a * term(a - 1, bb + 1));
+    float dpp = sgn * 0.25f *
+                ((float)(bb * (bb - 1)) * term(a + 2, bb - 2) -
+                 (float)(bb * (a + 1) + a * (bb + 1)) * term(a, bb) +
+                 (float)(a * (a - 1)) * term(a - 2, bb + 2));
+    float pd = 0.f, pdp = 0.f, pdpp = 0.f, Cprev = 0.f;
+    float2 U[4], U1[4], U2[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      U[i] = make_float2(0.f, 0.f);
+      U1[i] = U[i];
+      U2[i] = U[i];
+    }
+    const float fm = (float)m, fmp = (float)mp;
+    for (int l = lstart; l <= Li; ++l) {
+      if (l > mp) {
+        const float fl = (float)l;
+        const float Pl = (fl * fl - fm * fm) * (fl * fl - fmp * fmp);
+        const float r = rsqrtf(Pl);
+        const float A = fl * (2.f * fl - 1.f) * r;
+        const float Bc = (l > 1) ? fm * fmp / ((fl - 1.f) * fl) : 0.f;
+        const float xb = x - Bc;
+        const float nd = A * (xb * d - Cprev * pd);
+        const float ndp = A * (xb * dp - sx * d - Cprev * pdp);
+        const float ndpp = A * (xb * dpp - 2.f * sx * dp - x * d - Cprev * pdpp);
+        pd = d; pdp = dp; pdpp = dpp;
+        d = nd; dp = ndp; dpp = ndpp;
+        Cprev = Pl * r / (fl * (2.f * fl + 1.f));
+      }
+      if (valid && l >= mp) {
+        const float4* q = reinterpret_cast<const float4*>(sp + mch::pk_offset(l) + 4 * p);
+        const float4 s01 = q[0], s23 = q[1];
+        const float2 sv[4] = {make_float2(s01.x, s01.y), make_float2(s01.z, s01.w),
+                              make_float2(s23.x, s23.y), make_float2(s23.z, s23.w)};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          U[i].x += sv[i].x * d;   U[i].y += sv[i].y * d;
+          U1[i].x += sv[i].x * dp; U1[i].y += sv[i].y * dp;
+          U2[i].x += sv[i].x * dpp; U2[i].y += sv[i].y * dpp;
+        }
+      }
+    }
+    if (valid) {
+      const int us[4] = {mp, -mp, m, -m};
+      const int vs[4] = {m, -m, mp, -mp};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int u = us[i], v = vs[i];
+        float2 pa = za[abs(u)];
+        if (u < 0) pa.y = -pa.y;
+        float2 pg = zg[abs(v)];
+        if (v < 0) pg.y = -pg.y;
+        const float2 ph = make_float2(pa.x * pg.x - pa.y * pg.y, pa.x * pg.y + pa.y * pg.x);
+        const float q0r = ph.x * U[i].x - ph.y * U[i].y;
+        const float q0i = ph.x * U[i].y + ph.y * U[i].x;
+        const float q1r = ph.x * U1[i].x - ph.y * U1[i].y;
+        const float q1i = ph.x * U1[i].y + ph.y * U1[i].x;
+        const float q2r = ph.x * U2[i].x - ph.y * U2[i].y;
+        const float fu = (float)u, fv = (float)v;
+        acc[0] += q0r;
+        acc[1] += -fu * q0i;
+        acc[2] += q1r;
+        acc[3] += -fv * q0i;
+        acc[4] += -fu * fu * q0r;
+        acc[5] += -fu * q1i;
+        acc[6] += -fu * fv * q0r;
+        acc[7] += q2r;
+        acc[8] += -fv * q1i;
+        acc[9] += -fv * fv * q0r;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 10; ++i) acc[i] = mch::wave_sum(acc[i]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < 10; ++i) red[i * kMarchWaves + wave] = acc[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    float v = 0.f;
+    for (int w = 0; w < kMarchWaves; ++w) v += red[i * kMarchWaves + w];
+    out[i] = v;
+  }
+}
+
+// e^{i k theta} for k = 0..L into z (FP64 argument reduction).
+__device__ void phase_table(double theta, int L, float2* z) {
+  for (int k = threadIdx.x; k <= L; k += blockDim.x) {
+    double ang = (double)k * theta;
+    ang = ang - 6.283185307179586476925 * rint(ang / 6.283185307179586476925);
+    float sn, cs;
+    sincosf((float)ang, &sn, &cs);
+    z[k] = make_float2(cs, sn);
+  }
+}
+
+// 3x3 solve by LU with partial pivoting in FP64; returns false if a pivot is
+// exactly zero (torch.linalg.solve raises "singular" there, and the reference
+// then skips that candidate, Matcha.py:315-343).
+__device__ bool solve3(const double Hm[9], const double g[3], double x[3]) {
+  double A[3][4];
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c) A[r][c] = Hm[r * 3 + c];
+    A[r][3] = g[r];
+  }
+  for (int k = 0; k < 3; ++k) {
+    int piv = k;
+    double best = fabs(A[k][k]);
+    for (int r = k + 1; r < 3; ++r)
+      if (fabs(A[r][k]) > best) { best = fabs(A[r][k]); piv = r; }
+    if (best == 0.0) return false;
+    if (piv != k)
+      for (int c = 0; c < 4; ++c) { double t = A[k][c]; A[k][c] = A[piv][c]; A[piv][c] = t; }
+    for (int r = k + 1; r < 3; ++r) {
+      const double f = A[r][k] / A[k][k];
+      for (int c = k; c < 4; ++c) A[r][c] -= f * A[k][c];
+    }
+  }
+  for (int r = 2; r >= 0; --r) {
+    double v = A[r][3];
+    for (int c = r + 1; c < 3; ++c) v -= A[r][c] * x[c];
+    x[r] = v / A[r][r];
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(kMarchThreads) void so3_march_kernel(MarchArgs args,
+                                                                   const float* __restrict__ lgf_g) {
+  __shared__ float2 za[kMaxL + 1], zg[kMaxL + 1];
+  __shared__ float lgf[2 * kMaxL + 2];
+  __shared__ float red[10 * kMarchWaves];
+  __shared__ float ang[4];  // trial alpha, beta, gamma
+  const int nblk = args.B * args.C;
+  const int q = mch::xcd_remap(blockIdx.x, nblk);
+  const int b = q / args.C;
+  const int tid = threadIdx.x;
+  for (int k = tid; k <= 2 * args.Lmax + 1; k += blockDim.x) lgf[k] = lgf_g[k];
+  const float2* sp = args.sig_pk + (long long)b * args.pk_stride;
+  // thread 0 owns the optimiser state
+  float ca = args.alpha[q], cb = args.beta[q], cg = args.gamma[q];
+  float best = 0.f, lr = 0.f;
+  double gcur[3] = {0, 0, 0}, Hcur[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  float ev[10];
+  for (int li = 0; li < args.n_levels; ++li) {
+    const int Li = args.levels[li];
+    for (int st = 0; st <= args.num_steps; ++st) {
+      // thread 0 proposes the point to evaluate; everybody reads it from LDS
+      if (tid == 0) {
+        float ta = ca, tb = cb, tg = cg;
+        if (st > 0) {
+          double stepd[3];
+          if (args.newton) {
+            if (!solve3(Hcur, gcur, stepd)) { stepd[0] = stepd[1] = stepd[2] = 0.0; }
+          } else {
+            stepd[0] = -gcur[0]; stepd[1] = -gcur[1]; stepd[2] = -gcur[2];
+          }
+          ta = ca - lr * (float)stepd[0];
+          tb = cb - lr * (float)stepd[1];
+          tg = cg - lr * (float)stepd[2];
+        }
+        ang[0] = ta; ang[1] = tb; ang[2] = tg;
+      }
+      __syncthreads();
+      const float ta = ang[0], tb = ang[1], tg = ang[2];
+      phase_table((double)ta, Li, za);
+      phase_table((double)tg, Li, zg);
+      __syncthreads();
+      march_eval(sp, Li, tb, za, zg, lgf, red, ev);
+      if (tid == 0) {
+        const bool first = (st == 0);
+        const bool acc = first || (ev[0] > best);
+        if (!first) lr = acc ? lr * 1.1f : lr * 0.5f;
+        if (acc) {
+          ca = ta; cb = tb; cg = tg;
+          best = ev[0];
+          gcur[0] = ev[1]; gcur[1] = ev[2]; gcur[2] = ev[3];
+          Hcur[0] = ev[4]; Hcur[1] = ev[5]; Hcur[2] = ev[6];
+          Hcur[3] = ev[5]; Hcur[4] = ev[7]; Hcur[5] = ev[8];
+          Hcur[6] = ev[6]; Hcur[7] = ev[8]; Hcur[8] = ev[9];
+        }
+        if (first) lr = args.base_lr;
+      }
+      // every thread keeps (ca,cb,cg) consistent via the next broadcast
+      __syncthreads();
+    }
+  }
+  if (tid == 0) {
+    args.alpha[q] = ca;
+    args.beta[q] = cb;
+    args.gamma[q] = cg;
+    args.score[q] = best;
+  }
+}
+
+// Score, gradient and Hessian at given angles (one workgroup per candidate);
+// out[q*10 + k] = s, ga, gb, gg, Haa, Hab, Hag, Hbb, Hbg, Hgg.
+__global__ __launch_bounds__(kMarchThreads) void so3_eval_kernel(
+    const float2* __restrict__ sig_pk, int pk_stride, int C, int L, const float* __restrict__ al,
+    const float* __restrict__ be, const float* __restrict__ ga, const float* __restrict__ lgf_g,
+    float* __restrict__ out) {
+  __shared__ float2 za[kMaxL + 1], zg[kMaxL + 1];
+  __shared__ float lgf[2 * kMaxL + 2];
+  __shared__ float red[10 * kMarchWaves];
+  const int q = blockIdx.x;
+  const int b = q / C;
+  for (int k = threadIdx.x; k <= 2 * L + 1; k += blockDim.x) lgf[k] = lgf_g[k];
+  phase_table((double)al[q], L, za);
+  phase_table((double)ga[q], L, zg);
+  __syncthreads();
+  float ev[10];
+  march_eval(sig_pk + (long long)b * pk_stride, L, be[q], za, zg, lgf, red, ev);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 10; ++k) out[(long long)q * 10 + k] = ev[k];
+}
+
+// argmax over candidates (first maximum), Matcha.reduce_over_candidates.
+__global__ void reduce_candidates_kernel(const float* __restrict__ a, const float* __restrict__ bt,
+                                         const float* __restrict__ g, const float* __restrict__ s,
+                                         int B, int C, float* oa, float* ob, float* og, float* os) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  int bi = 0;
+  float bv = s[(long long)b * C];
+  for (int c = 1; c < C; ++c) {
+    const float v = s[(long long)b * C + c];
+    if (v > bv) { bv = v; bi = c; }
+  }
+  oa[b] = a[(long long)b * C + bi];
+  ob[b] = bt[(long long)b * C + bi];
+  og[b] = g[(long long)b * C + bi];
+  os[b] = bv;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* mch_last_error(void) { return mch::get_error(); }
+
+int mch_so3_plan_create(int L_max, int L0, int K, int max_batch, int max_candidates,
+                        void** out_plan) {
+  if (!out_plan) { mch::set_error("out_plan is NULL"); return MCH_ERR_ARG; }
+  *out_plan = nullptr;
+  if (L_max < 1 || L_max > kMaxL || L0 < 1 || L0 > L_max + 1 || K < 1 || max_batch < 1 ||
+      max_candidates < 1) {
+    mch::set_error("invalid so3 plan arguments (L_max=%d L0=%d K=%d batch=%d C=%d)", L_max, L0, K,
+                   max_batch, max_candidates);
+    return MCH_ERR_ARG;
+  }
+  const int n = 2 * L0 * K;
+  const int W = 2 * L0 - 1;
+  const size_t lds = (size_t)(W * W + W * n + n) * sizeof(float2);
+  if (lds > 160 * 1024 || (n * n + 255) / 256 > 64) {
+    mch::set_error("SOFFT grid too large for one workgroup (L0=%d, n=%d)", L0, n);
+    return MCH_ERR_UNSUPPORTED;
+  }
+  So3Plan* p = new So3Plan();
+  memset(p, 0, sizeof(*p));
+  p->Lmax = L_max; p->L0 = L0; p->K = K; p->n = n;
+  p->max_batch = max_batch; p->max_cand = max_candidates;
+  p->nsz = mch::dsize(L0 - 1);
+  std::vector<double> dd(mch::dsize(L0 - 1));
+  std::vector<float> dg((size_t)n * p->nsz);
+  for (int jb = 0; jb < n; ++jb) {
+    const double beta = (n == 1) ? 0.0 : M_PI * (double)jb / (double)(n - 1);
+    mch::host_wigner_d(beta, L0 - 1, dd.data());
+    for (int k = 0; k < p->nsz; ++k) dg[(size_t)jb * p->nsz + k] = (float)dd[k];
+  }
+  std::vector<float2> tw(n);
+  for (int k = 0; k < n; ++k) {
+    const double a = 2.0 * M_PI * (double)k / (double)n;
+    tw[k] = make_float2((float)cos(a), (float)sin(a));
+  }
+  std::vector<float> lg(2 * kMaxL + 2);
+  for (int k = 0; k < (int)lg.size(); ++k) lg[k] = (float)(lgamma((double)k + 1.0) / M_LN2);
+  int rc = MCH_OK;
+#define MCH_TRY(expr, what) do { if ((rc = mch::hip_check((expr), what)) != MCH_OK) goto fail; } while (0)
+  MCH_TRY(hipMalloc(&p->d_grid, dg.size() * sizeof(float)), "hipMalloc(d_grid)");
+  MCH_TRY(hipMemcpy(p->d_grid, dg.data(), dg.size() * sizeof(float), hipMemcpyHostToDevice), "copy d_grid");
+  MCH_TRY(hipMalloc(&p->twiddle, n * sizeof(float2)), "hipMalloc(twiddle)");
+  MCH_TRY(hipMemcpy(p->twiddle, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice), "copy twiddle");
+  MCH_TRY(hipMalloc(&p->lgf, lg.size() * sizeof(float)), "hipMalloc(lgf)");
+  MCH_TRY(hipMemcpy(p->lgf, lg.data(), lg.size() * sizeof(float), hipMemcpyHostToDevice), "copy lgf");
+  MCH_TRY(hipMalloc(&p->sig_pk, (size_t)max_batch * mch::pk_size(L_max) * sizeof(float2)), "hipMalloc(sig_pk)");
+  MCH_TRY(hipMalloc(&p->part_val, (size_t)max_batch * n * max_candidates * sizeof(float)), "hipMalloc(part_val)");
+  MCH_TRY(hipMalloc(&p->part_idx, (size_t)max_batch * n * max_candidates * sizeof(int)), "hipMalloc(part_idx)");
+  MCH_TRY(hipMalloc(&p->cand, (size_t)max_batch * max_candidates * 4 * sizeof(float)), "hipMalloc(cand)");
+  if (lds > 64 * 1024) {
+    MCH_TRY(hipFuncSetAttribute((const void*)sofft_slice_kernel<16>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+            "hipFuncSetAttribute(sofft_slice_kernel<16>)");
+    MCH_TRY(hipFuncSetAttribute((const void*)sofft_slice_kernel<64>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+            "hipFuncSetAttribute(sofft_slice_kernel<64>)");
+  }
+#undef MCH_TRY
+  *out_plan = p;
+  return MCH_OK;
+fail:
+  (void)hipFree(p->d_grid); (void)hipFree(p->twiddle); (void)hipFree(p->lgf); (void)hipFree(p->sig_pk);
+  (void)hipFree(p->part_val); (void)hipFree(p->part_idx); (void)hipFree(p->cand);
+  delete p;
+  return rc;
+}
+
+int mch_so3_plan_destroy(void* plan) {
+  So3Plan* p = (So3Plan*)plan;
+  if (!p) return MCH_OK;
+  (void)hipFree(p->d_grid); (void)hipFree(p->twiddle); (void)hipFree(p->lgf); (void)hipFree(p->sig_pk);
+  (void)hipFree(p->part_val); (void)hipFree(p->part_idx); (void)hipFree(p->cand);
+  delete p;
+  return MCH_OK;
+}
+
+// Top-C grid candidates: writes (B, C) alpha/beta/gamma/score.
+int mch_sofft_topk(void* plan, const void* sigma, long long ld_sigma, int B, int C,
+                   float* alpha, float* beta, float* gamma, float* score, hipStream_t stream) {
+  So3Plan* p = (So3Plan*)plan;
+  if (!p || !sigma || !alpha || !beta || !gamma || !score) {
+    mch::set_error("mch_sofft_topk: NULL argument");
+    return MCH_ERR_ARG;
+  }
+  if (B < 1 || B > p->max_batch || C < 1 || C > p->max_cand || ld_sigma < p->nsz) {
+    mch::set_error("mch_sofft_topk: bad sizes (B=%d C=%d ld=%lld; plan batch=%d C=%d nsz=%d)", B, C,
+                   ld_sigma, p->max_batch, p->max_cand, p->nsz);
+    return MCH_ERR_SIZE;
+  }
+  const int W = 2 * p->L0 - 1;
+  const size_t lds = (size_t)(W * W + W * p->n + p->n) * sizeof(float2);
+  const int per = (p->n * p->n + 255) / 256;
+  if (per <= 16) {
+    hipLaunchKernelGGL(sofft_slice_kernel<16>, dim3(p->n, B), dim3(256), lds, stream,
+                       (const float2*)sigma, ld_sigma, p->d_grid, p->twiddle, p->L0, p->n, p->nsz,
+                       C, p->part_val, p->part_idx, (float*)nullptr);
+  } else {
+    hipLaunchKernelGGL(sofft_slice_kernel<64>, dim3(p->n, B), dim3(256), lds, stream,
+                       (const float2*)sigma, ld_sigma, p->d_grid, p->twiddle, p->L0, p->n, p->nsz,
+                       C, p->part_val, p->part_idx, (float*)nullptr);
+  }
+  int rc = mch::hip_check(hipGetLastError(), "sofft_slice_kernel launch");
+  if (rc) return rc;
+  hipLaunchKernelGGL(sofft_merge_kernel, dim3(B), dim3(256), (size_t)p->n * C, stream,
+                     p->part_val, p->part_idx, p->n, C, alpha, beta, gamma, score, C);
+  return mch::hip_check(hipGetLastError(), "sofft_merge_kernel launch");
+}
+
+// Full score grid f[b, beta, alpha, gamma] (SOFFT.eval), float32 (B, n, n, n).
+int mch_sofft_eval(void* plan, const void* sigma, long long ld_sigma, int B, float* grid,
+                   hipStream_t stream) {
+  So3Plan* p = (So3Plan*)plan;
+  if (!p || !sigma || !grid) { mch::set_error("mch_sofft_eval: NULL argument"); return MCH_ERR_ARG; }
+  if (B < 1 || B > p->max_batch || ld_sigma < p->nsz) {
+    mch::set_error("mch_sofft_eval: bad sizes (B=%d ld=%lld)", B, ld_sigma);
+    return MCH_ERR_SIZE;
+  }
+  const int W = 2 * p->L0 - 1;
+  const size_t lds = (size_t)(W * W + W * p->n + p->n) * sizeof(float2);
+  const int per = (p->n * p->n + 255) / 256;
+  if (per <= 16) {
+    hipLaunchKernelGGL(sofft_slice_kernel<16>, dim3(p->n, B), dim3(256), lds, stream,
+                       (const float2*)sigma, ld_sigma, p->d_grid, p->twiddle, p->L0, p->n, p->nsz,
+                       0, p->part_val, p->part_idx, grid);
+  } else {
+    hipLaunchKernelGGL(sofft_slice_kernel<64>, dim3(p->n, B), dim3(256), lds, stream,
+                       (const float2*)sigma, ld_sigma, p->d_grid, p->twiddle, p->L0, p->n, p->nsz,
+                       0, p->part_val, p->part_idx, grid);
+  }
+  return mch::hip_check(hipGetLastError(), "sofft_slice_kernel launch");
+}
+
+// Newton march over the level schedule for (B, C) candidates (in/out).
+int mch_newton_march(void* plan, const void* sigma, long long ld_sigma, int B, int C,
+                     const int* levels, int n_levels, int num_steps, int newton, float base_lr,
+                     float* alpha, float* beta, float* gamma, float* score, hipStream_t stream) {
+  So3Plan* p = (So3Plan*)plan;
+  if (!p || !sigma || !levels || !alpha || !beta || !gamma || !score) {
+    mch::set_error("mch_newton_march: NULL argument");
+    return MCH_ERR_ARG;
+  }
+  if (B < 1 || B > p->max_batch || C < 1 || n_levels < 1 || n_levels > 16 || num_steps < 0 ||
+      ld_sigma < mch::dsize(p->Lmax)) {
+    mch::set_error("mch_newton_march: bad sizes (B=%d C=%d levels=%d steps=%d ld=%lld)", B, C,
+                   n_levels, num_steps, ld_sigma);
+    return MCH_ERR_SIZE;
+  }
+  MarchArgs a;
+  memset(&a, 0, sizeof(a));
+  for (int i = 0; i < n_levels; ++i) {
+    if (levels[i] < 0 || levels[i] > p->Lmax) {
+      mch::set_error("mch_newton_march: level %d outside [0, %d]", levels[i], p->Lmax);
+      return MCH_ERR_ARG;
+    }
+    a.levels[i] = levels[i];
+  }
+  const int per = mch::pk_size(p->Lmax) / 4;
+  const long long tot = (long long)per * B;
+  int nb = (int)((tot + 255) / 256);
+  if (nb > 65535 * 8) nb = 65535 * 8;
+  hipLaunchKernelGGL(so3_repack_kernel, dim3(nb), dim3(256), 0, stream, (const float2*)sigma,
+                     ld_sigma, p->sig_pk, p->Lmax, B);
+  int rc = mch::hip_check(hipGetLastError(), "so3_repack_kernel launch");
+  if (rc) return rc;
+  a.sig_pk = p->sig_pk;
+  a.pk_stride = mch::pk_size(p->Lmax);
+  a.B = B; a.C = C;
+  a.alpha = alpha; a.beta = beta; a.gamma = gamma; a.score = score;
+  a.n_levels = n_levels;
+  a.num_steps = num_steps;
+  a.newton = newton;
+  a.base_lr = base_lr;
+  a.Lmax = p->Lmax;
+  hipLaunchKernelGGL(so3_march_kernel, dim3(B * C), dim3(kMarchThreads), 0, stream, a, p->lgf);
+  return mch::hip_check(hipGetLastError(), "so3_march_kernel launch");
+}
+
+int mch_so3_evaluate(void* plan, const void* sigma, long long ld_sigma, int B, int C, int L,
+                     const float* alpha, const float* beta, const float* gamma, float* out,
+                     hipStream_t stream) {
+  So3Plan* p = (So3Plan*)plan;
+  if (!p || !sigma || !alpha || !beta || !gamma || !out) {
+    mch::set_error("mch_so3_evaluate: NULL argument");
+    return MCH_ERR_ARG;
+  }
+  if (B < 1 || B > p->max_batch || C < 1 || L < 0 || L > p->Lmax || ld_sigma < mch::dsize(p->Lmax)) {
+    mch::set_error("mch_so3_evaluate: bad sizes (B=%d C=%d L=%d)", B, C, L);
+    return MCH_ERR_SIZE;
+  }
+  const int per = mch::pk_size(p->Lmax) / 4;
+  const long long tot = (long long)per * B;
+  int nb = (int)((tot + 255) / 256);
+  if (nb > 65535 * 8) nb = 65535 * 8;
+  hipLaunchKernelGGL(so3_repack_kernel, dim3(nb), dim3(256), 0, stream, (const float2*)sigma,
+                     ld_sigma, p->sig_pk, p->Lmax, B);
+  int rc = mch::hip_check(hipGetLastError(), "so3_repack_kernel launch");
+  if (rc) return rc;
+  hipLaunchKernelGGL(so3_eval_kernel, dim3(B * C), dim3(kMarchThreads), 0, stream, p->sig_pk,
+                     mch::pk_size(p->Lmax), C, L, alpha, beta, gamma, p->lgf, out);
+  return mch::hip_check(hipGetLastError(), "so3_eval_kernel launch");
+}
+
+int mch_reduce_candidates(const float* alpha, const float* beta, const float* gamma,
+                          const float* score, int B, int C, float* out_alpha, float* out_beta,
+                          float* out_gamma, float* out_score, hipStream_t stream) {
+  if (B < 1 || C < 1) { mch::set_error("mch_reduce_candidates: bad sizes"); return MCH_ERR_SIZE; }
+  hipLaunchKernelGGL(reduce_candidates_kernel, dim3((B + 255) / 256), dim3(256), 0, stream, alpha,
+                     beta, gamma, score, B, C, out_alpha, out_beta, out_gamma, out_score);
+  return mch::hip_check(hipGetLastError(), "reduce_candidates_kernel launch");
+}
+
+// Full Matcha.search_orientations: grid top-C -> march -> best candidate.
+int mch_search_orientations(void* plan, const void* sigma, long long ld_sigma, int B, int C,
+                            const int* levels, int n_levels, int num_steps, int newton,
+                            float base_lr, float* out_alpha, float* out_beta, float* out_gamma,
+                            float* out_score, hipStream_t stream) {
+  So3Plan* p = (So3Plan*)plan;
+  if (!p) { mch::set_error("NULL plan"); return MCH_ERR_ARG; }
+  if (B > p->max_batch || C > p->max_cand) {
+    mch::set_error("mch_search_orientations: B=%d C=%d exceed plan (%d, %d)", B, C, p->max_batch,
+                   p->max_cand);
+    return MCH_ERR_SIZE;
+  }
+  float* ca = p->cand;
+  float* cb = ca + (size_t)p->max_batch * p->max_cand;
+  float* cg = cb + (size_t)p->max_batch * p->max_cand;
+  float* cs = cg + (size_t)p->max_batch * p->max_cand;
+  int rc = mch_sofft_topk(plan, sigma, ld_sigma, B, C, ca, cb, cg, cs, stream);
+  if (rc) return rc;
+  rc = mch_newton_march(plan, sigma, ld_sigma, B, C, levels, n_levels, num_steps, newton, base_lr,
+                        ca, cb, cg, cs, stream);
+  if (rc) return rc;
+  return mch_reduce_candidates(ca, cb, cg, cs, B, C, out_alpha, out_beta, out_gamma, out_score,
+                               stream);
+}
+
+// Host FP64 Wigner d table (textbook convention, D layout) -- exported so the
+// host side and tests can use the library's own tables.
+int mch_wigner_d_host(double beta, int L, double* out) {
+  if (!out || L < 0 || L > kMaxL) { mch::set_error("mch_wigner_d_host: bad args"); return MCH_ERR_ARG; }
+  mch::host_wigner_d(beta, L, out);
+  return MCH_OK;
+}
+
+}  // extern "C"
