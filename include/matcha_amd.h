@@ -89,6 +89,40 @@ int mch_search_orientations(void* plan, const void* sigma, long long ld_sigma, i
                             float base_lr, float* out_alpha, float* out_beta, float* out_gamma,
                             float* out_score, hipStream_t stream);
 
+/* ------------------------------------------------------- projection --- */
+
+/* Plan of the symmetry-adapted ball-harmonic projection: orbit table of the
+ * fundamental voxels (V0 x 16 flat indices), FP32 basis (Kpad x Ntot, K = 2*V0),
+ * per-column output maps.  Replaces FLEBasis3D.precomp (src/matcha/core/
+ * FLEBasis3D.py:111-540: NUFFT plans, SHT, interpolation matrices). */
+int mch_fle_plan_create(int N, int V0, int Kpad, int Ntot, const int* class_colpad,
+                        const int* orbit, const float* basis, const int* chan_plus,
+                        const int* chan_minus, const int* chan_m, int ne, int max_batch,
+                        void** out_plan);
+int mch_fle_plan_destroy(void* plan);
+
+/* a = B^H f: volumes (B, N^3) float32 (row stride ld_vol) -> coefficients
+ * (B, ne) complex64 (row stride lda), reference order.  Replaces
+ * FLEBasis3D.evaluate_t (src/matcha/core/FLEBasis3D.py:945-976,1026-1267). */
+int mch_fle_project(void* plan, const float* vols, long long ld_vol, int B, void* coeffs,
+                    long long lda, hipStream_t stream);
+
+/* The projection GEMM alone on the operand of the last mch_fle_project
+ * (timing aid; no reference counterpart). */
+int mch_fle_gemm_only(void* plan, int B, hipStream_t stream);
+
+/* Index table (lmax+1, 2*lmax+1, kdim) of coefficient positions for
+ * (l, m, k) (-1 = absent).  Replaces CrossCorrelationMatcher._precompute_indices
+ * (src/matcha/core/CrossCorrelationMatcher.py:133-181). */
+int mch_corr_plan_create(int lmax, int kdim, const int* idx_host, int max_batch, void** out_plan);
+int mch_corr_plan_destroy(void* plan);
+
+/* sigma[b, Dindex(l,u,v)] = (-1)^{u+v} sum_k aT[l,k,-v] conj(aV[b,l,k,-u]) and
+ * ||aV[b]|| (norms may be NULL).  Replaces CrossCorrelationMatcher.compute_sigma
+ * and the norm of get_sigma (src/matcha/core/CrossCorrelationMatcher.py:183-249). */
+int mch_compute_sigma(void* plan, const void* aT, const void* aV, long long ldv, int ne, int B,
+                      void* sigma, long long lds, float* norms, hipStream_t stream);
+
 /* Host FP64 Wigner d^l_{m'm}(beta), 0 <= l <= L, D layout (host memory).
  * Replaces spherical.Wigner(L).d (used by src/matcha/core/SOFFT.py:31-39). */
 int mch_wigner_d_host(double beta, int L, double* out);

@@ -38,6 +38,15 @@ _SIGNATURES = [
                                         c_void_p]),
     ("mch_so3_evaluate", c_int, [c_void_p, c_void_p, c_ll, c_int, c_int, c_int, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),
+    ("mch_fle_plan_create", c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(c_void_p)]),
+    ("mch_fle_plan_destroy", c_int, [c_void_p]),
+    ("mch_fle_project", c_int, [c_void_p, c_void_p, c_ll, c_int, c_void_p, c_ll, c_void_p]),
+    ("mch_fle_gemm_only", c_int, [c_void_p, c_int, c_void_p]),
+    ("mch_corr_plan_create", c_int, [c_int, c_int, c_void_p, c_int, ctypes.POINTER(c_void_p)]),
+    ("mch_corr_plan_destroy", c_int, [c_void_p]),
+    ("mch_compute_sigma", c_int, [c_void_p, c_void_p, c_void_p, c_ll, c_int, c_int, c_void_p, c_ll,
+                                  c_void_p, c_void_p]),
     ("mch_wigner_d_host", c_int, [c_double, c_int, ctypes.POINTER(c_double)]),
 ]
 

@@ -1,0 +1,333 @@
+"""Ball-harmonic expansion a = B^H f on the MI355X library.
+
+Drop-in for ``src/matcha/core/FLEBasis3D.py`` (class ``FLEBasis3D``, same
+constructor, attributes ``ne, lmax, kmax, ls, ks, ms, mds, lmds, cs,
+idlm_list, h``; ``evaluate_t``).  The reference approximates B^H f to ~eps by
+a type-2 NUFFT, two real SHTs and barycentric radial interpolation
+(``:945-976,1026-1267``).  Here the projection is computed exactly, as MFMA
+GEMMs, with the 16-element symmetry group D4h of the voxel grid folded in:
+
+* every ball voxel v is g*v0 for one fundamental voxel v0 (x >= 0, 0 <= y <= x,
+  z >= 0) and g = R^p My^t Mz^q (R: 90 degrees about z, My: y -> -y,
+  Mz: z -> -z);
+* under g, P_lm(cos theta) picks up (-1)^{q(l+m)} and e^{-i m phi} picks up
+  (-i)^{m p} and phi -> -phi for t = 1, so the orbit sum of f * conj(Y_lm)
+  collapses to P_lm(x0) [cos(m phi0) X1 + sin(m phi0) X2] (real part) and
+  [cos X3 + sin X4] (imaginary part), where X1..X4 are fixed combinations of
+  the 16 orbit values that depend only on the class ((l+m) mod 2, m mod 4);
+* a_{lkm} (m >= 0) = sum_{v0} X . W[v0, (l,k,m)] with
+  W = h c_lk w(v0) j_l(lambda_lk r0/radius) P_lm(x0) (cos | sin)(m phi0),
+  w = 1/|stabiliser(v0)|; a_{lk,-m} = (-1)^m conj(a_{lkm}) (f is real).
+
+The basis therefore spans ~V_ball/16 voxels and only m >= 0, i.e. about 1/8
+of the dense complex basis.  Tables are built here in FP64 and stored FP32.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+from scipy import optimize, special
+
+from .. import _lib
+
+N_CLASSES = 8  # (l+m) mod 2 in {0,1} x m mod 4 in {0,1,2,3}
+
+
+# ---------------------------------------------------------------------------
+# spectrum: roots of j_l by interlacing, the reference's ordering and cut
+# ---------------------------------------------------------------------------
+
+def _sph_jn_roots_interlaced(xmax: float):
+    """roots[l] = increasing roots of j_l below xmax, l = 0, 1, ...
+
+    Roots of j_{l+1} interlace those of j_l (lambda_{l,k} < lambda_{l+1,k} <
+    lambda_{l,k+1}), so each is bracketed by two consecutive roots of j_l."""
+    kmax0 = int(xmax / math.pi) + 2
+    prev = [math.pi * k for k in range(1, kmax0 + 1)]  # one beyond xmax
+    out = [[r for r in prev if r < xmax]]
+    l = 0
+    while True:
+        f = lambda x, n=l + 1: special.spherical_jn(n, x)
+        cur = []
+        for a, b in zip(prev[:-1], prev[1:]):
+            cur.append(optimize.brentq(f, a, b, xtol=1e-15, rtol=8.9e-16, maxiter=300))
+            if cur[-1] >= xmax:
+                break
+        # keep one root beyond xmax (bracket for the next l)
+        if not cur or cur[0] >= xmax:
+            break
+        if cur[-1] < xmax:
+            # need one more root beyond the last bracket
+            a = prev[-1]
+            b = a + math.pi
+            while f(a) * f(b) > 0:
+                b += math.pi / 2
+            cur.append(optimize.brentq(f, a, b, xtol=1e-15, rtol=8.9e-16, maxiter=300))
+        out.append([r for r in cur if r < xmax])
+        prev = cur
+        l += 1
+    return out
+
+
+def ball_spectrum(bandlimit: int):
+    """Basis enumeration with the reference's order and truncation
+    (``FLEBasis3D.lap_eig_ball``, FLEBasis3D.py:700-799): entries (k, l, m) with
+    m in 0,-1,1,-2,2,..., stable-sorted by lambda_lk, cut at
+    searchsorted(lambda, bandlimit*pi/2) - 1 and then at a trailing negative m."""
+    threshold = bandlimit * np.pi / 2
+    roots = _sph_jn_roots_interlaced(threshold + 1.0)
+    groups = []  # (k, l, lambda) in the reference's insertion order (k outer, l inner)
+    kmax = max(len(r) for r in roots)
+    for k in range(1, kmax + 1):
+        for l, rl in enumerate(roots):
+            if len(rl) >= k:
+                groups.append((k, l, rl[k - 1]))
+    ls, ks, ms, lmds = [], [], [], []
+    for k, l, lam in groups:
+        for j in range(2 * l + 1):
+            ls.append(l)
+            ks.append(k)
+            ms.append(((-1) ** j) * ((j + 1) // 2))
+            lmds.append(lam)
+    ls, ks, ms, lmds = (np.asarray(x) for x in (ls, ks, ms, lmds))
+    order = np.argsort(lmds, kind="stable")
+    ls, ks, ms, lmds = ls[order], ks[order], ms[order], lmds[order]
+    ne = int(np.searchsorted(lmds, threshold, side="left")) - 1
+    if ms[ne - 1] < 0:
+        ne -= 1
+    ne = max(ne, 1)
+    ls, ks, ms, lmds = ls[:ne], ks[:ne], ms[:ne], lmds[:ne]
+    mds = 2 * np.abs(ms) - (ms < 0)
+    cs = np.sqrt(2.0) / np.abs(special.spherical_jn(ls + 1, lmds))
+    return ls.astype(np.int64), ks.astype(np.int64), ms.astype(np.int64), mds.astype(np.int64), \
+        lmds.astype(np.float64), cs.astype(np.float64), ne
+
+
+# ---------------------------------------------------------------------------
+# geometry: fundamental voxels of D4h and their orbits
+# ---------------------------------------------------------------------------
+
+def _orbit(v):
+    """The 16 images R^p My^t Mz^q v, ordered p + 4 q + 8 t."""
+    out = []
+    for t in (0, 1):
+        for q in (0, 1):
+            for p in range(4):
+                x, y, z = v
+                if q:
+                    z = -z
+                if t:
+                    y = -y
+                for _ in range(p):
+                    x, y = -y, x
+                out.append((x, y, z))
+    return out
+
+
+def fundamental_voxels(N: int, radius: float):
+    """Fundamental voxels of the ball |v| <= radius (1 + 1e-13), v = index - N/2.
+
+    Returns coords (V0, 3) int, orbit flat indices (V0, 16) int32 (axis 0 = x),
+    weights 1/|stabiliser| (V0,)."""
+    R = int(math.floor(radius * (1 + 1e-13)))
+    if N // 2 < R:
+        raise ValueError(f"radius {radius} does not fit a box of {N} with symmetric coordinates")
+    coords, orbits, wts = [], [], []
+    lim = radius * (1 + 1e-13)
+    for x in range(0, R + 1):
+        for y in range(0, x + 1):
+            for z in range(0, R + 1):
+                if math.sqrt(x * x + y * y + z * z) > lim:
+                    continue
+                imgs = _orbit((x, y, z))
+                stab = sum(1 for g in imgs if g == (x, y, z))
+                coords.append((x, y, z))
+                orbits.append([((gx + N // 2) * N + (gy + N // 2)) * N + (gz + N // 2)
+                               for gx, gy, gz in imgs])
+                wts.append(1.0 / stab)
+    return np.asarray(coords, dtype=np.int64), np.asarray(orbits, dtype=np.int32), np.asarray(wts)
+
+
+def _legendre_orthonormal(lmax: int, x: np.ndarray):
+    """P[l][m] (m <= l): orthonormal associated Legendre with Condon-Shortley phase,
+    so that Y_lm(theta, phi) = P[l][m](cos theta) e^{i m phi}."""
+    x = np.asarray(x, dtype=np.float64)
+    s = np.sqrt(np.maximum(0.0, 1.0 - x * x))
+    P = [[None] * (l + 1) for l in range(lmax + 1)]
+    P[0][0] = np.full_like(x, 1.0 / math.sqrt(4 * math.pi))
+    for m in range(1, lmax + 1):
+        P[m][m] = -math.sqrt((2 * m + 1) / (2.0 * m)) * s * P[m - 1][m - 1]
+    for m in range(0, lmax):
+        P[m + 1][m] = math.sqrt(2 * m + 3) * x * P[m][m]
+    for m in range(0, lmax + 1):
+        for l in range(m + 2, lmax + 1):
+            a = math.sqrt((4.0 * l * l - 1) / (l * l - m * m))
+            b = math.sqrt(((l - 1.0) ** 2 - m * m) / (4.0 * (l - 1) ** 2 - 1))
+            P[l][m] = a * (x * P[l - 1][m] - b * P[l - 2][m])
+    return P
+
+
+def projection_tables(N: int, radius: float, spectrum, lmax: int, BK: int = 16):
+    """Host FP64 -> FP32 tables of the symmetry-adapted projection GEMM.
+
+    Returns dict with V0, Kpad, Ntot, class_cols, class_colpad, orbits (V0, 16)
+    int32, W (Kpad, Ntot) float32, chan_plus/chan_minus/chan_m (Ntot,) int32."""
+    ls, ks, ms, lmds, cs = spectrum
+    coords, orbits, w = fundamental_voxels(N, radius)
+    V0 = len(coords)
+    x, y, z = (coords[:, i].astype(np.float64) for i in range(3))
+    r = np.sqrt(x * x + y * y + z * z)
+    x0 = np.where(r > 0, z / np.where(r > 0, r, 1), 1.0)
+    phi0 = np.arctan2(y, x)
+    P = _legendre_orthonormal(lmax, x0)
+    index = {(int(l), int(k), int(m)): i for i, (l, k, m) in enumerate(zip(ls, ks, ms))}
+    classes = [[] for _ in range(N_CLASSES)]
+    for (l, k, m), i in index.items():
+        if m < 0:
+            continue
+        cls = ((l + m) % 2) * 4 + (m % 4)
+        classes[cls].append((l, k, m, i, index.get((l, k, -m), -1)))
+    for c in classes:
+        c.sort()
+    K = 2 * V0
+    Kpad = ((K + BK - 1) // BK) * BK
+    cols = [len(c) for c in classes]
+    colpad = [((n + 63) // 64) * 64 for n in cols]
+    Ntot = int(sum(colpad))
+    Wb = np.zeros((Kpad, Ntot), dtype=np.float32)
+    chan_plus = np.full(Ntot, -1, dtype=np.int32)
+    chan_minus = np.full(Ntot, -1, dtype=np.int32)
+    chan_m = np.zeros(Ntot, dtype=np.int32)
+    h = radius ** -1.5
+    jl_cache = {}
+    off = 0
+    for cls in range(N_CLASSES):
+        ch = classes[cls]
+        n = len(ch)
+        if n:
+            for (l, k, m, ip, im) in ch:
+                if (l, k) not in jl_cache:
+                    jl_cache[(l, k)] = special.spherical_jn(l, lmds[ip] * r / radius)
+            J = np.stack([jl_cache[(l, k)] for (l, k, m, ip, im) in ch], axis=1)
+            Pm = np.stack([P[l][m] for (l, k, m, ip, im) in ch], axis=1)
+            mm = np.array([m for (l, k, m, ip, im) in ch], dtype=np.float64)
+            cc = np.array([cs[ip] for (l, k, m, ip, im) in ch])
+            rad = (h * w)[:, None] * J * Pm * cc[None, :]
+            ang = phi0[:, None] * mm[None, :]
+            Wb[0:K:2, off:off + n] = rad * np.cos(ang)
+            Wb[1:K:2, off:off + n] = rad * np.sin(ang)
+            for j, (l, k, m, ip, im) in enumerate(ch):
+                chan_plus[off + j] = ip
+                chan_minus[off + j] = im if m > 0 else -1
+                chan_m[off + j] = m
+        off += colpad[cls]
+    return dict(V0=V0, Kpad=Kpad, Ntot=Ntot, class_cols=cols, class_colpad=colpad,
+                orbits=np.ascontiguousarray(orbits), W=Wb, chan_plus=chan_plus,
+                chan_minus=chan_minus, chan_m=chan_m)
+
+
+class FLEBasis3D:
+    """Same constructor as the reference (FLEBasis3D.py:39-109)."""
+
+    BK = 16  # K padding of the GEMM
+
+    def __init__(self, N, bandlimit, eps, maxitr=None, maxfun=None, max_l=None, mode="complex",
+                 sph_harm_solver="nvidia_torch", reduce_memory=True, batchsize=1, radius=1,
+                 device="cuda", dtype=torch.complex128, jl_zeros_path=None, cs_path=None,
+                 precision_mode="accurate"):
+        if mode != "complex":
+            raise ValueError("only mode='complex' is supported")
+        if maxfun or max_l:
+            raise ValueError("maxfun/max_l truncations are not supported")
+        self.batchsize = int(batchsize)
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError("matcha_amd runs on a GPU device only")
+        self.complexmode = True
+        self.precision_mode = precision_mode
+        self.reduce_memory = reduce_memory
+        self.eps = eps
+        self.N1 = int(N)
+        self.N = self.N1 + (self.N1 % 2)
+        self.radius = float(radius)
+        self.dtype = torch.complex64
+        self.h = self.radius ** -1.5
+        self.R = self.N // 2
+        ls, ks, ms, mds, lmds, cs, ne = ball_spectrum(bandlimit)
+        self.ls, self.ks, self.ms, self.mds, self.lmds, self.cs, self.ne = ls, ks, ms, mds, lmds, cs, ne
+        self.lmax = int(ls.max())
+        self.kmax = int(ks.max())
+        self.mmax = int(np.abs(ms).max())
+        self.idlm_list = [[[] for _ in range(2 * self.lmax + 1)] for _ in range(self.lmax + 1)]
+        for i in range(ne):
+            self.idlm_list[ls[i]][mds[i]].append(i)
+        self._build_tables()
+        self._plan = None
+        self._make_plan()
+
+    # -- tables ---------------------------------------------------------------
+    def _build_tables(self):
+        if self.N != self.N1:
+            raise ValueError("odd box sizes are not supported")
+        t = projection_tables(self.N, self.radius, (self.ls, self.ks, self.ms, self.lmds, self.cs),
+                              self.lmax, self.BK)
+        self.V0, self.Kpad, self.Ntot = t["V0"], t["Kpad"], t["Ntot"]
+        self.class_cols, self.class_colpad = t["class_cols"], t["class_colpad"]
+        self._orbits, self._basis = t["orbits"], t["W"]
+        self._chan_plus, self._chan_minus, self._chan_m = t["chan_plus"], t["chan_minus"], t["chan_m"]
+
+    def _make_plan(self):
+        handle = ctypes.c_void_p()
+        cc = (ctypes.c_int * N_CLASSES)(*self.class_colpad)
+        with torch.cuda.device(self.device):
+            rc = _lib.lib().mch_fle_plan_create(
+                self.N, self.V0, self.Kpad, self.Ntot, cc,
+                self._orbits.ctypes.data_as(ctypes.c_void_p),
+                self._basis.ctypes.data_as(ctypes.c_void_p),
+                self._chan_plus.ctypes.data_as(ctypes.c_void_p),
+                self._chan_minus.ctypes.data_as(ctypes.c_void_p),
+                self._chan_m.ctypes.data_as(ctypes.c_void_p),
+                self.ne, self.batchsize, ctypes.byref(handle))
+        _lib.check(rc, "mch_fle_plan_create")
+        self._plan = handle
+        self._basis = None  # the device copy is the one used
+
+    def __del__(self):
+        h = getattr(self, "_plan", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().mch_fle_plan_destroy(h)
+            except Exception:
+                pass
+
+    # -- transform ---------------------------------------------------------------
+    def evaluate_t(self, f: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """a = B^H f for a batch (Bt, N, N, N) (or one (N, N, N) volume) of real volumes.
+
+        Returns (Bt, ne) complex64.  Unlike the reference, the caller's tensor is
+        not modified (the reference zeroes its out-of-ball voxels in place)."""
+        if f.ndim == 3:
+            f = f.unsqueeze(0)
+        _lib.require_gpu_tensor(f, "volumes", device=self.device)
+        if f.dtype != torch.float32:
+            f = f.float()
+        if tuple(f.shape[1:]) != (self.N1, self.N1, self.N1):
+            raise ValueError(f"volumes must be (B, {self.N1}, {self.N1}, {self.N1}), got {tuple(f.shape)}")
+        Bt = f.shape[0]
+        if Bt > self.batchsize:
+            raise ValueError(f"batch {Bt} exceeds the configured batchsize {self.batchsize}")
+        f = f.contiguous()
+        if out is None:
+            out = torch.empty((Bt, self.ne), dtype=torch.complex64, device=self.device)
+        rc = _lib.lib().mch_fle_project(self._plan, f.data_ptr(), self.N1 ** 3, Bt, out.data_ptr(),
+                                        out.stride(0), _lib.stream_handle(self.device))
+        _lib.check(rc, "mch_fle_project")
+        return out
+
+    @property
+    def flops_per_volume(self) -> float:
+        """Algorithmic FLOPs of the projection GEMM per volume."""
+        return 2.0 * 2 * self.Kpad * self.Ntot

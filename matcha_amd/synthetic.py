@@ -1,0 +1,121 @@
+"""Synthetic workloads for the benchmark and tests (GPU, torch).
+
+Template: sum of isotropic Gaussian blobs, centres uniform in a ball of
+radius 0.35*N, sigma and amplitude uniform.  Particles: the same blobs with
+centres rotated by a Haar-uniform rotation R and moved by an integer shift t,
+i.e. V(x) = T(R^-1 (x - t)) evaluated exactly (no interpolation), plus white
+Gaussian noise at a given SNR with the signal power taken over a spherical
+mask (as ``add_awgn`` of the reference harness, ``src/matcha/example.py:93-124``).
+
+Coordinates are v = index - N/2 with axis 0 = x, 1 = y, 2 = z; rotations
+act on (x, y, z).  The orientation search returns (alpha, beta, gamma) with
+R = Rz(alpha) Ry(beta) Rz(gamma) for such a particle.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+
+def blob_params(rng: np.random.Generator, N: int, n_blobs: int, sigma_range, amp_range=(0.5, 1.0),
+                ball_frac=0.35):
+    R = ball_frac * N
+    cs = []
+    while len(cs) < n_blobs:
+        c = rng.uniform(-R, R, 3)
+        if np.linalg.norm(c) <= R:
+            cs.append(c)
+    sig = rng.uniform(*sigma_range, n_blobs)
+    amp = rng.uniform(*amp_range, n_blobs)
+    return np.asarray(cs), sig, amp
+
+
+def haar_rotations(rng: np.random.Generator, n: int) -> np.ndarray:
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], -2)
+    return R
+
+
+def euler_zyz_to_matrix(alpha, beta, gamma) -> np.ndarray:
+    """Rz(alpha) Ry(beta) Rz(gamma) acting on (x, y, z) column vectors."""
+    a, b, g = (np.asarray(t, dtype=np.float64) for t in (alpha, beta, gamma))
+
+    def rz(t):
+        c, s = np.cos(t), np.sin(t)
+        o, z = np.ones_like(t), np.zeros_like(t)
+        return np.stack([np.stack([c, -s, z], -1), np.stack([s, c, z], -1), np.stack([z, z, o], -1)], -2)
+
+    def ry(t):
+        c, s = np.cos(t), np.sin(t)
+        o, z = np.ones_like(t), np.zeros_like(t)
+        return np.stack([np.stack([c, z, s], -1), np.stack([z, o, z], -1), np.stack([-s, z, c], -1)], -2)
+
+    return rz(a) @ ry(b) @ rz(g)
+
+
+def geodesic_deg(R1: np.ndarray, R2: np.ndarray) -> np.ndarray:
+    M = np.einsum('...ji,...jk->...ik', R1, R2)
+    tr = np.trace(M, axis1=-2, axis2=-1)
+    return np.degrees(np.arccos(np.clip((tr - 1) / 2, -1.0, 1.0)))
+
+
+def render_blobs(N: int, centres: torch.Tensor, sig: torch.Tensor, amp: torch.Tensor,
+                 device) -> torch.Tensor:
+    """Volumes (P, N, N, N) float32 from per-volume blob centres (P, nb, 3)."""
+    ax = torch.arange(N, device=device, dtype=torch.float32) - N // 2
+    P, nb, _ = centres.shape
+    out = torch.zeros((P, N, N, N), dtype=torch.float32, device=device)
+    for j in range(nb):
+        c = centres[:, j]  # (P, 3)
+        inv = (-0.5 / (sig[j] * sig[j])).item()
+        gx = torch.exp(inv * (ax[None, :] - c[:, 0:1]) ** 2)  # (P, N)
+        gy = torch.exp(inv * (ax[None, :] - c[:, 1:2]) ** 2)
+        gz = torch.exp(inv * (ax[None, :] - c[:, 2:3]) ** 2)
+        out += amp[j].item() * gx[:, :, None, None] * gy[:, None, :, None] * gz[:, None, None, :]
+    return out
+
+
+def spherical_mask(N: int, radius: float, device) -> torch.Tensor:
+    ax = torch.arange(N, device=device, dtype=torch.float32) - N // 2
+    X, Y, Z = torch.meshgrid(ax, ax, ax, indexing="ij")
+    return (torch.sqrt(X * X + Y * Y + Z * Z) <= radius)
+
+
+class Workload:
+    """Template + particle generator for one box size."""
+
+    def __init__(self, N: int, n_blobs: int, sigma_range, seed: int, device, mask_radius: float):
+        self.N = N
+        self.device = torch.device(device)
+        self.rng = np.random.default_rng(seed)
+        c, s, a = blob_params(self.rng, N, n_blobs, sigma_range)
+        self.centres = c
+        self.sig = torch.tensor(s, dtype=torch.float32)
+        self.amp = torch.tensor(a, dtype=torch.float32)
+        self.mask = spherical_mask(N, mask_radius, self.device)
+        self.template = render_blobs(N, torch.tensor(c[None], dtype=torch.float32, device=self.device),
+                                     self.sig, self.amp, self.device)[0]
+        self.gen = torch.Generator(device=self.device)
+        self.gen.manual_seed(seed + 1)
+
+    def particles(self, P: int, max_shift: int, snr_db: float):
+        """(vols (P,N,N,N) float32 GPU, R (P,3,3), shifts (P,3) int)."""
+        R = haar_rotations(self.rng, P)
+        t = self.rng.integers(-max_shift, max_shift + 1, (P, 3))
+        cen = np.einsum('pij,bj->pbi', R, self.centres) + t[:, None, :]
+        vols = render_blobs(self.N, torch.tensor(cen, dtype=torch.float32, device=self.device),
+                            self.sig, self.amp, self.device)
+        if snr_db is not None and math.isfinite(snr_db):
+            m = self.mask
+            p = (vols[:, m] ** 2).mean(dim=1)
+            std = torch.sqrt(p / (10.0 ** (snr_db / 10.0)))
+            noise = torch.randn(vols.shape, device=self.device, generator=self.gen)
+            vols = vols + noise * std[:, None, None, None]
+        return vols.contiguous(), R, t

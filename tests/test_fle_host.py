@@ -1,0 +1,121 @@
+"""Host-side tables of the symmetry-adapted projection, checked on the CPU.
+
+The GPU kernels' arithmetic is emulated in numpy with the product's own FP32
+tables and compared with the FP64 brute-force oracle."""
+import numpy as np
+import pytest
+
+from matcha_amd.core import FLEBasis3D as P
+from oracle import fle
+
+
+@pytest.mark.parametrize("bl", [8, 16, 32, 33, 48])
+def test_spectrum_matches_oracle(bl):
+    a = P.ball_spectrum(bl)
+    b = fle.lap_eig_ball(0, bl)
+    for x, y in zip(a[:4], b[:4]):
+        assert np.array_equal(x, y)
+    assert a[6] == b[6]
+    assert np.max(np.abs(a[4] - b[4])) < 1e-12
+    assert np.max(np.abs(a[5] / b[5] - 1)) < 1e-12
+
+
+def test_reference_sizes():
+    # (ne, lmax, kmax) of the survey's table for the benchmark workloads
+    for bl, want in [(16, (1006, 19, 7)), (32, (8252, 42, 15)), (33, (9267, 44, 16)),
+                     (48, (28983, 67, 23))]:
+        ls, ks, ms, mds, lmds, cs, ne = P.ball_spectrum(bl)
+        assert (ne, ls.max(), ks.max()) == want
+
+
+def test_roots_pinned_by_mpmath():
+    mp = pytest.importorskip("mpmath")
+    ls, ks, ms, mds, lmds, cs, ne = P.ball_spectrum(16)
+    seen = set()
+    for l, k, lam in zip(ls, ks, lmds):
+        if (l, k) in seen or l % 5:
+            continue
+        seen.add((l, k))
+        ref = float(mp.besseljzero(mp.mpf(int(l)) + mp.mpf(0.5), int(k)))
+        assert abs(lam - ref) < 1e-11 * ref
+
+
+def test_fundamental_voxels_partition_the_ball():
+    N, radius = 32, 13
+    coords, orbits, w = P.fundamental_voxels(N, radius)
+    x, y, z, flat = fle.ball_voxels(N, radius)
+    # each ball voxel appears in exactly one orbit; weights count multiplicity
+    counts = {}
+    for orb, wt in zip(orbits, w):
+        for i in orb:
+            counts[int(i)] = counts.get(int(i), 0) + wt
+    assert set(counts) == set(int(i) for i in flat)
+    assert all(abs(c - 1.0) < 1e-12 for c in counts.values())
+
+
+def emulate_projection(t, vols, ne):
+    B = vols.shape[0]
+    f = vols.reshape(B, -1)
+    g = f[:, t["orbits"]]
+    V0, Kpad, Ntot = t["V0"], t["Kpad"], t["Ntot"]
+    X = np.zeros((8, 2 * B, Kpad))
+    for eps in range(2):
+        A = np.zeros((2, 4, B, V0))
+        Bi = np.zeros((2, 4, B, V0))
+        for tt in range(2):
+            s0 = s2 = s1r = s1i = 0
+            for q in range(2):
+                sg = -1 if (eps and q) else 1
+                h = g[:, :, 4 * q + 8 * tt:4 * q + 8 * tt + 4]
+                s0 = s0 + sg * h.sum(-1)
+                s2 = s2 + sg * (h[..., 0] - h[..., 1] + h[..., 2] - h[..., 3])
+                s1r = s1r + sg * (h[..., 0] - h[..., 2])
+                s1i = s1i + sg * (h[..., 3] - h[..., 1])
+            A[tt, 0], A[tt, 1], A[tt, 2], A[tt, 3] = s0, s1r, s2, s1r
+            Bi[tt, 1], Bi[tt, 3] = s1i, -s1i
+        for rho in range(4):
+            c = eps * 4 + rho
+            X[c, :B, 0:2 * V0:2] = A[0, rho] + A[1, rho]
+            X[c, :B, 1:2 * V0:2] = Bi[0, rho] - Bi[1, rho]
+            X[c, B:, 0:2 * V0:2] = Bi[0, rho] + Bi[1, rho]
+            X[c, B:, 1:2 * V0:2] = A[1, rho] - A[0, rho]
+    W = t["W"].astype(np.float64)
+    out = np.zeros((2 * B, Ntot))
+    off = 0
+    for c in range(8):
+        n = t["class_colpad"][c]
+        out[:, off:off + n] = X[c] @ W[:, off:off + n]
+        off += n
+    a = np.zeros((B, ne), dtype=complex)
+    for col in range(Ntot):
+        ip = t["chan_plus"][col]
+        if ip < 0:
+            continue
+        a[:, ip] = out[:B, col] + 1j * out[B:, col]
+        im = t["chan_minus"][col]
+        if im >= 0:
+            sg = -1 if t["chan_m"][col] % 2 else 1
+            a[:, im] = sg * (out[:B, col] - 1j * out[B:, col])
+    return a
+
+
+@pytest.mark.parametrize("N,bl,radius", [(24, 8, 11), (20, 10, 9.5)])
+def test_symmetric_gemm_emulation_matches_brute_force(N, bl, radius):
+    sp = P.ball_spectrum(bl)
+    ls, ks, ms, mds, lmds, cs, ne = sp
+    t = P.projection_tables(N, radius, (ls, ks, ms, lmds, cs), int(ls.max()))
+    rng = np.random.default_rng(N)
+    vols = rng.standard_normal((3, N, N, N))
+    ref = fle.evaluate_t(vols, N, bl, radius)
+    got = emulate_projection(t, vols, ne)
+    assert np.max(np.abs(got - ref)) < 1e-6 * np.max(np.abs(ref))
+
+
+def test_basis_is_discretely_orthonormal():
+    # h^2 sum_v psi_i conj(psi_j) ~ delta_ij on a fine grid (pins c_lk, h, Y normalisation)
+    N, bl, radius = 40, 6, 19
+    B, flat = fle.basis_matrix(N, bl, radius)
+    G = B.conj().T @ B
+    assert np.max(np.abs(np.diag(G) - 1)) < 0.05
+    off = G - np.diag(np.diag(G))
+    assert np.max(np.abs(off)) < 0.05

@@ -1,0 +1,109 @@
+"""GPU parity of the ball-harmonic projection and sigma (HIP C-ABI) against the FP64 oracle.
+
+Tolerance: FP32 path, max |a_gpu - a_ref| <= 2e-5 * max |a_ref| (coefficients)
+and 1e-4 relative for sigma and norms.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fle, so3, wigner as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _fle(dev, N, bl, radius, B):
+    from matcha_amd.core.FLEBasis3D import FLEBasis3D
+    return FLEBasis3D(N, bl, 1e-4, batchsize=B, radius=radius, device=dev)
+
+
+@pytest.mark.parametrize("N,bl,radius,B", [(24, 8, 11, 3), (20, 10, 9.5, 1), (32, 12, 15, 5)])
+def test_projection_small_exact(gpu_device, N, bl, radius, B):
+    rng = np.random.default_rng(N + bl)
+    vols = rng.standard_normal((B, N, N, N)).astype(np.float32)
+    f = _fle(gpu_device, N, bl, radius, B)
+    got = f.evaluate_t(torch.tensor(vols, device=gpu_device)).cpu().numpy()
+    ref = fle.evaluate_t(vols.astype(np.float64), N, bl, radius)
+    assert got.shape == ref.shape
+    assert np.max(np.abs(got - ref)) < 2e-5 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("N,bl,radius", [(64, 16, 28), (96, 32, 45)])
+def test_projection_benchmark_sizes_subset(gpu_device, N, bl, radius):
+    rng = np.random.default_rng(N)
+    vol = fle.blob_volume(rng, N, 30, sigma_range=(1.0, 5.0))
+    vol = vol + 0.3 * rng.standard_normal(vol.shape)
+    vols = vol[None].astype(np.float32)
+    f = _fle(gpu_device, N, bl, radius, 2)
+    got = f.evaluate_t(torch.tensor(vols, device=gpu_device)).cpu().numpy()[0]
+    ls, ms = f.ls, f.ms
+    # a spread of coefficients incl. the highest l and |m|
+    pick = sorted(set([0, 1, 2, f.ne - 1, f.ne - 2, int(np.argmax(ls)), int(np.argmax(np.abs(ms)))]
+                      + list(rng.choice(f.ne, 25, replace=False))))
+    ref = fle.evaluate_t_subset(vols.astype(np.float64), N, bl, radius, pick)[0]
+    scale = np.linalg.norm(got)
+    assert np.max(np.abs(got[pick] - ref)) < 2e-5 * scale
+
+
+def test_projection_rotation_covariance_full_size(gpu_device):
+    # a 90-degree grid rotation about y acts on (a_{lk,m})_m by the Wigner matrix D^l
+    N, bl, radius = 96, 32, 45
+    rng = np.random.default_rng(7)
+    vol = fle.blob_volume(rng, N, 40, sigma_range=(1.0, 5.0))
+    rot = fle.rotate_y90(vol)
+    f = _fle(gpu_device, N, bl, radius, 2)
+    a = f.evaluate_t(torch.tensor(np.stack([vol, rot]).astype(np.float32), device=gpu_device)).cpu().numpy()
+    a0, a1 = a[0].astype(np.complex128), a[1].astype(np.complex128)
+    ls, ks, ms = f.ls, f.ks, f.ms
+    lmax = int(ls.max())
+    # a1[l,k,:] = d^l(pi/2) a0[l,k,:] (established on the CPU with the brute-force oracle)
+    err = 0.0
+    for l in range(0, lmax + 1, 3):
+        d = W.wigner_d(np.array([np.pi / 2]), l)[0][W.dsize(l - 1):W.dsize(l)].reshape(2 * l + 1, 2 * l + 1)
+        for k in range(1, ks.max() + 1):
+            idx = {int(m): i for i, (ll, kk, m) in enumerate(zip(ls, ks, ms)) if ll == l and kk == k}
+            if len(idx) != 2 * l + 1:
+                continue
+            v0 = np.array([a0[idx[m]] for m in range(-l, l + 1)])
+            v1 = np.array([a1[idx[m]] for m in range(-l, l + 1)])
+            err = max(err, np.max(np.abs(d @ v0 - v1)) / np.linalg.norm(a0))
+    assert err < 2e-5
+
+
+def test_sigma_kernel_matches_oracle(gpu_device):
+    from matcha_amd.core.CrossCorrelationMatcher import CrossCorrelationMatcher
+    N, bl, radius, B = 24, 8, 11, 4
+    ccm = CrossCorrelationMatcher(N, gpu_device, batchsize=B, bandlimit=bl, radius=radius)
+    rng = np.random.default_rng(3)
+    ne = ccm.fle.ne
+    aT = (rng.standard_normal(ne) + 1j * rng.standard_normal(ne)).astype(np.complex64)
+    aV = (rng.standard_normal((B, ne)) + 1j * rng.standard_normal((B, ne))).astype(np.complex64)
+    ccm._aT = torch.tensor(aT, device=gpu_device)
+    ccm.template_coefficients_alphas_torch = torch.nn.functional.pad(ccm._aT, (0, 1))
+    ccm.compute_sigma(torch.tensor(aV, device=gpu_device), 0, B)
+    got = ccm.sigma.cpu().numpy()
+    ref = fle.compute_sigma(aT.astype(np.complex128), aV.astype(np.complex128), ccm.fle.ls,
+                            ccm.fle.mds, ccm.lmax)
+    assert np.max(np.abs(got - ref)) < 1e-5 * np.max(np.abs(ref))
+    assert np.allclose(ccm.bh_norms.cpu().numpy(), np.linalg.norm(aV, axis=1), rtol=1e-5)
+
+
+def test_get_sigma_end_to_end_matches_oracle(gpu_device):
+    from matcha_amd.core.CrossCorrelationMatcher import CrossCorrelationMatcher
+    N, bl, radius, B = 24, 8, 11, 3
+    rng = np.random.default_rng(11)
+    tmpl = fle.blob_volume(rng, N, 8, sigma_range=(1.0, 2.5))
+    vols = np.stack([fle.rotate_y90(tmpl), tmpl, tmpl + 0.5 * rng.standard_normal(tmpl.shape)])
+    ccm = CrossCorrelationMatcher(N, gpu_device, batchsize=4, bandlimit=bl, radius=radius)
+    ccm.set_template(tmpl.astype(np.float32))
+    sig, norms = ccm.get_sigma(torch.tensor(vols.astype(np.float32), device=gpu_device))
+    sig = sig.cpu().numpy()
+    aT = fle.evaluate_t(tmpl[None], N, bl, radius)[0]
+    aV = fle.evaluate_t(vols, N, bl, radius)
+    ref = fle.compute_sigma(aT, aV, ccm.fle.ls, ccm.fle.mds, ccm.lmax)
+    assert np.max(np.abs(sig[:B] - ref)) < 1e-4 * np.max(np.abs(ref))
+    assert np.all(sig[B:] == 0)
+    assert abs(float(ccm.bh_norm_template) - np.linalg.norm(aT)) < 1e-4 * np.linalg.norm(aT)
+    # the unrotated copy correlates best at the identity rotation
+    s_id = so3.score(np.array([0.0]), np.array([0.0]), np.array([0.0]), ref[1], ccm.lmax)[0]
+    assert s_id == pytest.approx(np.linalg.norm(aT) ** 2, rel=1e-6)

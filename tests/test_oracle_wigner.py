@@ -68,3 +68,23 @@ def test_group_law_under_composition():
         o, n = W.dsize(l - 1), 2 * l + 1
         M = lambda X: X[o:o + n * n].reshape(n, n)
         assert np.max(np.abs(M(Da) @ M(Db) @ M(Dg) - M(D))) < 1e-12
+
+
+def test_projection_rotation_covariance_oracle():
+    # pins the oracle's projection conventions: a 90-degree grid rotation about y
+    # maps a_{lk,.} to d^l(pi/2) a_{lk,.}
+    from oracle import fle
+    N, bl, radius = 24, 10, 11
+    rng = np.random.default_rng(0)
+    vol = rng.standard_normal((N, N, N))
+    a = fle.evaluate_t(np.stack([vol, fle.rotate_y90(vol)]), N, bl, radius)
+    ls, ks, ms, mds, lmds, cs, ne = fle.lap_eig_ball(N, bl)
+    for l in range(int(ls.max()) + 1):
+        d = W.wigner_d(np.array([np.pi / 2]), l)[0][W.dsize(l - 1):W.dsize(l)].reshape(2 * l + 1, 2 * l + 1)
+        for k in range(1, ks.max() + 1):
+            idx = {int(m): i for i, (ll, kk, m) in enumerate(zip(ls, ks, ms)) if ll == l and kk == k}
+            if len(idx) != 2 * l + 1:
+                continue
+            v0 = np.array([a[0, idx[m]] for m in range(-l, l + 1)])
+            v1 = np.array([a[1, idx[m]] for m in range(-l, l + 1)])
+            assert np.max(np.abs(d @ v0 - v1)) < 1e-12 * max(1.0, np.linalg.norm(v0))
