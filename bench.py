@@ -1,10 +1,12 @@
 """Benchmark: aligned volumes/s for Matcha's orientation-search hot path on one MI355X.
 
-A step = one batch of synthetic particles through the hot path on the GPU:
+A step = one batch of synthetic particles aligned on the GPU by two
+rotation/shift alternations (matcha_amd.align.Alternation), each:
   CrossCorrelationMatcher.get_sigma  (ball-harmonic projection + sigma)
   Matcha.search_orientations         (SOFFT grid + frequency-marched Newton)
-  ShiftMatcher shift search          (when available in this build)
-with every input already resident in HBM.  Prints one JSON line.
+  ShiftMatcher.search_shifts         (rotated-reference FFT cross-correlation)
+with the particles re-extracted at the running shift before the second one.
+Every input is already resident in HBM.  Prints one JSON line.
 
 Workload (box 96, L = 32): 96^3 template of 120 Gaussian blobs (sigma
 U(1,5) px, amplitude U(0.5,1), centres in a ball of 0.35*96), particles =
@@ -45,6 +47,7 @@ def parse():
     p.add_argument("--batch", type=int, default=None)
     p.add_argument("--snr", type=float, default=0.0)
     p.add_argument("--pool", type=int, default=4, help="distinct particle batches cycled")
+    p.add_argument("--alternations", type=int, default=2)
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-coeffs", type=int, default=100)
@@ -90,7 +93,9 @@ def main():
 
     from matcha_amd import _lib
     from matcha_amd.core.CrossCorrelationMatcher import CrossCorrelationMatcher
+    from matcha_amd.align import Alternation
     from matcha_amd.core.Matcha import Matcha
+    from matcha_amd.core.ShiftMatcher import ShiftMatcher
     from matcha_amd.synthetic import Workload, euler_zyz_to_matrix, geodesic_deg
 
     w = dict(WORKLOADS[args.workload])
@@ -105,14 +110,16 @@ def main():
     ccm.set_template(wl.template)
     matcher = Matcha(B, dev, w["L_max"], {"Li": w["Li"], "candidates": w["C"], "num_steps": w["steps"],
                                           "oversampling_factor_K": w["K"]})
+    sm = ShiftMatcher({"N": w["N"], "num_subtomograms_per_batch": B, "shift_upsample_factor": 1,
+                       "shift_search_radius": w["max_shift"], "voxel_size": 1.0}, dev)
+    sm.set_reference(wl.template.cpu().numpy())
+    pipe = Alternation(ccm, matcher, sm, num_alternations=args.alternations)
     pool = [wl.particles(B, w["max_shift"], args.snr) for _ in range(max(1, args.pool))]
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
 
     def step(i):
-        vols = pool[i % len(pool)][0]
-        sigma, norms = ccm.get_sigma(vols)
-        return matcher.search_orientations(sigma)
+        return pipe.run(pool[i % len(pool)][0])
 
     for i in range(args.warmup):
         step(i)
@@ -125,15 +132,32 @@ def main():
     vps = args.steps * B / elapsed
 
     # accuracy over the pool (untimed)
-    errs = []
+    errs, shift_ok = [], []
     for vols, R, t in pool:
-        sigma, _ = ccm.get_sigma(vols)
-        a, b, g, s = (x.cpu().numpy().astype(np.float64) for x in matcher.search_orientations(sigma))
+        a, b, g, ns, sh, ss = pipe.run(vols)
+        a, b, g = (x.cpu().numpy().astype(np.float64) for x in (a, b, g))
         errs.append(geodesic_deg(R, euler_zyz_to_matrix(a, b, g)))
+        shift_ok.append(np.all(sh.cpu().numpy() == t[:, ::-1], axis=1))
     errs = np.concatenate(errs)
+    shift_ok = np.concatenate(shift_ok)
+
+    # per-stage time of one alternation (HIP events on the compute stream)
+    stream = torch.cuda.current_stream(dev)
+    vols0 = pool[0][0]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(stream)
+    sigma, norms = ccm.get_sigma(vols0)
+    ev[1].record(stream)
+    a0, b0, g0, s0 = matcher.search_orientations(sigma)
+    ev[2].record(stream)
+    from matcha_amd.core.ShiftMatcher import euler_to_shift_rotations
+    sm.search_shifts(subtomos=vols0, rotation_tracker=euler_to_shift_rotations(a0, b0, g0))
+    ev[3].record(stream)
+    torch.cuda.synchronize()
+    stage_ms = {"get_sigma": ev[0].elapsed_time(ev[1]), "search_orientations": ev[1].elapsed_time(ev[2]),
+                "search_shifts": ev[2].elapsed_time(ev[3])}
 
     # dominant kernel: the projection GEMM, timed alone with HIP events on its stream
-    stream = torch.cuda.current_stream(dev)
     fle_ = ccm.fle
     ccm.fle.evaluate_t(pool[0][0])
     reps = 20
@@ -165,8 +189,10 @@ def main():
                          f"{w['bandlimit']} radius {w['radius']}, march {w['Li']}->{w['L_max']}, "
                          f"grid {2 * min(w['Li']) * w['K']}^3, {w['C']} candidates, {w['steps']} Newton "
                          f"steps, batch {B}, SNR {args.snr} dB, plain FP32 path"),
-            "path": "get_sigma + search_orientations",
+            "path": f"{args.alternations} x (get_sigma + search_orientations + search_shifts)",
+            "stage_ms_per_batch": stage_ms,
             "median_err_deg": float(np.median(errs)),
+            "shift_exact_frac": float(shift_ok.mean()),
             "p90_err_deg": float(np.percentile(errs, 90)),
             "particles_checked": int(errs.size),
             "setup_s": t_setup,

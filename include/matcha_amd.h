@@ -123,6 +123,45 @@ int mch_corr_plan_destroy(void* plan);
 int mch_compute_sigma(void* plan, const void* aT, const void* aV, long long ldv, int ne, int B,
                       void* sigma, long long lds, float* norms, hipStream_t stream);
 
+/* ------------------------------------------------------------ shift --- */
+
+/* Plan of the FFT shift match: the centred Fourier reference (complex64
+ * (N,N,N), host memory, as built by ShiftMatcher.set_reference), the cosine
+ * background mask radius (box_size / (2 voxel_size)), FFT tables and
+ * workspaces.  N must be a multiple of 4 of the form 2^a 3^b, <= 128.
+ * Replaces ShiftMatcher.set_reference + setup_utils mask tables
+ * (src/matcha/core/ShiftMatcher.py:805-822,442-472; src/matcha/utils/setup_utils.py:239-301). */
+int mch_shift_plan_create(int N, const void* ref_host, float box_radius, int max_batch,
+                          void** out_plan);
+int mch_shift_plan_destroy(void* plan);
+
+/* Integer-lag shift search (shift_upsample_factor 1): rotate the reference by
+ * rot (B, 3x3 row-major acting on (x, y, z) = array axes (2, 1, 0)), extract
+ * the patch at the prior shift_zyx (B, 3) (NULL = 0), cosine-masked
+ * background, roll, rFFT, cross power with the masked half spectrum (x ctf
+ * (B, N, N, N/2+1) if not NULL), inverse FFT with the half axis zero-padded,
+ * first argmax.  out_shift (B, 3) in the reference's flipped order, out_score
+ * (B,) normalised by the spectra's norms.  Replaces estimate_shift_like_relion
+ * + compute_shift_local (src/matcha/core/ShiftMatcher.py:94-282,491-674). */
+int mch_shift_search(void* plan, const float* subtomos, long long ld_vol, int B,
+                     const float* shift_zyx, const float* rot, const float* ctf, float* out_shift,
+                     float* out_score, hipStream_t stream);
+
+/* Shifted patches patch[a] = vol[a + flip(shift_zyx)] (trilinear for fractional
+ * shifts, zeros outside, border slice zeroed), volumes (B, N^3) float32; shift_zyx
+ * (B, 3) or NULL.  Replaces extract_shifted_patch_batch without normalisation
+ * (src/matcha/core/ShiftMatcher.py:558-616). */
+int mch_extract_patches(const float* vols, long long ld_vol, int B, int N, const float* shift_zyx,
+                        float* out, long long ld_out, hipStream_t stream);
+
+/* Shift-search rotations (B, 9) from search_orientations' Euler angles:
+ * rb = P R^T P with R = Rz(alpha) Ry(beta) Rz(gamma) on array axes (0,1,2) and P
+ * the axis reversal; the device-side equivalent of update_rotation_estimate's
+ * x<->z swap followed by search_shifts' conj + Euler round trip
+ * (src/matcha/utils/rotation_ops.py:23-34; src/matcha/core/ShiftMatcher.py:913). */
+int mch_euler_to_shift_rot(const float* alpha, const float* beta, const float* gamma, int B,
+                           float* rot, hipStream_t stream);
+
 /* Host FP64 Wigner d^l_{m'm}(beta), 0 <= l <= L, D layout (host memory).
  * Replaces spherical.Wigner(L).d (used by src/matcha/core/SOFFT.py:31-39). */
 int mch_wigner_d_host(double beta, int L, double* out);
