@@ -138,8 +138,10 @@ def test_alternation_recovers_rotation_and_shift(gpu_device):
     sm = ShiftMatcher({"N": N, "num_subtomograms_per_batch": B, "shift_upsample_factor": 1},
                       gpu_device)
     sm.set_reference(wl.template.cpu().numpy())
-    a, b, g, ns, shift, ss = Alternation(ccm, m, sm, num_alternations=2).run(vols)
+    # three alternations, the reference default (num_alternations: 3): at L = 16 the first
+    # orientation is too coarse for every shift, the second fixes the shifts
+    a, b, g, ns, shift, ss = Alternation(ccm, m, sm, num_alternations=3).run(vols)
     err = geodesic_deg(R, euler_zyz_to_matrix(*(x.cpu().numpy().astype(np.float64) for x in (a, b, g))))
-    assert np.median(err) < 1.0, err
+    assert np.max(err) < 1.0, err
     # particles were moved by +t along array axes (0,1,2); the shift comes back flipped
     assert np.array_equal(shift.cpu().numpy().astype(int), t[:, ::-1])
