@@ -6,14 +6,14 @@
 //   compute_shift (complex space, no phase normalisation, score normalised,
 //   upsample 1) :94-282, estimate_shift_like_relion :619-674.
 //
-// Per particle, with all buffers in HBM:
-//   rotref : R[kz,ky,kx] = trilinear(ref_F, rot * (x - c) + c), ifftshift(z,y),
-//            x mask_full (x ctf), ||R||^2 partials
+// Per batch (6 launches):
 //   bg     : background average of the shifted patch over the outer/cosine shell
-//   fwd_x  : shifted + cosine-masked + rolled patch rows -> FFT along x (keep 0..N/2)
-//   fwd_y  : FFT along y
-//   zpass  : FFT along z (1/N^3), ||S||^2 partials, P = conj(R) S, inverse FFT along z
-//   inv_y  : inverse FFT along y
+//   fwd_x  : shifted + cosine-masked + rolled patch rows, two real rows per
+//            complex FFT along x; keeps kx = 0..N/2, written as T[b][kx][z][y]
+//   yz     : one workgroup per (b, kx) plane held in LDS: FFT along y and z
+//            (1/N^3), ||S||^2, the rotated reference sampled on the fly
+//            (trilinear, ifftshift, mask_full, ctf), ||R||^2, P = conj(R) S,
+//            inverse FFT along z and y
 //   inv_x  : inverse FFT along x of the half axis zero-padded to N, real part,
 //            first argmax per tile of lines
 //   final  : first argmax per particle, wrap to (-N/2, N/2], flip to the
@@ -26,10 +26,11 @@
 
 namespace {
 
-constexpr int kFftThreads = 256;
-constexpr int kLines = 16;     // lines per workgroup
+constexpr int kRowThreads = 256;  // x-pass workgroups
+constexpr int kRows = 16;         // rows per x-pass workgroup
+constexpr int kPlaneThreads = 512;
 constexpr int kMaxN = 128;
-constexpr int kMaxIt = 4;      // items per thread per stage (kLines * N / 2 / 256)
+constexpr int kMaxIt = 4;         // FFT items per thread per stage and chunk
 constexpr int kMaxRad = 8;
 
 struct Radices {
@@ -41,16 +42,17 @@ struct ShiftPlan {
   int N, H, max_batch;
   float radius, radius_p, width, wsum;  // cosine mask geometry, sum of weights
   Radices rad;
-  float2* ref;      // (N, N, N) fftshifted reference spectrum
+  float4* refpair;  // (N, N, N): (ref[z][y][x], ref[z][y][x+1]) of the centred reference
   float2* twid;     // (N) e^{-2 pi i j / N}
-  float2* R;        // (B, N, N, H)
-  float2* T;        // (B, N, N, H)
-  float* rnorm_part;  // (B, N)
-  float* snorm_part;  // (B, N * ntile_h)
+  float2* T;        // (B, H, N, N) work spectrum
+  float* rnorm_part;  // (B, H)
+  float* snorm_part;  // (B, H)
   float* bg_part;     // (B, N)
+  float* bg_avg;      // (B)
   float* amax_val;    // (B, ntile_x)
   int* amax_idx;      // (B, ntile_x)
-  int ntile_h, ntile_x;
+  int ntile_x;
+  size_t plane_lds;
 };
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -61,38 +63,41 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 // (x + i y) * (i c)
 __device__ __forceinline__ float2 cmuli(float2 a, float c) { return make_float2(-c * a.y, c * a.x); }
 
-// In-place mixed-radix Stockham FFT of L lines of length N held in LDS
-// (buf[line * N + n]).  sign = -1 forward, +1 inverse (unscaled).
-__device__ void fft_lines(float2* buf, int L, int N, const Radices& rd, const float2* tw, float sign) {
+// In-place mixed-radix Stockham FFT of nl lines in LDS: element n of line l is
+// buf[l * ls + n * es].  sign = -1 forward, +1 inverse (unscaled).  Work items
+// run along the lines (lanes_on_lines, for ls == 1) or along the elements, so
+// that consecutive lanes touch consecutive LDS words.  Needs
+// nl * N / min(radix) <= NT * kMaxIt.
+template <int NT>
+__device__ void fft_lds(float2* buf, int nl, int ls, int es, int N, const Radices& rd,
+                        const float2* tw, float sign, bool lanes_on_lines) {
   int p = 1;
   for (int s = 0; s < rd.n; ++s) {
     const int R = rd.r[s];
     const int per = N / R;
-    const int items = L * per;
+    const int items = nl * per;
     float2 v[kMaxIt][4];
-    int info[kMaxIt];
 #pragma unroll
     for (int it = 0; it < kMaxIt; ++it) {
-      const int idx = threadIdx.x + it * kFftThreads;
-      info[it] = -1;
+      const int idx = threadIdx.x + it * NT;
       if (idx < items) {
-        const int line = idx / per, i = idx - line * per;
-        info[it] = idx;
-        const float2* src = buf + line * N + i;
+        const int line = lanes_on_lines ? idx % nl : idx / per;
+        const int i = lanes_on_lines ? idx / nl : idx % per;
+        const float2* src = buf + line * ls + i * es;
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (r < R) v[it][r] = src[r * per];
+          if (r < R) v[it][r] = src[r * per * es];
       }
     }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < kMaxIt; ++it) {
-      if (info[it] < 0) continue;
-      const int idx = info[it];
-      const int line = idx / per, i = idx - line * per;
+      const int idx = threadIdx.x + it * NT;
+      if (idx >= items) continue;
+      const int line = lanes_on_lines ? idx % nl : idx / per;
+      const int i = lanes_on_lines ? idx / nl : idx % per;
       const int k = i % p;
       const int step = N / (p * R);
-      // twiddles e^{sign 2 pi i r k / (p R)}
 #pragma unroll
       for (int r = 1; r < 4; ++r) {
         if (r < R) {
@@ -124,10 +129,10 @@ __device__ void fft_lines(float2* buf, int L, int N, const Radices& rd, const fl
         y[3] = csub(b, d);
       }
       const int j = (i / p) * p * R + k;
-      float2* dst = buf + line * N + j;
+      float2* dst = buf + line * ls + j * es;
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        if (r < R) dst[r * p] = y[r];
+        if (r < R) dst[r * p * es] = y[r];
     }
     __syncthreads();
     p *= R;
@@ -194,76 +199,23 @@ __device__ void make_shift_spec(const float* __restrict__ shift_zyx, int b, int 
   }
 }
 
+// cosine-masked value: inner r < radius keeps v, outer r > radius_p is bg,
+// the shell blends with 0.5 + 0.5 cos(pi (radius_p - r) / width)
+__device__ __forceinline__ float cosine_masked(float v, float bg, int d0, int d1, int d2, float radius,
+                                               float radius_p, float width) {
+  const float r2 = (float)(d0 * d0 + d1 * d1 + d2 * d2);
+  const float r = sqrtf(r2);
+  if (r < radius) return v;
+  if (r > radius_p) return bg;
+  const float rc = 0.5f + 0.5f * cosf(3.14159265358979323846f * (radius_p - r) / width);
+  return (1.f - rc) * v + rc * bg;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// rotated reference half spectrum
+// background of the cosine mask
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void shift_rotref_kernel(const float2* __restrict__ ref, int N,
-                                                           int H, const float* __restrict__ rot,
-                                                           const float* __restrict__ ctf,
-                                                           float2* __restrict__ Rout,
-                                                           float* __restrict__ rnorm_part) {
-  const int zq = blockIdx.x;  // output z' (after ifftshift)
-  const int b = blockIdx.y;
-  const float* m = rot + b * 9;
-  // sampling grid of the reference: d = h = N samples, w = N+1 samples,
-  // centre (de-1)/2 with de = N+1 (N even); normalised by (size_e - 2)
-  const float c = (float)N * 0.5f;
-  const float de = (float)(N + 1);
-  const float sc = 2.0f / (de - 2.0f);
-  const float um = (float)(N - 1);
-  const int zsrc = (zq + N / 2) % N;
-  float nrm = 0.f;
-  for (int t = threadIdx.x; t < N * H; t += blockDim.x) {
-    const int yq = t / H, kx = t - yq * H;
-    const int ysrc = (yq + N / 2) % N;
-    const int xsrc = kx + N / 2;
-    const float bx = (float)xsrc - c, by = (float)ysrc - c, bz = (float)zsrc - c;
-    const float rx = m[0] * bx + m[1] * by + m[2] * bz + c;
-    const float ry = m[3] * bx + m[4] * by + m[5] * bz + c;
-    const float rz = m[6] * bx + m[7] * by + m[8] * bz + c;
-    const float ix = ((rx * sc - 1.0f) + 1.0f) * 0.5f * um;
-    const float iy = ((ry * sc - 1.0f) + 1.0f) * 0.5f * um;
-    const float iz = ((rz * sc - 1.0f) + 1.0f) * 0.5f * um;
-    const float fz0 = floorf(iz), fy0 = floorf(iy), fx0 = floorf(ix);
-    const int z0 = (int)fz0, y0 = (int)fy0, x0 = (int)fx0;
-    const float fz = iz - fz0, fy = iy - fy0, fx = ix - fx0;
-    float2 acc = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-      for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-        for (int dx = 0; dx < 2; ++dx) {
-          const int zi = z0 + dz, yi = y0 + dy, xi = x0 + dx;
-          if (zi < 0 || zi >= N || yi < 0 || yi >= N || xi < 0 || xi >= N) continue;
-          const float w = (dz ? fz : 1.f - fz) * (dy ? fy : 1.f - fy) * (dx ? fx : 1.f - fx);
-          const float2 r = ref[((long long)zi * N + yi) * N + xi];
-          acc.x += w * r.x;
-          acc.y += w * r.y;
-        }
-    // mask_full of the half spectrum (x wraps with the half length H)
-    const int kp = zq < N / 2 ? zq : zq - N;
-    const int ip = yq < N / 2 ? yq : yq - N;
-    const int jp = kx < N / 2 ? kx : kx - H;
-    if (kp * kp + ip * ip + jp * jp > (N / 2) * (N / 2)) acc = make_float2(0.f, 0.f);
-    if (ctf) {
-      const float cv = ctf[(((long long)b * N + zq) * N + yq) * H + kx];
-      acc.x *= cv;
-      acc.y *= cv;
-    }
-    Rout[(((long long)b * N + zq) * N + yq) * H + kx] = acc;
-    nrm += acc.x * acc.x + acc.y * acc.y;
-  }
-  nrm = mch::wave_sum(nrm);
-  __shared__ float red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nrm;
-  __syncthreads();
-  if (threadIdx.x == 0) rnorm_part[b * N + zq] = red[0] + red[1] + red[2] + red[3];
-}
-
-// background sum over the outer shell and the cosine edge of the shifted patch
 __global__ __launch_bounds__(256) void shift_bg_kernel(const float* __restrict__ vols, long long ldv,
                                                        int N, const float* __restrict__ shift_zyx,
                                                        float radius, float radius_p, float width,
@@ -273,10 +225,11 @@ __global__ __launch_bounds__(256) void shift_bg_kernel(const float* __restrict__
   make_shift_spec(shift_zyx, b, N, sp);
   const float* v = vols + (long long)b * ldv;
   float s = 0.f;
+  const int dz = z - N / 2;
   for (int t = threadIdx.x; t < N * N; t += blockDim.x) {
     const int y = t / N, x = t - y * N;
-    const float dz = (float)(z - N / 2), dy = (float)(y - N / 2), dx = (float)(x - N / 2);
-    const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+    const int dy = y - N / 2, dx = x - N / 2;
+    const float r = sqrtf((float)(dx * dx + dy * dy + dz * dz));
     if (r < radius) continue;  // inner
     const float val = patch_value(v, N, sp, z, y, x);
     if (r > radius_p) {
@@ -293,168 +246,196 @@ __global__ __launch_bounds__(256) void shift_bg_kernel(const float* __restrict__
   if (threadIdx.x == 0) bg_part[b * N + z] = red[0] + red[1] + red[2] + red[3];
 }
 
-// forward FFT along x of the shifted, cosine-masked, rolled patch rows
-__global__ __launch_bounds__(kFftThreads) void shift_fwd_x_kernel(
-    const float* __restrict__ vols, long long ldv, int N, int H, int B,
-    const float* __restrict__ shift_zyx, float radius, float radius_p, float width, float wsum,
-    const float* __restrict__ bg_part, Radices rd, const float2* __restrict__ twg,
-    float2* __restrict__ T) {
-  __shared__ float2 buf[kLines * kMaxN];
+__global__ void shift_bg_final_kernel(const float* __restrict__ bg_part, int N, int B, float wsum,
+                                      float* __restrict__ bg_avg) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  float s = 0.f;
+  for (int z = 0; z < N; ++z) s += bg_part[b * N + z];
+  bg_avg[b] = wsum > 0.f ? s / wsum : 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// forward FFT along x, two real rows per complex line
+// ---------------------------------------------------------------------------
+// workgroup: particle b, plane z, rows y0 .. y0+15 of the rolled, shifted,
+// cosine-masked patch.  Complex line j carries rows y0+j (real) and y0+8+j (imag).
+__global__ __launch_bounds__(kRowThreads) void shift_fwd_x_kernel(
+    const float* __restrict__ vols, long long ldv, int N, int H, const float* __restrict__ shift_zyx,
+    float radius, float radius_p, float width, const float* __restrict__ bg_avg, Radices rd,
+    const float2* __restrict__ twg, float2* __restrict__ T) {
+  __shared__ float2 buf[(kRows / 2) * kMaxN];
   __shared__ float2 tw[kMaxN];
-  __shared__ float bgs[kLines];
-  const long long line0 = (long long)blockIdx.x * kLines;  // lines (b, z, y)
-  const long long nl = (long long)B * N * N;
+  const int ytile = blockIdx.x, z = blockIdx.y, b = blockIdx.z;
+  const int y0 = ytile * kRows;
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
-  if (threadIdx.x < kLines) {
-    const long long g = line0 + threadIdx.x;
-    float bg = 0.f;
-    if (g < nl) {
-      const int b = (int)(g / ((long long)N * N));
-      float s = 0.f;
-      for (int zz = 0; zz < N; ++zz) s += bg_part[b * N + zz];
-      bg = wsum > 0.f ? s / wsum : 0.f;
-    }
-    bgs[threadIdx.x] = bg;
+  ShiftSpec sp;
+  make_shift_spec(shift_zyx, b, N, sp);
+  const float bg = bg_avg[b];
+  const float* v = vols + (long long)b * ldv;
+  // roll by -N/2: rolled[i] = patch[(i + N/2) mod N]
+  const int a0 = (z + N / 2) % N;
+  for (int t = threadIdx.x; t < kRows * N; t += blockDim.x) {
+    const int r = t / N, x = t - r * N;
+    const int y = y0 + r;
+    const int a1 = (y + N / 2) % N, a2 = (x + N / 2) % N;
+    const float pv = patch_value(v, N, sp, a0, a1, a2);
+    const float val = cosine_masked(pv, bg, a0 - N / 2, a1 - N / 2, a2 - N / 2, radius, radius_p, width);
+    const int line = r & (kRows / 2 - 1);
+    float* dst = reinterpret_cast<float*>(buf + line * N + x);
+    dst[r >= kRows / 2 ? 1 : 0] = val;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < kLines * N; t += blockDim.x) {
-    const int l = t / N, x = t - l * N;
-    const long long g = line0 + l;
-    float val = 0.f;
-    if (g < nl) {
-      const int b = (int)(g / ((long long)N * N));
-      const int rem = (int)(g - (long long)b * N * N);
-      const int z = rem / N, y = rem - z * N;
-      // roll by -N/2: rolled[i] = patch[(i + N/2) mod N]
-      const int a0 = (z + N / 2) % N, a1 = (y + N / 2) % N, a2 = (x + N / 2) % N;
-      ShiftSpec sp;
-      make_shift_spec(shift_zyx, b, N, sp);
-      const float v = patch_value(vols + (long long)b * ldv, N, sp, a0, a1, a2);
-      const float dz = (float)(a0 - N / 2), dy = (float)(a1 - N / 2), dx = (float)(a2 - N / 2);
-      const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-      const float bg = bgs[l];
-      if (r > radius_p) {
-        val = bg;
-      } else if (r >= radius) {
-        const float rc = 0.5f + 0.5f * cosf(3.14159265358979323846f * (radius_p - r) / width);
-        val = (1.f - rc) * v + rc * bg;
-      } else {
-        val = v;
-      }
+  fft_lds<kRowThreads>(buf, kRows / 2, N, 1, N, rd, tw, -1.f, false);
+  // unpack A = (Z[k] + conj Z[N-k]) / 2 (row y0+j), B = (Z[k] - conj Z[N-k]) / (2i) (row y0+8+j)
+  float2* out = T + (long long)b * H * N * N;
+  for (int t = threadIdx.x; t < H * kRows; t += blockDim.x) {
+    const int kx = t / kRows, r = t - kx * kRows;
+    const int line = r & (kRows / 2 - 1);
+    const float2 Zk = buf[line * N + kx];
+    const float2 Zn = buf[line * N + (N - kx) % N];
+    float2 o;
+    if (r < kRows / 2) {
+      o = make_float2(0.5f * (Zk.x + Zn.x), 0.5f * (Zk.y - Zn.y));
+    } else {
+      // (Zk - conj Zn) / (2i)
+      const float dx = Zk.x - Zn.x, dy = Zk.y + Zn.y;
+      o = make_float2(0.5f * dy, -0.5f * dx);
     }
-    buf[l * N + x] = make_float2(val, 0.f);
-  }
-  __syncthreads();
-  fft_lines(buf, kLines, N, rd, tw, -1.f);
-  for (int t = threadIdx.x; t < kLines * H; t += blockDim.x) {
-    const int l = t / H, kx = t - l * H;
-    const long long g = line0 + l;
-    if (g < nl) T[g * H + kx] = buf[l * N + kx];
+    out[((long long)kx * N + z) * N + y0 + r] = o;
   }
 }
 
-// FFT along y (lines: fixed (b, z), kx tile) in place; sign -1 forward, +1 inverse
-__global__ __launch_bounds__(kFftThreads) void shift_y_kernel(float2* __restrict__ T, int N, int H,
-                                                              Radices rd,
-                                                              const float2* __restrict__ twg,
-                                                              float sign) {
-  __shared__ float2 buf[kLines * kMaxN];
-  __shared__ float2 tw[kMaxN];
-  const int tile = blockIdx.x, z = blockIdx.y, b = blockIdx.z;
-  const int kx0 = tile * kLines;
-  const int L = min(kLines, H - kx0);
+// ---------------------------------------------------------------------------
+// one (b, kx) plane: FFT y, z; cross power with the rotated reference; inverse
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kPlaneThreads) void shift_yz_kernel(
+    float2* __restrict__ T, const float4* __restrict__ refpair, int N, int H,
+    const float* __restrict__ rot, const float* __restrict__ ctf, Radices rd,
+    const float2* __restrict__ twg, float* __restrict__ rnorm_part, float* __restrict__ snorm_part) {
+  extern __shared__ float4 smem4[];
+  float2* plane = reinterpret_cast<float2*>(smem4);  // [z][y]
+  float2* tw = plane + N * N;
+  __shared__ float red[2][kPlaneThreads / 64];
+  const int kx = blockIdx.x, b = blockIdx.y;
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
-  float2* base = T + ((long long)b * N + z) * N * H;
-  for (int t = threadIdx.x; t < kLines * N; t += blockDim.x) {
-    const int y = t / kLines, l = t - y * kLines;
-    buf[l * N + y] = (l < L) ? base[(long long)y * H + kx0 + l] : make_float2(0.f, 0.f);
-  }
+  float2* P = T + ((long long)b * H + kx) * N * N;
+  for (int t = threadIdx.x; t < N * N; t += blockDim.x) plane[t] = P[t];
   __syncthreads();
-  fft_lines(buf, kLines, N, rd, tw, sign);
-  for (int t = threadIdx.x; t < kLines * N; t += blockDim.x) {
-    const int y = t / kLines, l = t - y * kLines;
-    if (l < L) base[(long long)y * H + kx0 + l] = buf[l * N + y];
-  }
-}
-
-// FFT along z (forward, 1/N^3), ||S||^2, P = conj(R) S, inverse FFT along z; in place
-__global__ __launch_bounds__(kFftThreads) void shift_z_kernel(float2* __restrict__ T,
-                                                              const float2* __restrict__ Rr, int N,
-                                                              int H, Radices rd,
-                                                              const float2* __restrict__ twg,
-                                                              float* __restrict__ snorm_part,
-                                                              int ntile_h) {
-  __shared__ float2 buf[kLines * kMaxN];
-  __shared__ float2 tw[kMaxN];
-  __shared__ float red[4];
-  const int tile = blockIdx.x, y = blockIdx.y, b = blockIdx.z;
-  const int kx0 = tile * kLines;
-  const int L = min(kLines, H - kx0);
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
-  const long long plane = (long long)N * H;
-  float2* base = T + (long long)b * N * plane + (long long)y * H + kx0;
-  const float2* rb = Rr + (long long)b * N * plane + (long long)y * H + kx0;
-  for (int t = threadIdx.x; t < kLines * N; t += blockDim.x) {
-    const int z = t / kLines, l = t - z * kLines;
-    buf[l * N + z] = (l < L) ? base[(long long)z * plane + l] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  fft_lines(buf, kLines, N, rd, tw, -1.f);
+  const int chunk = (kMaxIt * kPlaneThreads * 2) / N;  // lines per FFT call (radix >= 2)
+  // forward: along y (lines z, contiguous), then along z (lines y, stride N)
+  for (int l0 = 0; l0 < N; l0 += chunk)
+    fft_lds<kPlaneThreads>(plane + l0 * N, min(chunk, N - l0), N, 1, N, rd, tw, -1.f, false);
+  for (int l0 = 0; l0 < N; l0 += chunk)
+    fft_lds<kPlaneThreads>(plane + l0, min(chunk, N - l0), 1, N, N, rd, tw, -1.f, true);
+  // cross power with the rotated reference sampled here
+  const float* m = rot + b * 9;
+  const float c = (float)N * 0.5f;
+  const float sc = 2.0f / (float)(N - 1);  // 2 / (size_e - 2) with size_e = N + 1
+  const float um = (float)(N - 1);
   const float inv = 1.0f / ((float)N * (float)N * (float)N);
-  float nrm = 0.f;
-  for (int t = threadIdx.x; t < kLines * N; t += blockDim.x) {
-    const int z = t / kLines, l = t - z * kLines;
-    float2 s = buf[l * N + z];
+  const int xsrc = kx + N / 2;
+  const int jp = kx < N / 2 ? kx : kx - H;
+  float ns = 0.f, nr = 0.f;
+  for (int t = threadIdx.x; t < N * N; t += blockDim.x) {
+    const int kz = t / N, ky = t - kz * N;
+    float2 s = plane[t];
     s.x *= inv;
     s.y *= inv;
-    float2 pr = make_float2(0.f, 0.f);
-    if (l < L) {
-      nrm += s.x * s.x + s.y * s.y;
-      const float2 r = rb[(long long)z * plane + l];
-      // conj(r) * s
-      pr = make_float2(r.x * s.x + r.y * s.y, r.x * s.y - r.y * s.x);
+    ns += s.x * s.x + s.y * s.y;
+    float2 rv = make_float2(0.f, 0.f);
+    const int kp = kz < N / 2 ? kz : kz - N;
+    const int ip = ky < N / 2 ? ky : ky - N;
+    if (kp * kp + ip * ip + jp * jp <= (N / 2) * (N / 2)) {
+      const int zsrc = (kz + N / 2) % N, ysrc = (ky + N / 2) % N;
+      const float bx = (float)xsrc - c, by = (float)ysrc - c, bz = (float)zsrc - c;
+      const float rx = m[0] * bx + m[1] * by + m[2] * bz + c;
+      const float ry = m[3] * bx + m[4] * by + m[5] * bz + c;
+      const float rz = m[6] * bx + m[7] * by + m[8] * bz + c;
+      const float ix = ((rx * sc - 1.0f) + 1.0f) * 0.5f * um;
+      const float iy = ((ry * sc - 1.0f) + 1.0f) * 0.5f * um;
+      const float iz = ((rz * sc - 1.0f) + 1.0f) * 0.5f * um;
+      const float fz0 = floorf(iz), fy0 = floorf(iy), fx0 = floorf(ix);
+      const int z0 = (int)fz0, y0 = (int)fy0, x0 = (int)fx0;
+      const float fz = iz - fz0, fy = iy - fy0, fx = ix - fx0;
+      if (x0 >= -1 && x0 < N) {
+        const bool left_in = x0 >= 0;
+        const int xp = left_in ? x0 : 0;
+#pragma unroll
+        for (int dz = 0; dz < 2; ++dz) {
+          const int zi = z0 + dz;
+          if (zi < 0 || zi >= N) continue;
+#pragma unroll
+          for (int dy = 0; dy < 2; ++dy) {
+            const int yi = y0 + dy;
+            if (yi < 0 || yi >= N) continue;
+            const float w = (dz ? fz : 1.f - fz) * (dy ? fy : 1.f - fy);
+            const float4 q = refpair[((long long)zi * N + yi) * N + xp];
+            // (value at x0, value at x0 + 1)
+            const float2 v0 = left_in ? make_float2(q.x, q.y) : make_float2(0.f, 0.f);
+            const float2 v1 = left_in ? make_float2(q.z, q.w) : make_float2(q.x, q.y);
+            rv.x += w * ((1.f - fx) * v0.x + fx * v1.x);
+            rv.y += w * ((1.f - fx) * v0.y + fx * v1.y);
+          }
+        }
+      }
+      if (ctf) {
+        const float cv = ctf[(((long long)b * N + kz) * N + ky) * H + kx];
+        rv.x *= cv;
+        rv.y *= cv;
+      }
     }
-    buf[l * N + z] = pr;
+    nr += rv.x * rv.x + rv.y * rv.y;
+    // conj(R) * S
+    plane[t] = make_float2(rv.x * s.x + rv.y * s.y, rv.x * s.y - rv.y * s.x);
   }
-  nrm = mch::wave_sum(nrm);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = nrm;
+  ns = mch::wave_sum(ns);
+  nr = mch::wave_sum(nr);
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = ns;
+    red[1][threadIdx.x >> 6] = nr;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) snorm_part[(long long)b * N * ntile_h + y * ntile_h + tile] = red[0] + red[1] + red[2] + red[3];
-  fft_lines(buf, kLines, N, rd, tw, 1.f);
-  for (int t = threadIdx.x; t < kLines * N; t += blockDim.x) {
-    const int z = t / kLines, l = t - z * kLines;
-    if (l < L) base[(long long)z * plane + l] = buf[l * N + z];
+  if (threadIdx.x == 0) {
+    float a = 0.f, bsum = 0.f;
+    for (int w = 0; w < kPlaneThreads / 64; ++w) { a += red[0][w]; bsum += red[1][w]; }
+    snorm_part[b * H + kx] = a;
+    rnorm_part[b * H + kx] = bsum;
   }
+  // inverse: along z, then along y
+  for (int l0 = 0; l0 < N; l0 += chunk)
+    fft_lds<kPlaneThreads>(plane + l0, min(chunk, N - l0), 1, N, N, rd, tw, 1.f, true);
+  for (int l0 = 0; l0 < N; l0 += chunk)
+    fft_lds<kPlaneThreads>(plane + l0 * N, min(chunk, N - l0), N, 1, N, rd, tw, 1.f, false);
+  for (int t = threadIdx.x; t < N * N; t += blockDim.x) P[t] = plane[t];
 }
 
-// inverse FFT along x of the half axis zero-padded to N; real part; argmax per tile
-__global__ __launch_bounds__(kFftThreads) void shift_inv_x_kernel(const float2* __restrict__ T, int N,
-                                                                  int H, int B, Radices rd,
-                                                                  const float2* __restrict__ twg,
-                                                                  float* __restrict__ amax_val,
-                                                                  int* __restrict__ amax_idx,
-                                                                  int ntile_x) {
-  __shared__ float2 buf[kLines * kMaxN];
+// ---------------------------------------------------------------------------
+// inverse FFT along x of the zero-padded half axis, real part, argmax per tile
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kRowThreads) void shift_inv_x_kernel(
+    const float2* __restrict__ T, int N, int H, Radices rd, const float2* __restrict__ twg,
+    float* __restrict__ amax_val, int* __restrict__ amax_idx, int ntile_x) {
+  __shared__ float2 buf[kRows * kMaxN];
   __shared__ float2 tw[kMaxN];
   __shared__ float rv[4];
   __shared__ int ri[4];
-  const long long line0 = (long long)blockIdx.x * kLines;  // lines (b, tz, ty), per-b tiles
-  const int b = (int)(line0 / ((long long)N * N));
+  const int ytile = blockIdx.x, tz = blockIdx.y, b = blockIdx.z;
+  const int y0 = ytile * kRows;
   for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
-  for (int t = threadIdx.x; t < kLines * N; t += blockDim.x) {
-    const int l = t / N, x = t - l * N;
-    const long long g = line0 + l;
-    buf[l * N + x] = (x < H) ? T[g * H + x] : make_float2(0.f, 0.f);
+  const float2* in = T + (long long)b * H * N * N;
+  for (int t = threadIdx.x; t < kRows * N; t += blockDim.x) {
+    const int x = t / kRows, r = t - x * kRows;
+    buf[r * N + x] = (x < H) ? in[((long long)x * N + tz) * N + y0 + r] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  fft_lines(buf, kLines, N, rd, tw, 1.f);
+  fft_lds<kRowThreads>(buf, kRows, N, 1, N, rd, tw, 1.f, false);
   float bv = -INFINITY;
   int bi = 0x7fffffff;
-  const int rem0 = (int)(line0 - (long long)b * N * N);
-  for (int t = threadIdx.x; t < kLines * N; t += blockDim.x) {
-    const int l = t / N, x = t - l * N;
-    const float v = buf[l * N + x].x;
-    const int flat = (rem0 + l) * N + x;  // (tz * N + ty) * N + tx
+  for (int t = threadIdx.x; t < kRows * N; t += blockDim.x) {
+    const int r = t / N, x = t - r * N;
+    const float v = buf[r * N + x].x;
+    const int flat = (tz * N + y0 + r) * N + x;
     if (v > bv || (v == bv && flat < bi)) { bv = v; bi = flat; }
   }
 #pragma unroll
@@ -468,7 +449,7 @@ __global__ __launch_bounds__(kFftThreads) void shift_inv_x_kernel(const float2* 
   if (threadIdx.x == 0) {
     for (int w = 1; w < 4; ++w)
       if (rv[w] > rv[0] || (rv[w] == rv[0] && ri[w] < ri[0])) { rv[0] = rv[w]; ri[0] = ri[w]; }
-    const int tile = rem0 / kLines;
+    const int tile = tz * (N / kRows) + ytile;
     amax_val[(long long)b * ntile_x + tile] = rv[0];
     amax_idx[(long long)b * ntile_x + tile] = ri[0];
   }
@@ -477,7 +458,7 @@ __global__ __launch_bounds__(kFftThreads) void shift_inv_x_kernel(const float2* 
 __global__ __launch_bounds__(256) void shift_final_kernel(const float* __restrict__ amax_val,
                                                           const int* __restrict__ amax_idx,
                                                           int ntile_x, const float* __restrict__ rn,
-                                                          const float* __restrict__ sn, int nsn,
+                                                          const float* __restrict__ sn, int H,
                                                           int N, float* __restrict__ out_shift,
                                                           float* __restrict__ out_score) {
   const int b = blockIdx.x;
@@ -491,8 +472,10 @@ __global__ __launch_bounds__(256) void shift_final_kernel(const float* __restric
     const int i = amax_idx[(long long)b * ntile_x + t];
     if (v > bv || (v == bv && i < bi)) { bv = v; bi = i; }
   }
-  for (int t = threadIdx.x; t < N; t += blockDim.x) r2 += rn[(long long)b * N + t];
-  for (int t = threadIdx.x; t < nsn; t += blockDim.x) s2 += sn[(long long)b * nsn + t];
+  for (int t = threadIdx.x; t < H; t += blockDim.x) {
+    r2 += rn[(long long)b * H + t];
+    s2 += sn[(long long)b * H + t];
+  }
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     const float ov = __shfl_xor(bv, off, 64);
@@ -522,8 +505,7 @@ __global__ __launch_bounds__(256) void shift_final_kernel(const float* __restric
     out_shift[b * 3 + 0] = (float)s2i;
     out_shift[b * 3 + 1] = (float)s1;
     out_shift[b * 3 + 2] = (float)s0;
-    const float den = sqrtf(S2) * sqrtf(R2);
-    out_score[b] = v / den;
+    out_score[b] = v / (sqrtf(S2) * sqrtf(R2));
   }
 }
 
@@ -597,8 +579,8 @@ int mch_shift_plan_create(int N, const void* ref_host /* complex64 (N,N,N) */, f
                           int max_batch, void** out_plan) {
   if (!out_plan || !ref_host) { mch::set_error("mch_shift_plan_create: NULL argument"); return MCH_ERR_ARG; }
   *out_plan = nullptr;
-  if (N < 4 || N > kMaxN || (N % 4) || max_batch < 1) {
-    mch::set_error("mch_shift_plan_create: N=%d unsupported (multiple of 4, <= %d)", N, kMaxN);
+  if (N < 16 || N > kMaxN || (N % kRows) || max_batch < 1) {
+    mch::set_error("mch_shift_plan_create: N=%d unsupported (multiple of %d, <= %d)", N, kRows, kMaxN);
     return MCH_ERR_UNSUPPORTED;
   }
   Radices rd;
@@ -611,9 +593,28 @@ int mch_shift_plan_create(int N, const void* ref_host /* complex64 (N,N,N) */, f
     mch::set_error("mch_shift_plan_create: N=%d is not 2^a 3^b", N);
     return MCH_ERR_UNSUPPORTED;
   }
-  if ((kLines * N / 2 + kFftThreads - 1) / kFftThreads > kMaxIt) {
-    mch::set_error("mch_shift_plan_create: N too large for the FFT tiling");
+  // FFT tiling limit of the row passes (kRows lines of N, radix >= 2)
+  if (kRows * N / 2 > kRowThreads * kMaxIt) {
+    mch::set_error("mch_shift_plan_create: N=%d too large for the row FFT tiling", N);
     return MCH_ERR_UNSUPPORTED;
+  }
+  const size_t plane_lds = ((size_t)N * N + N) * sizeof(float2);
+  int dev = 0, maxlds = 0;
+  int rc = mch::hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  rc = mch::hip_check(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
+                      "hipDeviceGetAttribute");
+  if (rc) return rc;
+  if ((size_t)maxlds < plane_lds + 1024) {
+    mch::set_error("mch_shift_plan_create: a %dx%d plane needs %zu B of LDS, device allows %d", N, N,
+                   plane_lds, maxlds);
+    return MCH_ERR_UNSUPPORTED;
+  }
+  if (plane_lds > 64 * 1024) {
+    rc = mch::hip_check(hipFuncSetAttribute((const void*)shift_yz_kernel,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_lds),
+                        "hipFuncSetAttribute(shift_yz_kernel)");
+    if (rc) return rc;
   }
   ShiftPlan* p = new ShiftPlan();
   memset(p, 0, sizeof(*p));
@@ -624,7 +625,8 @@ int mch_shift_plan_create(int N, const void* ref_host /* complex64 (N,N,N) */, f
   p->radius = box_radius;
   p->width = 5.0f;
   p->radius_p = box_radius + 5.0f;
-  // sum of background weights: |outer| + sum(raised cosine), FP32 like the reference
+  p->plane_lds = plane_lds;
+  // sum of background weights: |outer| + sum(raised cosine)
   {
     double wsum = 0.0;
     for (int z = 0; z < N; ++z)
@@ -638,8 +640,7 @@ int mch_shift_plan_create(int N, const void* ref_host /* complex64 (N,N,N) */, f
         }
     p->wsum = (float)wsum;
   }
-  p->ntile_h = (p->H + kLines - 1) / kLines;
-  p->ntile_x = N * N / kLines;
+  p->ntile_x = N * (N / kRows);
   std::vector<float2> tw(N);
   for (int k = 0; k < N; ++k) {
     const double a = -2.0 * M_PI * (double)k / (double)N;
@@ -647,26 +648,35 @@ int mch_shift_plan_create(int N, const void* ref_host /* complex64 (N,N,N) */, f
   }
   const size_t vol = (size_t)N * N * N;
   const size_t half = (size_t)N * N * p->H;
-  int rc = MCH_OK;
+  std::vector<float4> pair(vol);
+  {
+    const float2* r = (const float2*)ref_host;
+    for (size_t i = 0; i < vol; ++i) {
+      const float2 a = r[i];
+      const bool last = ((i % N) == (size_t)(N - 1));
+      const float2 b = last ? make_float2(0.f, 0.f) : r[i + 1];
+      pair[i] = make_float4(a.x, a.y, b.x, b.y);
+    }
+  }
 #define MCH_TRY(expr, what) do { if ((rc = mch::hip_check((expr), what)) != MCH_OK) goto fail; } while (0)
-  MCH_TRY(hipMalloc(&p->ref, vol * sizeof(float2)), "hipMalloc(ref)");
-  MCH_TRY(hipMemcpy(p->ref, ref_host, vol * sizeof(float2), hipMemcpyHostToDevice), "copy ref");
+  MCH_TRY(hipMalloc(&p->refpair, vol * sizeof(float4)), "hipMalloc(refpair)");
+  MCH_TRY(hipMemcpy(p->refpair, pair.data(), vol * sizeof(float4), hipMemcpyHostToDevice), "copy ref");
   MCH_TRY(hipMalloc(&p->twid, N * sizeof(float2)), "hipMalloc(twid)");
   MCH_TRY(hipMemcpy(p->twid, tw.data(), N * sizeof(float2), hipMemcpyHostToDevice), "copy twid");
-  MCH_TRY(hipMalloc(&p->R, (size_t)max_batch * half * sizeof(float2)), "hipMalloc(R)");
   MCH_TRY(hipMalloc(&p->T, (size_t)max_batch * half * sizeof(float2)), "hipMalloc(T)");
-  MCH_TRY(hipMalloc(&p->rnorm_part, (size_t)max_batch * N * sizeof(float)), "hipMalloc");
-  MCH_TRY(hipMalloc(&p->snorm_part, (size_t)max_batch * N * p->ntile_h * sizeof(float)), "hipMalloc");
+  MCH_TRY(hipMalloc(&p->rnorm_part, (size_t)max_batch * p->H * sizeof(float)), "hipMalloc");
+  MCH_TRY(hipMalloc(&p->snorm_part, (size_t)max_batch * p->H * sizeof(float)), "hipMalloc");
   MCH_TRY(hipMalloc(&p->bg_part, (size_t)max_batch * N * sizeof(float)), "hipMalloc");
+  MCH_TRY(hipMalloc(&p->bg_avg, (size_t)max_batch * sizeof(float)), "hipMalloc");
   MCH_TRY(hipMalloc(&p->amax_val, (size_t)max_batch * p->ntile_x * sizeof(float)), "hipMalloc");
   MCH_TRY(hipMalloc(&p->amax_idx, (size_t)max_batch * p->ntile_x * sizeof(int)), "hipMalloc");
 #undef MCH_TRY
   *out_plan = p;
   return MCH_OK;
 fail:
-  (void)hipFree(p->ref); (void)hipFree(p->twid); (void)hipFree(p->R); (void)hipFree(p->T);
+  (void)hipFree(p->refpair); (void)hipFree(p->twid); (void)hipFree(p->T);
   (void)hipFree(p->rnorm_part); (void)hipFree(p->snorm_part); (void)hipFree(p->bg_part);
-  (void)hipFree(p->amax_val); (void)hipFree(p->amax_idx);
+  (void)hipFree(p->bg_avg); (void)hipFree(p->amax_val); (void)hipFree(p->amax_idx);
   delete p;
   return rc;
 }
@@ -674,9 +684,9 @@ fail:
 int mch_shift_plan_destroy(void* plan) {
   ShiftPlan* p = (ShiftPlan*)plan;
   if (!p) return MCH_OK;
-  (void)hipFree(p->ref); (void)hipFree(p->twid); (void)hipFree(p->R); (void)hipFree(p->T);
+  (void)hipFree(p->refpair); (void)hipFree(p->twid); (void)hipFree(p->T);
   (void)hipFree(p->rnorm_part); (void)hipFree(p->snorm_part); (void)hipFree(p->bg_part);
-  (void)hipFree(p->amax_val); (void)hipFree(p->amax_idx);
+  (void)hipFree(p->bg_avg); (void)hipFree(p->amax_val); (void)hipFree(p->amax_idx);
   delete p;
   return MCH_OK;
 }
@@ -697,33 +707,25 @@ int mch_shift_search(void* plan, const float* subtomos, long long ld_vol, int B,
     mch::set_error("mch_shift_search: bad sizes (B=%d max %d)", B, p->max_batch);
     return MCH_ERR_SIZE;
   }
-  hipLaunchKernelGGL(shift_rotref_kernel, dim3(N, B), dim3(256), 0, stream, p->ref, N, H, rot, ctf,
-                     p->R, p->rnorm_part);
-  int rc = mch::hip_check(hipGetLastError(), "shift_rotref_kernel");
-  if (rc) return rc;
+  int rc;
   hipLaunchKernelGGL(shift_bg_kernel, dim3(N, B), dim3(256), 0, stream, subtomos, ld_vol, N,
                      shift_zyx, p->radius, p->radius_p, p->width, p->bg_part);
   if ((rc = mch::hip_check(hipGetLastError(), "shift_bg_kernel"))) return rc;
-  const long long nl = (long long)B * N * N;
-  hipLaunchKernelGGL(shift_fwd_x_kernel, dim3((unsigned)((nl + kLines - 1) / kLines)),
-                     dim3(kFftThreads), 0, stream, subtomos, ld_vol, N, H, B, shift_zyx, p->radius,
-                     p->radius_p, p->width, p->wsum, p->bg_part, p->rad, p->twid, p->T);
+  hipLaunchKernelGGL(shift_bg_final_kernel, dim3((B + 63) / 64), dim3(64), 0, stream, p->bg_part, N,
+                     B, p->wsum, p->bg_avg);
+  if ((rc = mch::hip_check(hipGetLastError(), "shift_bg_final_kernel"))) return rc;
+  hipLaunchKernelGGL(shift_fwd_x_kernel, dim3(N / kRows, N, B), dim3(kRowThreads), 0, stream,
+                     subtomos, ld_vol, N, H, shift_zyx, p->radius, p->radius_p, p->width, p->bg_avg,
+                     p->rad, p->twid, p->T);
   if ((rc = mch::hip_check(hipGetLastError(), "shift_fwd_x_kernel"))) return rc;
-  hipLaunchKernelGGL(shift_y_kernel, dim3(p->ntile_h, N, B), dim3(kFftThreads), 0, stream, p->T, N,
-                     H, p->rad, p->twid, -1.f);
-  if ((rc = mch::hip_check(hipGetLastError(), "shift_y_kernel"))) return rc;
-  hipLaunchKernelGGL(shift_z_kernel, dim3(p->ntile_h, N, B), dim3(kFftThreads), 0, stream, p->T,
-                     p->R, N, H, p->rad, p->twid, p->snorm_part, p->ntile_h);
-  if ((rc = mch::hip_check(hipGetLastError(), "shift_z_kernel"))) return rc;
-  hipLaunchKernelGGL(shift_y_kernel, dim3(p->ntile_h, N, B), dim3(kFftThreads), 0, stream, p->T, N,
-                     H, p->rad, p->twid, 1.f);
-  if ((rc = mch::hip_check(hipGetLastError(), "shift_y_kernel(inverse)"))) return rc;
-  hipLaunchKernelGGL(shift_inv_x_kernel, dim3((unsigned)(nl / kLines)), dim3(kFftThreads), 0, stream,
-                     p->T, N, H, B, p->rad, p->twid, p->amax_val, p->amax_idx, p->ntile_x);
+  hipLaunchKernelGGL(shift_yz_kernel, dim3(H, B), dim3(kPlaneThreads), p->plane_lds, stream, p->T,
+                     p->refpair, N, H, rot, ctf, p->rad, p->twid, p->rnorm_part, p->snorm_part);
+  if ((rc = mch::hip_check(hipGetLastError(), "shift_yz_kernel"))) return rc;
+  hipLaunchKernelGGL(shift_inv_x_kernel, dim3(N / kRows, N, B), dim3(kRowThreads), 0, stream, p->T,
+                     N, H, p->rad, p->twid, p->amax_val, p->amax_idx, p->ntile_x);
   if ((rc = mch::hip_check(hipGetLastError(), "shift_inv_x_kernel"))) return rc;
   hipLaunchKernelGGL(shift_final_kernel, dim3(B), dim3(256), 0, stream, p->amax_val, p->amax_idx,
-                     p->ntile_x, p->rnorm_part, p->snorm_part, N * p->ntile_h, N, out_shift,
-                     out_score);
+                     p->ntile_x, p->rnorm_part, p->snorm_part, H, N, out_shift, out_score);
   return mch::hip_check(hipGetLastError(), "shift_final_kernel");
 }
 
