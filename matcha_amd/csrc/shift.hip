@@ -6,137 +6,70 @@
 //   compute_shift (complex space, no phase normalisation, score normalised,
 //   upsample 1) :94-282, estimate_shift_like_relion :619-674.
 //
-// Per batch (6 launches):
-//   bg     : background average of the shifted patch over the outer/cosine shell
-//   fwd_x  : shifted + cosine-masked + rolled patch rows, two real rows per
-//            complex FFT along x; keeps kx = 0..N/2, written as T[b][kx][z][y]
-//   yz     : one workgroup per (b, kx) plane held in LDS: FFT along y and z
-//            (1/N^3), ||S||^2, the rotated reference sampled on the fly
-//            (trilinear, ifftshift, mask_full, ctf), ||R||^2, P = conj(R) S,
-//            inverse FFT along z and y
+// Per batch (4 launches):
+//   fwd_x  : one (b, z) slice: shifted, rolled patch split by linearity of the
+//            cosine mask into A = (1 - w) v and bg * w (w = background weight);
+//            FFT along x of A, two real rows per complex line, kx = 0..N/2
+//            written as T[b][kx][z][y]; the slice's share of sum(w v)
+//   yz     : one workgroup per (b, kx) plane held in LDS: T + bg * XM (XM = the
+//            x-spectrum of w, precomputed), FFT along y and z (1/N^3), ||S||^2,
+//            the rotated reference sampled on the fly (trilinear, ifftshift,
+//            mask_full, ctf), ||R||^2, P = conj(R) S, inverse FFT along z and y
 //   inv_x  : inverse FFT along x of the half axis zero-padded to N, real part,
-//            first argmax per tile of lines
+//            first argmax per half slice
 //   final  : first argmax per particle, wrap to (-N/2, N/2], flip to the
 //            reference's (z,y,x) labels, score / (||S|| ||R||)
-// All 1-D transforms are in-LDS mixed-radix (4, 3, 2) Stockham FFTs.
+// All 1-D transforms are compile-time four-step FFTs in LDS (fft_lds.hpp).
 #include "common.hpp"
+#include "fft_lds.hpp"
 
 #include <math.h>
 #include <vector>
 
 namespace {
 
-constexpr int kRowThreads = 256;  // x-pass workgroups
-constexpr int kRows = 16;         // rows per x-pass workgroup
-constexpr int kPlaneThreads = 512;
-constexpr int kMaxN = 128;
-constexpr int kMaxIt = 4;         // FFT items per thread per stage and chunk
-constexpr int kMaxRad = 8;
+constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
+// threads of the row passes (N/2 lines): whole waves (the block reductions
+// assume NT % 64 == 0) and a multiple of N (a thread keeps one column x)
+template <int N> constexpr int row_threads() {
+  return (2 * N) % 64 == 0 ? 2 * N : N / gcd_c(N, 64) * 64;
+}
+template <int N> constexpr int plane_threads() { return 4 * N; }
+// loads issued back to back before their values are used
+template <int PER> constexpr int load_batch() { return PER % 8 == 0 ? 8 : (PER % 4 == 0 ? 4 : (PER % 2 == 0 ? 2 : 1)); }
 
-struct Radices {
-  int n;
-  int r[kMaxRad];
-};
+constexpr int kMaxLut = 2048;  // background-weight table entries (enough for N <= 128)
 
 struct ShiftPlan {
   int N, H, max_batch;
-  float radius, radius_p, width, wsum;  // cosine mask geometry, sum of weights
-  Radices rad;
-  float4* refpair;  // (N, N, N): (ref[z][y][x], ref[z][y][x+1]) of the centred reference
+  float radius, radius_p, width, wsum;  // cosine mask geometry, sum of background weights
+  int lut_lo, lut_n;  // w(r2) tabulated for lut_lo <= r2 < lut_lo + lut_n; 0 below, 1 above
+  float4* refpair;  // (N, N, N + 1): (ref[z][y][x-1], ref[z][y][x]) of the centred reference
   float2* twid;     // (N) e^{-2 pi i j / N}
-  float2* T;        // (B, H, N, N) work spectrum
+  float* lut;       // (lut_n)
+  float2* XM;       // (H, N, N): x-spectrum of the rolled background weight, [kx][z][y]
+  float2* T;        // (B, H, N, N) work spectrum, [b][kx][z][y]
   float* rnorm_part;  // (B, H)
   float* snorm_part;  // (B, H)
   float* bg_part;     // (B, N)
-  float* bg_avg;      // (B)
-  float* amax_val;    // (B, ntile_x)
-  int* amax_idx;      // (B, ntile_x)
+  float* amax_val;    // (B, N)
+  int* amax_idx;      // (B, N)
   int ntile_x;
-  size_t plane_lds;
 };
 
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// Raw buffer resource over nbytes at p: loads at byte offsets >= nbytes return 0
+// (hardware range check), so out-of-volume samples need no branch.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)nbytes, 0x00020000);
 }
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-// (x + i y) * (i c)
-__device__ __forceinline__ float2 cmuli(float2 a, float c) { return make_float2(-c * a.y, c * a.x); }
-
-// In-place mixed-radix Stockham FFT of nl lines in LDS: element n of line l is
-// buf[l * ls + n * es].  sign = -1 forward, +1 inverse (unscaled).  Work items
-// run along the lines (lanes_on_lines, for ls == 1) or along the elements, so
-// that consecutive lanes touch consecutive LDS words.  Needs
-// nl * N / min(radix) <= NT * kMaxIt.
-template <int NT>
-__device__ void fft_lds(float2* buf, int nl, int ls, int es, int N, const Radices& rd,
-                        const float2* tw, float sign, bool lanes_on_lines) {
-  int p = 1;
-  for (int s = 0; s < rd.n; ++s) {
-    const int R = rd.r[s];
-    const int per = N / R;
-    const int items = nl * per;
-    float2 v[kMaxIt][4];
-#pragma unroll
-    for (int it = 0; it < kMaxIt; ++it) {
-      const int idx = threadIdx.x + it * NT;
-      if (idx < items) {
-        const int line = lanes_on_lines ? idx % nl : idx / per;
-        const int i = lanes_on_lines ? idx / nl : idx % per;
-        const float2* src = buf + line * ls + i * es;
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          if (r < R) v[it][r] = src[r * per * es];
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int it = 0; it < kMaxIt; ++it) {
-      const int idx = threadIdx.x + it * NT;
-      if (idx >= items) continue;
-      const int line = lanes_on_lines ? idx % nl : idx / per;
-      const int i = lanes_on_lines ? idx / nl : idx % per;
-      const int k = i % p;
-      const int step = N / (p * R);
-#pragma unroll
-      for (int r = 1; r < 4; ++r) {
-        if (r < R) {
-          float2 w = tw[(r * k * step) % N];
-          if (sign > 0.f) w.y = -w.y;
-          v[it][r] = cmul(v[it][r], w);
-        }
-      }
-      float2 y[4];
-      if (R == 2) {
-        y[0] = cadd(v[it][0], v[it][1]);
-        y[1] = csub(v[it][0], v[it][1]);
-      } else if (R == 3) {
-        const float2 t = cadd(v[it][1], v[it][2]);
-        const float2 d = csub(v[it][1], v[it][2]);
-        const float2 m = make_float2(v[it][0].x - 0.5f * t.x, v[it][0].y - 0.5f * t.y);
-        const float2 q = cmuli(d, sign * 0.86602540378443864676f);
-        y[0] = cadd(v[it][0], t);
-        y[1] = cadd(m, q);
-        y[2] = csub(m, q);
-      } else {
-        const float2 a = cadd(v[it][0], v[it][2]);
-        const float2 b = csub(v[it][0], v[it][2]);
-        const float2 c = cadd(v[it][1], v[it][3]);
-        const float2 d = cmuli(csub(v[it][1], v[it][3]), sign);
-        y[0] = cadd(a, c);
-        y[1] = cadd(b, d);
-        y[2] = csub(a, c);
-        y[3] = csub(b, d);
-      }
-      const int j = (i / p) * p * R + k;
-      float2* dst = buf + line * ls + j * es;
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        if (r < R) dst[r * p * es] = y[r];
-    }
-    __syncthreads();
-    p *= R;
-  }
+constexpr unsigned kOutOfRange = 0x80000000u;
+__device__ __forceinline__ float4 load_f4(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  typedef unsigned v4u __attribute__((ext_vector_type(4)));
+  const v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+}
+__device__ __forceinline__ float load_f1(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
 }
 
 __device__ __forceinline__ float tri_sample_real(const float* __restrict__ v, int N, float z, float y,
@@ -199,195 +132,234 @@ __device__ void make_shift_spec(const float* __restrict__ shift_zyx, int b, int 
   }
 }
 
-// cosine-masked value: inner r < radius keeps v, outer r > radius_p is bg,
-// the shell blends with 0.5 + 0.5 cos(pi (radius_p - r) / width)
-__device__ __forceinline__ float cosine_masked(float v, float bg, int d0, int d1, int d2, float radius,
-                                               float radius_p, float width) {
-  const float r2 = (float)(d0 * d0 + d1 * d1 + d2 * d2);
-  const float r = sqrtf(r2);
-  if (r < radius) return v;
-  if (r > radius_p) return bg;
-  const float rc = 0.5f + 0.5f * cosf(3.14159265358979323846f * (radius_p - r) / width);
-  return (1.f - rc) * v + rc * bg;
+// Background weight of the cosine mask at squared distance r2 from the centre:
+// 0 for r < radius (the patch is kept), 1 for r > radius_p (replaced by the
+// background average), 0.5 + 0.5 cos(pi (radius_p - r) / width) in the shell.
+// The masked patch is (1 - w) v + w bg.  Tabulated per integer r2 at plan time.
+float bg_weight_host(int r2, float radius, float radius_p, float width) {
+  const float r = sqrtf((float)r2);
+  if (r < radius) return 0.f;
+  if (r > radius_p) return 1.f;
+  return 0.5f + 0.5f * cosf(3.14159265358979323846f * (radius_p - r) / width);
 }
+__device__ __forceinline__ float bg_weight(int r2, const float* lut, int lut_lo, int lut_n) {
+  const int i = r2 - lut_lo;
+  return i < 0 ? 0.f : (i >= lut_n ? 1.f : lut[i]);
+}
+
+template <int NT>
+__device__ __forceinline__ float block_sum(float v, float* red) {
+  static_assert(NT % 64 == 0, "whole waves");
+  v = mch::wave_sum(v);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  float s = 0.f;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) s += red[w];
+  return s;
+}
+
+// (i + n/2) mod n for 0 <= i < n
+template <int N>
+__device__ __forceinline__ int roll_half(int i) { return i + N / 2 < N ? i + N / 2 : i - N / 2; }
 
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// background of the cosine mask
+// forward FFT along x of one (b, z) slice, two real rows per complex line
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void shift_bg_kernel(const float* __restrict__ vols, long long ldv,
-                                                       int N, const float* __restrict__ shift_zyx,
-                                                       float radius, float radius_p, float width,
-                                                       float* __restrict__ bg_part) {
+// The rolled, shifted patch is split by linearity of the mask: masked =
+// A + bg * w with A = (1 - w) v.  This pass transforms A and accumulates the
+// background numerator sum(w v) of the slice; the plane pass adds bg * XM.
+// Complex line j carries rows j (real part) and j + N/2 (imaginary part).
+// Each thread keeps one column x (loads) and one row y (spectrum stores).
+template <int N>
+__global__ __launch_bounds__(row_threads<N>()) void shift_fwd_x_kernel(
+    const float* __restrict__ vols, long long ldv, const float* __restrict__ shift_zyx,
+    const float* __restrict__ lutg, int lut_lo, int lut_n, const float2* __restrict__ twg,
+    float2* __restrict__ T, float* __restrict__ bg_part) {
+  constexpr int NT = row_threads<N>(), NL = N / 2, S = N + 1, H = N / 2 + 1, YSTEP = NT / N;
+  static_assert(NT % N == 0 && N % YSTEP == 0, "column split");
+  extern __shared__ float4 smem4[];
+  float2* buf = reinterpret_cast<float2*>(smem4);  // [line][x], stride S
+  float2* tw = buf + NL * S;
+  float* lut = reinterpret_cast<float*>(tw + N);
+  __shared__ float red[NT / 64];
   const int z = blockIdx.x, b = blockIdx.y;
-  ShiftSpec sp;
-  make_shift_spec(shift_zyx, b, N, sp);
-  const float* v = vols + (long long)b * ldv;
-  float s = 0.f;
-  const int dz = z - N / 2;
-  for (int t = threadIdx.x; t < N * N; t += blockDim.x) {
-    const int y = t / N, x = t - y * N;
-    const int dy = y - N / 2, dx = x - N / 2;
-    const float r = sqrtf((float)(dx * dx + dy * dy + dz * dz));
-    if (r < radius) continue;  // inner
-    const float val = patch_value(v, N, sp, z, y, x);
-    if (r > radius_p) {
-      s += val;
-    } else {
-      const float rc = 0.5f + 0.5f * cosf(3.14159265358979323846f * (radius_p - r) / width);
-      s += rc * val;
-    }
-  }
-  s = mch::wave_sum(s);
-  __shared__ float red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
+  for (int k = threadIdx.x; k < lut_n; k += NT) lut[k] = lutg[k];
   __syncthreads();
-  if (threadIdx.x == 0) bg_part[b * N + z] = red[0] + red[1] + red[2] + red[3];
-}
-
-__global__ void shift_bg_final_kernel(const float* __restrict__ bg_part, int N, int B, float wsum,
-                                      float* __restrict__ bg_avg) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= B) return;
-  float s = 0.f;
-  for (int z = 0; z < N; ++z) s += bg_part[b * N + z];
-  bg_avg[b] = wsum > 0.f ? s / wsum : 0.f;
-}
-
-// ---------------------------------------------------------------------------
-// forward FFT along x, two real rows per complex line
-// ---------------------------------------------------------------------------
-// workgroup: particle b, plane z, rows y0 .. y0+15 of the rolled, shifted,
-// cosine-masked patch.  Complex line j carries rows y0+j (real) and y0+8+j (imag).
-__global__ __launch_bounds__(kRowThreads) void shift_fwd_x_kernel(
-    const float* __restrict__ vols, long long ldv, int N, int H, const float* __restrict__ shift_zyx,
-    float radius, float radius_p, float width, const float* __restrict__ bg_avg, Radices rd,
-    const float2* __restrict__ twg, float2* __restrict__ T) {
-  __shared__ float2 buf[(kRows / 2) * kMaxN];
-  __shared__ float2 tw[kMaxN];
-  const int ytile = blockIdx.x, z = blockIdx.y, b = blockIdx.z;
-  const int y0 = ytile * kRows;
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
   ShiftSpec sp;
   make_shift_spec(shift_zyx, b, N, sp);
-  const float bg = bg_avg[b];
   const float* v = vols + (long long)b * ldv;
   // roll by -N/2: rolled[i] = patch[(i + N/2) mod N]
-  const int a0 = (z + N / 2) % N;
-  for (int t = threadIdx.x; t < kRows * N; t += blockDim.x) {
-    const int r = t / N, x = t - r * N;
-    const int y = y0 + r;
-    const int a1 = (y + N / 2) % N, a2 = (x + N / 2) % N;
-    const float pv = patch_value(v, N, sp, a0, a1, a2);
-    const float val = cosine_masked(pv, bg, a0 - N / 2, a1 - N / 2, a2 - N / 2, radius, radius_p, width);
-    const int line = r & (kRows / 2 - 1);
-    float* dst = reinterpret_cast<float*>(buf + line * N + x);
-    dst[r >= kRows / 2 ? 1 : 0] = val;
+  const int x = threadIdx.x % N, yb = threadIdx.x / N;
+  const int a0 = roll_half<N>(z), a2 = roll_half<N>(x);
+  const int r2zx = (a0 - N / 2) * (a0 - N / 2) + (a2 - N / 2) * (a2 - N / 2);
+  float bgs = 0.f;
+  auto deposit = [&](int y, float pv) {
+    const int a1 = roll_half<N>(y);
+    const float w = bg_weight(r2zx + (a1 - N / 2) * (a1 - N / 2), lut, lut_lo, lut_n);
+    bgs += w * pv;
+    float* dst = reinterpret_cast<float*>(buf + (y < NL ? y : y - NL) * S + x);
+    dst[y >= NL ? 1 : 0] = (1.f - w) * pv;
+  };
+  if (sp.integral) {
+    // integral shift: one buffer load per voxel, out-of-volume reads return 0
+    const __amdgpu_buffer_rsrc_t rv = make_rsrc(v, (unsigned)(N * N * N) * 4u);
+    const int zs = a0 + sp.si[0], xs = a2 + sp.si[2];
+    const bool zxin = zs >= 0 && zs < N && xs >= 0 && xs < N;
+    const int zxoff = zs * N * N + xs;
+    constexpr int PER = N / YSTEP;
+    constexpr int U = load_batch<PER>();
+    for (int base = 0; base < PER; base += U) {
+      float pv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int y = yb + (base + u) * YSTEP;
+        const int ys = roll_half<N>(y) + sp.si[1];
+        const bool ok = zxin && ys >= 0 && ys < N;
+        pv[u] = load_f1(rv, ok ? (unsigned)(zxoff + ys * N) * 4u : kOutOfRange);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) deposit(yb + (base + u) * YSTEP, pv[u]);
+    }
+  } else {
+    for (int y = yb; y < N; y += YSTEP) deposit(y, patch_value(v, N, sp, a0, roll_half<N>(y), a2));
   }
-  __syncthreads();
-  fft_lds<kRowThreads>(buf, kRows / 2, N, 1, N, rd, tw, -1.f, false);
-  // unpack A = (Z[k] + conj Z[N-k]) / 2 (row y0+j), B = (Z[k] - conj Z[N-k]) / (2i) (row y0+8+j)
-  float2* out = T + (long long)b * H * N * N;
-  for (int t = threadIdx.x; t < H * kRows; t += blockDim.x) {
-    const int kx = t / kRows, r = t - kx * kRows;
-    const int line = r & (kRows / 2 - 1);
-    const float2 Zk = buf[line * N + kx];
-    const float2 Zn = buf[line * N + (N - kx) % N];
+  bgs = block_sum<NT>(bgs, red);  // its barrier also publishes buf
+  if (threadIdx.x == 0) bg_part[b * N + z] = bgs;
+  mch::fft::forward<N, NL, NT, -1>(buf, S, 1, tw);
+  // unpack A = (Z[k] + conj Z[N-k]) / 2 (row j), B = (Z[k] - conj Z[N-k]) / (2i) (row j + N/2)
+  const int y = threadIdx.x % N;
+  const float2* line = buf + (y < NL ? y : y - NL) * S;
+  float2* out = T + (long long)b * H * N * N + (long long)z * N + y;
+  for (int kx = threadIdx.x / N; kx < H; kx += YSTEP) {
+    const float2 Zk = line[mch::fft::slot_of_freq<N>(kx)];
+    const float2 Zn = line[mch::fft::slot_of_freq<N>(kx == 0 ? 0 : N - kx)];
     float2 o;
-    if (r < kRows / 2) {
+    if (y < NL) {
       o = make_float2(0.5f * (Zk.x + Zn.x), 0.5f * (Zk.y - Zn.y));
     } else {
-      // (Zk - conj Zn) / (2i)
       const float dx = Zk.x - Zn.x, dy = Zk.y + Zn.y;
       o = make_float2(0.5f * dy, -0.5f * dx);
     }
-    out[((long long)kx * N + z) * N + y0 + r] = o;
+    out[(long long)kx * N * N] = o;
   }
 }
 
 // ---------------------------------------------------------------------------
 // one (b, kx) plane: FFT y, z; cross power with the rotated reference; inverse
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kPlaneThreads) void shift_yz_kernel(
-    float2* __restrict__ T, const float4* __restrict__ refpair, int N, int H,
-    const float* __restrict__ rot, const float* __restrict__ ctf, Radices rd,
+template <int N>
+__global__ __launch_bounds__(plane_threads<N>()) void shift_yz_kernel(
+    float2* __restrict__ T, const float2* __restrict__ XM, const float* __restrict__ bg_part, float wsum,
+    const float4* __restrict__ refpair, const float* __restrict__ rot, const float* __restrict__ ctf,
     const float2* __restrict__ twg, float* __restrict__ rnorm_part, float* __restrict__ snorm_part) {
+  constexpr int NT = plane_threads<N>(), S = N + 1, H = N / 2 + 1, ZSTEP = NT / N;
+  static_assert(NT % 64 == 0 && NT % N == 0, "whole waves, column split");
   extern __shared__ float4 smem4[];
-  float2* plane = reinterpret_cast<float2*>(smem4);  // [z][y]
-  float2* tw = plane + N * N;
-  __shared__ float red[2][kPlaneThreads / 64];
-  const int kx = blockIdx.x, b = blockIdx.y;
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
+  float2* plane = reinterpret_cast<float2*>(smem4);  // [z][y], stride S
+  float2* tw = plane + N * S;
+  __shared__ float red[2][NT / 64];
+  // XCD-aware order: the 8 XCDs take consecutive ranges of (b, kx) planes, so
+  // neighbouring planes of one particle (which sample neighbouring planes of
+  // the reference) share one L2
+  const int nplanes = gridDim.x;
+  const int lb = mch::xcd_remap(blockIdx.x, nplanes);
+  const int kx = lb % H, b = lb / H;
+  for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
+  // background average of particle b; every wave sums the partials in the same order
+  float bgs = 0.f;
+  for (int zz = threadIdx.x & 63; zz < N; zz += 64) bgs += bg_part[b * N + zz];
+  bgs = mch::wave_sum(bgs);
+  const float bg = wsum > 0.f ? bgs / wsum : 0.f;
   float2* P = T + ((long long)b * H + kx) * N * N;
-  for (int t = threadIdx.x; t < N * N; t += blockDim.x) plane[t] = P[t];
+  const float2* M = XM + (long long)kx * N * N;
+  // thread column y (fixed), rows z = zb, zb + ZSTEP, ...
+  const int cy = threadIdx.x % N, zb = threadIdx.x / N;
+  {
+    constexpr int PER = N / ZSTEP;
+    constexpr int U = load_batch<PER>();
+    for (int base = 0; base < PER; base += U) {
+      float2 a[U], mm[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int zz = zb + (base + u) * ZSTEP;
+        a[u] = P[zz * N + cy];
+        mm[u] = M[zz * N + cy];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        plane[(zb + (base + u) * ZSTEP) * S + cy] = make_float2(a[u].x + bg * mm[u].x, a[u].y + bg * mm[u].y);
+    }
+  }
   __syncthreads();
-  const int chunk = (kMaxIt * kPlaneThreads * 2) / N;  // lines per FFT call (radix >= 2)
-  // forward: along y (lines z, contiguous), then along z (lines y, stride N)
-  for (int l0 = 0; l0 < N; l0 += chunk)
-    fft_lds<kPlaneThreads>(plane + l0 * N, min(chunk, N - l0), N, 1, N, rd, tw, -1.f, false);
-  for (int l0 = 0; l0 < N; l0 += chunk)
-    fft_lds<kPlaneThreads>(plane + l0, min(chunk, N - l0), 1, N, N, rd, tw, -1.f, true);
-  // cross power with the rotated reference sampled here
+  mch::fft::forward<N, N, NT, -1>(plane, S, 1, tw);  // along y: lines are rows z
+  mch::fft::forward<N, N, NT, -1>(plane, 1, S, tw);  // along z: lines are columns y
+  // slot (sz, sy) now holds frequency (freq(sz), freq(sy)); cross power in place.
+  // The thread keeps frequency ky = cy (natural order across lanes, so that
+  // neighbouring lanes sample neighbouring points of the reference) in slot
+  // column sy = slot(ky): everything that depends on ky is hoisted.
   const float* m = rot + b * 9;
+  const float m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4], m5 = m[5], m6 = m[6], m7 = m[7],
+              m8 = m[8];
   const float c = (float)N * 0.5f;
   const float sc = 2.0f / (float)(N - 1);  // 2 / (size_e - 2) with size_e = N + 1
   const float um = (float)(N - 1);
   const float inv = 1.0f / ((float)N * (float)N * (float)N);
-  const int xsrc = kx + N / 2;
   const int jp = kx < N / 2 ? kx : kx - H;
+  const float bx = (float)(kx + N / 2) - c;
+  const int ky = cy, sy = mch::fft::slot_of_freq<N>(cy);
+  const int ip = ky < N / 2 ? ky : ky - N;
+  const int r2xy = ip * ip + jp * jp;
+  const float by = (float)roll_half<N>(ky) - c;
+  const float px = m0 * bx + m1 * by + c, py = m3 * bx + m4 * by + c, pz = m6 * bx + m7 * by + c;
+  const __amdgpu_buffer_rsrc_t rref = make_rsrc(refpair, (unsigned)(N * N * (N + 1)) * 16u);
+  const float* ctfb = ctf ? ctf + (long long)b * N * N * H : nullptr;
   float ns = 0.f, nr = 0.f;
-  for (int t = threadIdx.x; t < N * N; t += blockDim.x) {
-    const int kz = t / N, ky = t - kz * N;
-    float2 s = plane[t];
+#pragma unroll 2
+  for (int i = 0; i < N / ZSTEP; ++i) {
+    const int sz = zb + i * ZSTEP;
+    const int kz = mch::fft::freq_of_slot<N>(sz);
+    const int kp = kz < N / 2 ? kz : kz - N;
+    const bool inball = kp * kp + r2xy <= (N / 2) * (N / 2);
+    const float bz = (float)roll_half<N>(kz) - c;
+    const float rx = fmaf(m2, bz, px), ry = fmaf(m5, bz, py), rz = fmaf(m8, bz, pz);
+    const float ix = ((rx * sc - 1.0f) + 1.0f) * 0.5f * um;
+    const float iy = ((ry * sc - 1.0f) + 1.0f) * 0.5f * um;
+    const float iz = ((rz * sc - 1.0f) + 1.0f) * 0.5f * um;
+    const float fz0 = floorf(iz), fy0 = floorf(iy), fx0 = floorf(ix);
+    const int z0 = (int)fz0, y0 = (int)fy0, x0 = (int)fx0;
+    const float fz = iz - fz0, fy = iy - fy0, fx = ix - fx0;
+    // pair column x0 + 1 holds (ref[x0], ref[x0 + 1]) for -1 <= x0 < N; corners
+    // outside the volume and samples outside the ball read as zero
+    const bool xok = inball && x0 >= -1 && x0 < N;
+    const unsigned o00 = (unsigned)((z0 * N + y0) * (N + 1) + x0 + 1) * 16u;
+    const bool z0ok = z0 >= 0 && z0 < N, z1ok = z0 >= -1 && z0 < N - 1;
+    const bool y0ok = y0 >= 0 && y0 < N, y1ok = y0 >= -1 && y0 < N - 1;
+    const float4 q00 = load_f4(rref, xok && z0ok && y0ok ? o00 : kOutOfRange);
+    const float4 q01 = load_f4(rref, xok && z0ok && y1ok ? o00 + (N + 1) * 16u : kOutOfRange);
+    const float4 q10 = load_f4(rref, xok && z1ok && y0ok ? o00 + N * (N + 1) * 16u : kOutOfRange);
+    const float4 q11 = load_f4(rref, xok && z1ok && y1ok ? o00 + (N + 1) * (N + 1) * 16u : kOutOfRange);
+    float cv = 1.f;
+    if (ctfb && inball) cv = ctfb[(kz * N + ky) * H + kx];
+    const float s00 = (1.f - fz) * (1.f - fy), s01 = (1.f - fz) * fy, s10 = fz * (1.f - fy), s11 = fz * fy;
+    float2 rv;
+    rv.x = s00 * fmaf(fx, q00.z - q00.x, q00.x) + s01 * fmaf(fx, q01.z - q01.x, q01.x) +
+           s10 * fmaf(fx, q10.z - q10.x, q10.x) + s11 * fmaf(fx, q11.z - q11.x, q11.x);
+    rv.y = s00 * fmaf(fx, q00.w - q00.y, q00.y) + s01 * fmaf(fx, q01.w - q01.y, q01.y) +
+           s10 * fmaf(fx, q10.w - q10.y, q10.y) + s11 * fmaf(fx, q11.w - q11.y, q11.y);
+    rv.x *= cv;
+    rv.y *= cv;
+    float2* slot = plane + sz * S + sy;
+    float2 s = *slot;
     s.x *= inv;
     s.y *= inv;
     ns += s.x * s.x + s.y * s.y;
-    float2 rv = make_float2(0.f, 0.f);
-    const int kp = kz < N / 2 ? kz : kz - N;
-    const int ip = ky < N / 2 ? ky : ky - N;
-    if (kp * kp + ip * ip + jp * jp <= (N / 2) * (N / 2)) {
-      const int zsrc = (kz + N / 2) % N, ysrc = (ky + N / 2) % N;
-      const float bx = (float)xsrc - c, by = (float)ysrc - c, bz = (float)zsrc - c;
-      const float rx = m[0] * bx + m[1] * by + m[2] * bz + c;
-      const float ry = m[3] * bx + m[4] * by + m[5] * bz + c;
-      const float rz = m[6] * bx + m[7] * by + m[8] * bz + c;
-      const float ix = ((rx * sc - 1.0f) + 1.0f) * 0.5f * um;
-      const float iy = ((ry * sc - 1.0f) + 1.0f) * 0.5f * um;
-      const float iz = ((rz * sc - 1.0f) + 1.0f) * 0.5f * um;
-      const float fz0 = floorf(iz), fy0 = floorf(iy), fx0 = floorf(ix);
-      const int z0 = (int)fz0, y0 = (int)fy0, x0 = (int)fx0;
-      const float fz = iz - fz0, fy = iy - fy0, fx = ix - fx0;
-      if (x0 >= -1 && x0 < N) {
-        const bool left_in = x0 >= 0;
-        const int xp = left_in ? x0 : 0;
-#pragma unroll
-        for (int dz = 0; dz < 2; ++dz) {
-          const int zi = z0 + dz;
-          if (zi < 0 || zi >= N) continue;
-#pragma unroll
-          for (int dy = 0; dy < 2; ++dy) {
-            const int yi = y0 + dy;
-            if (yi < 0 || yi >= N) continue;
-            const float w = (dz ? fz : 1.f - fz) * (dy ? fy : 1.f - fy);
-            const float4 q = refpair[((long long)zi * N + yi) * N + xp];
-            // (value at x0, value at x0 + 1)
-            const float2 v0 = left_in ? make_float2(q.x, q.y) : make_float2(0.f, 0.f);
-            const float2 v1 = left_in ? make_float2(q.z, q.w) : make_float2(q.x, q.y);
-            rv.x += w * ((1.f - fx) * v0.x + fx * v1.x);
-            rv.y += w * ((1.f - fx) * v0.y + fx * v1.y);
-          }
-        }
-      }
-      if (ctf) {
-        const float cv = ctf[(((long long)b * N + kz) * N + ky) * H + kx];
-        rv.x *= cv;
-        rv.y *= cv;
-      }
-    }
     nr += rv.x * rv.x + rv.y * rv.y;
     // conj(R) * S
-    plane[t] = make_float2(rv.x * s.x + rv.y * s.y, rv.x * s.y - rv.y * s.x);
+    *slot = make_float2(rv.x * s.x + rv.y * s.y, rv.x * s.y - rv.y * s.x);
   }
   ns = mch::wave_sum(ns);
   nr = mch::wave_sum(nr);
@@ -395,48 +367,79 @@ __global__ __launch_bounds__(kPlaneThreads) void shift_yz_kernel(
     red[0][threadIdx.x >> 6] = ns;
     red[1][threadIdx.x >> 6] = nr;
   }
-  __syncthreads();
+  __syncthreads();  // also publishes the cross power to every thread
   if (threadIdx.x == 0) {
     float a = 0.f, bsum = 0.f;
-    for (int w = 0; w < kPlaneThreads / 64; ++w) { a += red[0][w]; bsum += red[1][w]; }
+    for (int w = 0; w < NT / 64; ++w) { a += red[0][w]; bsum += red[1][w]; }
     snorm_part[b * H + kx] = a;
     rnorm_part[b * H + kx] = bsum;
   }
-  // inverse: along z, then along y
-  for (int l0 = 0; l0 < N; l0 += chunk)
-    fft_lds<kPlaneThreads>(plane + l0, min(chunk, N - l0), 1, N, N, rd, tw, 1.f, true);
-  for (int l0 = 0; l0 < N; l0 += chunk)
-    fft_lds<kPlaneThreads>(plane + l0 * N, min(chunk, N - l0), N, 1, N, rd, tw, 1.f, false);
-  for (int t = threadIdx.x; t < N * N; t += blockDim.x) P[t] = plane[t];
+  mch::fft::inverse<N, N, NT, 1>(plane, 1, S, tw);  // along z
+  mch::fft::inverse<N, N, NT, 1>(plane, S, 1, tw);  // along y
+  for (int zz = zb; zz < N; zz += ZSTEP) P[zz * N + cy] = plane[zz * S + cy];
 }
 
 // ---------------------------------------------------------------------------
-// inverse FFT along x of the zero-padded half axis, real part, argmax per tile
+// inverse FFT along x of the zero-padded half axis, real part, argmax
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kRowThreads) void shift_inv_x_kernel(
-    const float2* __restrict__ T, int N, int H, Radices rd, const float2* __restrict__ twg,
-    float* __restrict__ amax_val, int* __restrict__ amax_idx, int ntile_x) {
-  __shared__ float2 buf[kRows * kMaxN];
-  __shared__ float2 tw[kMaxN];
-  __shared__ float rv[4];
-  __shared__ int ri[4];
-  const int ytile = blockIdx.x, tz = blockIdx.y, b = blockIdx.z;
-  const int y0 = ytile * kRows;
-  for (int k = threadIdx.x; k < N; k += blockDim.x) tw[k] = twg[k];
-  const float2* in = T + (long long)b * H * N * N;
-  for (int t = threadIdx.x; t < kRows * N; t += blockDim.x) {
-    const int x = t / kRows, r = t - x * kRows;
-    buf[r * N + x] = (x < H) ? in[((long long)x * N + tz) * N + y0 + r] : make_float2(0.f, 0.f);
+// For a row X (kx = 0..N/2, zero above), Re(sum_{k<=N/2} X_k e^{2 pi i k n/N}) =
+// y[n] / 2 with y the inverse DFT of the Hermitian row Y (Y_0 = 2 Re X_0,
+// Y_{N/2} = 2 Re X_{N/2}, Y_k = X_k, Y_{N-k} = conj X_k).  Two such real rows
+// go through one complex FFT as Y1 + i Y2: rows j and j + N/2 of plane z.
+template <int N>
+__global__ __launch_bounds__(row_threads<N>()) void shift_inv_x_kernel(
+    const float2* __restrict__ T, const float2* __restrict__ twg, float* __restrict__ amax_val,
+    int* __restrict__ amax_idx) {
+  constexpr int NT = row_threads<N>(), NL = N / 2, S = N + 1, H = N / 2 + 1;
+  static_assert(NT % 64 == 0, "whole waves");
+  extern __shared__ float4 smem4[];
+  float2* buf = reinterpret_cast<float2*>(smem4);  // [line][x], stride S
+  float2* tw = buf + NL * S;
+  __shared__ float rv[NT / 64];
+  __shared__ int ri[NT / 64];
+  const int z = blockIdx.x, b = blockIdx.y;
+  for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
+  const float2* in = T + (long long)b * H * N * N + (long long)z * N;
+  {
+    constexpr int ITEMS = NL * H;
+    constexpr int U = 4;
+    for (int base = 0; base < ITEMS; base += U * NT) {
+      float2 x1[U], x2[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int t = base + u * NT + threadIdx.x;
+        const int j = t % NL, k = t / NL;
+        const long long o = t < ITEMS ? (long long)k * N * N + j : 0;
+        x1[u] = in[o];
+        x2[u] = in[o + NL];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int t = base + u * NT + threadIdx.x;
+        if (t >= ITEMS) break;
+        const int j = t % NL, k = t / NL;
+        float2* line = buf + j * S;
+        if (k == 0 || k == N / 2) {
+          line[k] = make_float2(2.f * x1[u].x, 2.f * x2[u].x);
+        } else {
+          line[k] = make_float2(x1[u].x - x2[u].y, x1[u].y + x2[u].x);
+          line[N - k] = make_float2(x1[u].x + x2[u].y, x2[u].x - x1[u].y);
+        }
+      }
+    }
   }
   __syncthreads();
-  fft_lds<kRowThreads>(buf, kRows, N, 1, N, rd, tw, 1.f, false);
+  mch::fft::forward<N, NL, NT, 1>(buf, S, 1, tw);
   float bv = -INFINITY;
   int bi = 0x7fffffff;
-  for (int t = threadIdx.x; t < kRows * N; t += blockDim.x) {
-    const int r = t / N, x = t - r * N;
-    const float v = buf[r * N + x].x;
-    const int flat = (tz * N + y0 + r) * N + x;
-    if (v > bv || (v == bv && flat < bi)) { bv = v; bi = flat; }
+  for (int t = threadIdx.x; t < NL * N; t += NT) {
+    const int j = t / N, s = t - (t / N) * N;
+    const float2 zv = buf[j * S + s];
+    const int xx = mch::fft::freq_of_slot<N>(s);
+    const int f1 = (z * N + j) * N + xx, f2 = f1 + NL * N;
+    const float v1 = 0.5f * zv.x, v2 = 0.5f * zv.y;
+    if (v1 > bv || (v1 == bv && f1 < bi)) { bv = v1; bi = f1; }
+    if (v2 > bv || (v2 == bv && f2 < bi)) { bv = v2; bi = f2; }
   }
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
@@ -447,11 +450,12 @@ __global__ __launch_bounds__(kRowThreads) void shift_inv_x_kernel(
   if ((threadIdx.x & 63) == 0) { rv[threadIdx.x >> 6] = bv; ri[threadIdx.x >> 6] = bi; }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < 4; ++w)
-      if (rv[w] > rv[0] || (rv[w] == rv[0] && ri[w] < ri[0])) { rv[0] = rv[w]; ri[0] = ri[w]; }
-    const int tile = tz * (N / kRows) + ytile;
-    amax_val[(long long)b * ntile_x + tile] = rv[0];
-    amax_idx[(long long)b * ntile_x + tile] = ri[0];
+    float v = rv[0];
+    int i = ri[0];
+    for (int w = 1; w < NT / 64; ++w)
+      if (rv[w] > v || (rv[w] == v && ri[w] < i)) { v = rv[w]; i = ri[w]; }
+    amax_val[(long long)b * N + z] = v;
+    amax_idx[(long long)b * N + z] = i;
   }
 }
 
@@ -549,6 +553,93 @@ __global__ void euler_to_shift_rot_kernel(const float* __restrict__ al, const fl
 }
 
 // ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace {
+
+template <int N> constexpr size_t row_lds() { return ((size_t)(N / 2) * (N + 1) + N) * sizeof(float2); }
+template <int N> constexpr size_t plane_lds() { return ((size_t)N * (N + 1) + N) * sizeof(float2); }
+
+// Checks the LDS the kernels for box N need against the device and raises the
+// dynamic-LDS limit of those that exceed the default 64 KiB.
+template <int N>
+int configure_kernels() {
+  int dev = 0, maxlds = 0;
+  int rc = mch::hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  rc = mch::hip_check(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
+                      "hipDeviceGetAttribute");
+  if (rc) return rc;
+  const size_t fwd_lds = row_lds<N>() + kMaxLut * sizeof(float);
+  if ((size_t)maxlds < plane_lds<N>() + 1024 || (size_t)maxlds < fwd_lds + 1024) {
+    mch::set_error("mch_shift_plan_create: a %dx%d plane needs %zu B of LDS, device allows %d", N, N,
+                   plane_lds<N>(), maxlds);
+    return MCH_ERR_UNSUPPORTED;
+  }
+  if (plane_lds<N>() > 64 * 1024) {
+    rc = mch::hip_check(hipFuncSetAttribute((const void*)shift_yz_kernel<N>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_lds<N>()),
+                        "hipFuncSetAttribute(shift_yz_kernel)");
+    if (rc) return rc;
+  }
+  if (fwd_lds > 64 * 1024) {
+    rc = mch::hip_check(hipFuncSetAttribute((const void*)shift_fwd_x_kernel<N>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)fwd_lds),
+                        "hipFuncSetAttribute(shift_fwd_x_kernel)");
+    if (rc) return rc;
+  }
+  if (row_lds<N>() > 64 * 1024) {
+    rc = mch::hip_check(hipFuncSetAttribute((const void*)shift_inv_x_kernel<N>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)row_lds<N>()),
+                        "hipFuncSetAttribute(shift_inv_x_kernel)");
+    if (rc) return rc;
+  }
+  return MCH_OK;
+}
+
+void free_plan(ShiftPlan* p) {
+  if (!p) return;
+  (void)hipFree(p->refpair); (void)hipFree(p->twid); (void)hipFree(p->lut); (void)hipFree(p->XM);
+  (void)hipFree(p->T); (void)hipFree(p->rnorm_part); (void)hipFree(p->snorm_part);
+  (void)hipFree(p->bg_part); (void)hipFree(p->amax_val); (void)hipFree(p->amax_idx);
+  delete p;
+}
+
+struct SearchArgs {
+  const float* subtomos;
+  long long ld_vol;
+  int B;
+  const float* shift_zyx;
+  const float* rot;
+  const float* ctf;
+  float* out_shift;
+  float* out_score;
+};
+
+// fwd_x -> yz -> inv_x -> final, all on one stream
+template <int N>
+int launch_search(ShiftPlan* p, const SearchArgs& a, hipStream_t stream) {
+  constexpr int H = N / 2 + 1;
+  int rc;
+  hipLaunchKernelGGL(shift_fwd_x_kernel<N>, dim3(N, a.B), dim3(row_threads<N>()),
+                     row_lds<N>() + (size_t)p->lut_n * sizeof(float), stream, a.subtomos, a.ld_vol,
+                     a.shift_zyx, p->lut, p->lut_lo, p->lut_n, p->twid, p->T, p->bg_part);
+  if ((rc = mch::hip_check(hipGetLastError(), "shift_fwd_x_kernel"))) return rc;
+  hipLaunchKernelGGL(shift_yz_kernel<N>, dim3(H * a.B), dim3(plane_threads<N>()), plane_lds<N>(), stream,
+                     p->T, p->XM, p->bg_part, p->wsum, p->refpair, a.rot, a.ctf, p->twid, p->rnorm_part,
+                     p->snorm_part);
+  if ((rc = mch::hip_check(hipGetLastError(), "shift_yz_kernel"))) return rc;
+  hipLaunchKernelGGL(shift_inv_x_kernel<N>, dim3(N, a.B), dim3(row_threads<N>()), row_lds<N>(), stream,
+                     p->T, p->twid, p->amax_val, p->amax_idx);
+  if ((rc = mch::hip_check(hipGetLastError(), "shift_inv_x_kernel"))) return rc;
+  hipLaunchKernelGGL(shift_final_kernel, dim3(a.B), dim3(256), 0, stream, p->amax_val, p->amax_idx,
+                     p->ntile_x, p->rnorm_part, p->snorm_part, H, N, a.out_shift, a.out_score);
+  return mch::hip_check(hipGetLastError(), "shift_final_kernel");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -579,115 +670,128 @@ int mch_shift_plan_create(int N, const void* ref_host /* complex64 (N,N,N) */, f
                           int max_batch, void** out_plan) {
   if (!out_plan || !ref_host) { mch::set_error("mch_shift_plan_create: NULL argument"); return MCH_ERR_ARG; }
   *out_plan = nullptr;
-  if (N < 16 || N > kMaxN || (N % kRows) || max_batch < 1) {
-    mch::set_error("mch_shift_plan_create: N=%d unsupported (multiple of %d, <= %d)", N, kRows, kMaxN);
-    return MCH_ERR_UNSUPPORTED;
+  if (max_batch < 1) {
+    mch::set_error("mch_shift_plan_create: max_batch=%d", max_batch);
+    return MCH_ERR_ARG;
   }
-  Radices rd;
-  rd.n = 0;
-  int n = N;
-  while (n % 4 == 0) { rd.r[rd.n++] = 4; n /= 4; }
-  while (n % 3 == 0) { rd.r[rd.n++] = 3; n /= 3; }
-  while (n % 2 == 0) { rd.r[rd.n++] = 2; n /= 2; }
-  if (n != 1 || rd.n > kMaxRad) {
-    mch::set_error("mch_shift_plan_create: N=%d is not 2^a 3^b", N);
-    return MCH_ERR_UNSUPPORTED;
+  int rc = MCH_OK;
+  switch (N) {
+    case 16: rc = configure_kernels<16>(); break;
+    case 32: rc = configure_kernels<32>(); break;
+    case 48: rc = configure_kernels<48>(); break;
+    case 64: rc = configure_kernels<64>(); break;
+    case 96: rc = configure_kernels<96>(); break;
+    case 128: rc = configure_kernels<128>(); break;
+    default:
+      mch::set_error("mch_shift_plan_create: box N=%d unsupported (16, 32, 48, 64, 96 or 128)", N);
+      return MCH_ERR_UNSUPPORTED;
   }
-  // FFT tiling limit of the row passes (kRows lines of N, radix >= 2)
-  if (kRows * N / 2 > kRowThreads * kMaxIt) {
-    mch::set_error("mch_shift_plan_create: N=%d too large for the row FFT tiling", N);
-    return MCH_ERR_UNSUPPORTED;
-  }
-  const size_t plane_lds = ((size_t)N * N + N) * sizeof(float2);
-  int dev = 0, maxlds = 0;
-  int rc = mch::hip_check(hipGetDevice(&dev), "hipGetDevice");
   if (rc) return rc;
-  rc = mch::hip_check(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
-                      "hipDeviceGetAttribute");
-  if (rc) return rc;
-  if ((size_t)maxlds < plane_lds + 1024) {
-    mch::set_error("mch_shift_plan_create: a %dx%d plane needs %zu B of LDS, device allows %d", N, N,
-                   plane_lds, maxlds);
-    return MCH_ERR_UNSUPPORTED;
-  }
-  if (plane_lds > 64 * 1024) {
-    rc = mch::hip_check(hipFuncSetAttribute((const void*)shift_yz_kernel,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_lds),
-                        "hipFuncSetAttribute(shift_yz_kernel)");
-    if (rc) return rc;
-  }
   ShiftPlan* p = new ShiftPlan();
   memset(p, 0, sizeof(*p));
   p->N = N;
   p->H = N / 2 + 1;
   p->max_batch = max_batch;
-  p->rad = rd;
   p->radius = box_radius;
   p->width = 5.0f;
   p->radius_p = box_radius + 5.0f;
-  p->plane_lds = plane_lds;
-  // sum of background weights: |outer| + sum(raised cosine)
+  p->ntile_x = N;
+  const int H = p->H;
+  // background weight per integer r2 (monotone: 0, shell, 1) and its table
+  const int r2max = 3 * (N / 2) * (N / 2);
+  std::vector<float> wr2(r2max + 1);
+  int lo = r2max + 1, hi = -1;
+  for (int r2 = 0; r2 <= r2max; ++r2) {
+    wr2[r2] = bg_weight_host(r2, p->radius, p->radius_p, p->width);
+    if (wr2[r2] != 0.f && lo > r2max) lo = r2;
+    if (wr2[r2] != 1.f) hi = r2;
+  }
+  if (lo > hi) { lo = hi + 1; }  // no shell: everything 0 below lo, 1 above hi
+  p->lut_lo = lo;
+  p->lut_n = hi >= lo ? hi - lo + 1 : 0;
+  if (p->lut_n > kMaxLut) {
+    mch::set_error("mch_shift_plan_create: cosine mask shell of %d radii exceeds %d", p->lut_n, kMaxLut);
+    delete p;
+    return MCH_ERR_UNSUPPORTED;
+  }
+  std::vector<float> lut(p->lut_n > 0 ? p->lut_n : 1, 0.f);
+  for (int i = 0; i < p->lut_n; ++i) lut[i] = wr2[lo + i];
+  // weights of the rolled patch (offset (i + N/2) mod N - N/2), their sum and
+  // x-spectrum XM[kx][z][y] = sum_x w(z, y, x) e^{-2 pi i kx x / N}
+  std::vector<float2> xm((size_t)H * N * N);
   {
+    std::vector<double> c(N), s(N);
+    for (int k = 0; k < N; ++k) {
+      c[k] = cos(-2.0 * M_PI * k / N);
+      s[k] = sin(-2.0 * M_PI * k / N);
+    }
+    std::vector<float> row(N);
     double wsum = 0.0;
     for (int z = 0; z < N; ++z)
-      for (int y = 0; y < N; ++y)
+      for (int y = 0; y < N; ++y) {
+        const int d0 = (z + N / 2) % N - N / 2, d1 = (y + N / 2) % N - N / 2;
         for (int x = 0; x < N; ++x) {
-          const double r = sqrt((double)((z - N / 2) * (z - N / 2) + (y - N / 2) * (y - N / 2) +
-                                         (x - N / 2) * (x - N / 2)));
-          if (r < p->radius) continue;
-          if (r > p->radius_p) wsum += 1.0;
-          else wsum += (float)(0.5 + 0.5 * cos(M_PI * (p->radius_p - r) / 5.0));
+          const int d2 = (x + N / 2) % N - N / 2;
+          row[x] = wr2[d0 * d0 + d1 * d1 + d2 * d2];
+          wsum += row[x];
         }
+        for (int kx = 0; kx < H; ++kx) {
+          double re = 0.0, im = 0.0;
+          for (int x = 0; x < N; ++x) {
+            if (row[x] == 0.f) continue;
+            const int j = (kx * x) % N;
+            re += row[x] * c[j];
+            im += row[x] * s[j];
+          }
+          xm[((size_t)kx * N + z) * N + y] = make_float2((float)re, (float)im);
+        }
+      }
     p->wsum = (float)wsum;
   }
-  p->ntile_x = N * (N / kRows);
   std::vector<float2> tw(N);
   for (int k = 0; k < N; ++k) {
     const double a = -2.0 * M_PI * (double)k / (double)N;
     tw[k] = make_float2((float)cos(a), (float)sin(a));
   }
-  const size_t vol = (size_t)N * N * N;
-  const size_t half = (size_t)N * N * p->H;
-  std::vector<float4> pair(vol);
+  const size_t half = (size_t)N * N * H;
+  // pair table: column xi = 0..N holds (ref[xi - 1], ref[xi]), zero outside the volume
+  const size_t npair = (size_t)N * N * (N + 1);
+  std::vector<float4> pair(npair);
   {
     const float2* r = (const float2*)ref_host;
-    for (size_t i = 0; i < vol; ++i) {
-      const float2 a = r[i];
-      const bool last = ((i % N) == (size_t)(N - 1));
-      const float2 b = last ? make_float2(0.f, 0.f) : r[i + 1];
-      pair[i] = make_float4(a.x, a.y, b.x, b.y);
-    }
+    const float2 zero = make_float2(0.f, 0.f);
+    for (size_t zy = 0; zy < (size_t)N * N; ++zy)
+      for (int xi = 0; xi <= N; ++xi) {
+        const float2 a = xi > 0 ? r[zy * N + xi - 1] : zero;
+        const float2 b = xi < N ? r[zy * N + xi] : zero;
+        pair[zy * (N + 1) + xi] = make_float4(a.x, a.y, b.x, b.y);
+      }
   }
 #define MCH_TRY(expr, what) do { if ((rc = mch::hip_check((expr), what)) != MCH_OK) goto fail; } while (0)
-  MCH_TRY(hipMalloc(&p->refpair, vol * sizeof(float4)), "hipMalloc(refpair)");
-  MCH_TRY(hipMemcpy(p->refpair, pair.data(), vol * sizeof(float4), hipMemcpyHostToDevice), "copy ref");
+  MCH_TRY(hipMalloc(&p->refpair, npair * sizeof(float4)), "hipMalloc(refpair)");
+  MCH_TRY(hipMemcpy(p->refpair, pair.data(), npair * sizeof(float4), hipMemcpyHostToDevice), "copy ref");
   MCH_TRY(hipMalloc(&p->twid, N * sizeof(float2)), "hipMalloc(twid)");
   MCH_TRY(hipMemcpy(p->twid, tw.data(), N * sizeof(float2), hipMemcpyHostToDevice), "copy twid");
+  MCH_TRY(hipMalloc(&p->lut, lut.size() * sizeof(float)), "hipMalloc(lut)");
+  MCH_TRY(hipMemcpy(p->lut, lut.data(), lut.size() * sizeof(float), hipMemcpyHostToDevice), "copy lut");
+  MCH_TRY(hipMalloc(&p->XM, half * sizeof(float2)), "hipMalloc(XM)");
+  MCH_TRY(hipMemcpy(p->XM, xm.data(), half * sizeof(float2), hipMemcpyHostToDevice), "copy XM");
   MCH_TRY(hipMalloc(&p->T, (size_t)max_batch * half * sizeof(float2)), "hipMalloc(T)");
-  MCH_TRY(hipMalloc(&p->rnorm_part, (size_t)max_batch * p->H * sizeof(float)), "hipMalloc");
-  MCH_TRY(hipMalloc(&p->snorm_part, (size_t)max_batch * p->H * sizeof(float)), "hipMalloc");
+  MCH_TRY(hipMalloc(&p->rnorm_part, (size_t)max_batch * H * sizeof(float)), "hipMalloc");
+  MCH_TRY(hipMalloc(&p->snorm_part, (size_t)max_batch * H * sizeof(float)), "hipMalloc");
   MCH_TRY(hipMalloc(&p->bg_part, (size_t)max_batch * N * sizeof(float)), "hipMalloc");
-  MCH_TRY(hipMalloc(&p->bg_avg, (size_t)max_batch * sizeof(float)), "hipMalloc");
   MCH_TRY(hipMalloc(&p->amax_val, (size_t)max_batch * p->ntile_x * sizeof(float)), "hipMalloc");
   MCH_TRY(hipMalloc(&p->amax_idx, (size_t)max_batch * p->ntile_x * sizeof(int)), "hipMalloc");
 #undef MCH_TRY
   *out_plan = p;
   return MCH_OK;
 fail:
-  (void)hipFree(p->refpair); (void)hipFree(p->twid); (void)hipFree(p->T);
-  (void)hipFree(p->rnorm_part); (void)hipFree(p->snorm_part); (void)hipFree(p->bg_part);
-  (void)hipFree(p->bg_avg); (void)hipFree(p->amax_val); (void)hipFree(p->amax_idx);
-  delete p;
+  free_plan(p);
   return rc;
 }
 
 int mch_shift_plan_destroy(void* plan) {
-  ShiftPlan* p = (ShiftPlan*)plan;
-  if (!p) return MCH_OK;
-  (void)hipFree(p->refpair); (void)hipFree(p->twid); (void)hipFree(p->T);
-  (void)hipFree(p->rnorm_part); (void)hipFree(p->snorm_part); (void)hipFree(p->bg_part);
-  (void)hipFree(p->bg_avg); (void)hipFree(p->amax_val); (void)hipFree(p->amax_idx);
-  delete p;
+  free_plan((ShiftPlan*)plan);
   return MCH_OK;
 }
 
@@ -702,31 +806,23 @@ int mch_shift_search(void* plan, const float* subtomos, long long ld_vol, int B,
     mch::set_error("mch_shift_search: NULL argument");
     return MCH_ERR_ARG;
   }
-  const int N = p->N, H = p->H;
+  const int N = p->N;
   if (B < 1 || B > p->max_batch || ld_vol < (long long)N * N * N) {
     mch::set_error("mch_shift_search: bad sizes (B=%d max %d)", B, p->max_batch);
     return MCH_ERR_SIZE;
   }
-  int rc;
-  hipLaunchKernelGGL(shift_bg_kernel, dim3(N, B), dim3(256), 0, stream, subtomos, ld_vol, N,
-                     shift_zyx, p->radius, p->radius_p, p->width, p->bg_part);
-  if ((rc = mch::hip_check(hipGetLastError(), "shift_bg_kernel"))) return rc;
-  hipLaunchKernelGGL(shift_bg_final_kernel, dim3((B + 63) / 64), dim3(64), 0, stream, p->bg_part, N,
-                     B, p->wsum, p->bg_avg);
-  if ((rc = mch::hip_check(hipGetLastError(), "shift_bg_final_kernel"))) return rc;
-  hipLaunchKernelGGL(shift_fwd_x_kernel, dim3(N / kRows, N, B), dim3(kRowThreads), 0, stream,
-                     subtomos, ld_vol, N, H, shift_zyx, p->radius, p->radius_p, p->width, p->bg_avg,
-                     p->rad, p->twid, p->T);
-  if ((rc = mch::hip_check(hipGetLastError(), "shift_fwd_x_kernel"))) return rc;
-  hipLaunchKernelGGL(shift_yz_kernel, dim3(H, B), dim3(kPlaneThreads), p->plane_lds, stream, p->T,
-                     p->refpair, N, H, rot, ctf, p->rad, p->twid, p->rnorm_part, p->snorm_part);
-  if ((rc = mch::hip_check(hipGetLastError(), "shift_yz_kernel"))) return rc;
-  hipLaunchKernelGGL(shift_inv_x_kernel, dim3(N / kRows, N, B), dim3(kRowThreads), 0, stream, p->T,
-                     N, H, p->rad, p->twid, p->amax_val, p->amax_idx, p->ntile_x);
-  if ((rc = mch::hip_check(hipGetLastError(), "shift_inv_x_kernel"))) return rc;
-  hipLaunchKernelGGL(shift_final_kernel, dim3(B), dim3(256), 0, stream, p->amax_val, p->amax_idx,
-                     p->ntile_x, p->rnorm_part, p->snorm_part, H, N, out_shift, out_score);
-  return mch::hip_check(hipGetLastError(), "shift_final_kernel");
+  const SearchArgs a{subtomos, ld_vol, B, shift_zyx, rot, ctf, out_shift, out_score};
+  switch (N) {
+    case 16: return launch_search<16>(p, a, stream);
+    case 32: return launch_search<32>(p, a, stream);
+    case 48: return launch_search<48>(p, a, stream);
+    case 64: return launch_search<64>(p, a, stream);
+    case 96: return launch_search<96>(p, a, stream);
+    case 128: return launch_search<128>(p, a, stream);
+    default: break;
+  }
+  mch::set_error("mch_shift_search: plan with unsupported N=%d", N);
+  return MCH_ERR_UNSUPPORTED;
 }
 
 }  // extern "C"
