@@ -1,7 +1,13 @@
 """Benchmark: aligned volumes/s for Matcha's orientation-search hot path on one MI355X.
 
-A step = one batch of synthetic particles aligned on the GPU by two
-rotation/shift alternations (matcha_amd.align.Alternation), each:
+Headline (the metric's own definition): the tuned path -- projection GEMM on
+16-bit hi/lo split operands (three MFMA products, FP32 accumulation) -- against
+the plain FP32 path, both measured in this run on the same synthetic particles,
+each required to reach <= 1 deg median geodesic error.  ``value`` is the tuned
+path's throughput; the plain path's throughput and the ratio are in ``config``.
+
+A step = one batch of particles aligned on the GPU by two rotation/shift
+alternations (matcha_amd.align.Alternation), each:
   CrossCorrelationMatcher.get_sigma  (ball-harmonic projection + sigma)
   Matcha.search_orientations         (SOFFT grid + frequency-marched Newton)
   ShiftMatcher.search_shifts         (rotated-reference FFT cross-correlation)
@@ -44,9 +50,12 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="box96_L32", choices=sorted(WORKLOADS))
-    p.add_argument("--batch", type=int, default=None)
+    p.add_argument("--batch", type=int, default=None, help="plain-path batch (default: the workload's)")
+    p.add_argument("--tuned-batch", type=int, default=None, help="tuned-path batch (default: the workload's)")
+    p.add_argument("--tuned-precision", default="fp16x3", choices=["bf16x3", "fp16x3"])
+    p.add_argument("--no-plain", action="store_true", help="skip the plain FP32 path")
     p.add_argument("--snr", type=float, default=0.0)
-    p.add_argument("--pool", type=int, default=4, help="distinct particle batches cycled")
+    p.add_argument("--pool", type=int, default=256, help="distinct particles cycled through")
     p.add_argument("--alternations", type=int, default=2)
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -83,6 +92,80 @@ def cpu_baseline(w, args):
     }
 
 
+def run_path(name, torch, dev, w, args, pool_vols, pool_R, pool_t, precision, B, ccm, matcher, sm):
+    """Throughput, accuracy and per-stage times of one path."""
+    from matcha_amd.align import Alternation
+    from matcha_amd.core.ShiftMatcher import euler_to_shift_rotations
+    from matcha_amd.synthetic import euler_zyz_to_matrix, geodesic_deg
+    ccm.fle.set_compute_precision(precision)
+    pipe = Alternation(ccm, matcher, sm, num_alternations=args.alternations)
+    P = pool_vols.shape[0]
+    nb = max(1, P // B)
+    batches = [pool_vols[i * B:(i + 1) * B] for i in range(nb)]
+    for i in range(args.warmup):
+        pipe.run(batches[i % nb])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        pipe.run(batches[i % nb])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    vps = args.steps * B / elapsed
+    # accuracy over the pool (untimed)
+    errs, shift_ok = [], []
+    for i in range(nb):
+        a, b, g, ns, sh, ss = pipe.run(batches[i])
+        a, b, g = (x.cpu().numpy().astype(np.float64) for x in (a, b, g))
+        R = pool_R[i * B:(i + 1) * B]
+        t = pool_t[i * B:(i + 1) * B]
+        errs.append(geodesic_deg(R, euler_zyz_to_matrix(a, b, g)))
+        shift_ok.append(np.all(sh.cpu().numpy() == t[:, ::-1], axis=1))
+    errs = np.concatenate(errs)
+    shift_ok = np.concatenate(shift_ok)
+    # per-stage time of one alternation (HIP events on the compute stream)
+    stream = torch.cuda.current_stream(dev)
+    vols0 = batches[0]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(stream)
+    sigma, norms = ccm.get_sigma(vols0)
+    ev[1].record(stream)
+    a0, b0, g0, s0 = matcher.search_orientations(sigma)
+    ev[2].record(stream)
+    sm.search_shifts(subtomos=vols0, rotation_tracker=euler_to_shift_rotations(a0, b0, g0))
+    ev[3].record(stream)
+    torch.cuda.synchronize()
+    return {
+        "precision": precision, "batch": B, "volumes_per_s": vps, "ms_per_step": elapsed / args.steps * 1e3,
+        "median_err_deg": float(np.median(errs)), "p90_err_deg": float(np.percentile(errs, 90)),
+        "shift_exact_frac": float(shift_ok.mean()), "particles_checked": int(errs.size),
+        "stage_ms_per_batch": {"get_sigma": ev[0].elapsed_time(ev[1]),
+                               "search_orientations": ev[1].elapsed_time(ev[2]),
+                               "search_shifts": ev[2].elapsed_time(ev[3])},
+    }
+
+
+def time_gemm(torch, dev, ccm, vols, precision, reps=20):
+    """Average duration of the projection GEMM launch (HIP events on its stream)."""
+    from matcha_amd import _lib
+    fle_ = ccm.fle
+    fle_.set_compute_precision(precision)
+    fle_.evaluate_t(vols)
+    B = vols.shape[0]
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        _lib.check(_lib.lib().mch_fle_gemm_only(fle_._plan, B, _lib.stream_handle(dev)), "gemm")
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def _lib_depth(fle_):
+    from matcha_amd import _lib
+    return _lib.lib().mch_fle_tuned_depth(fle_._plan)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -91,116 +174,87 @@ def main():
         sys.exit(2)
     import torch
 
-    from matcha_amd import _lib
     from matcha_amd.core.CrossCorrelationMatcher import CrossCorrelationMatcher
-    from matcha_amd.align import Alternation
     from matcha_amd.core.Matcha import Matcha
     from matcha_amd.core.ShiftMatcher import ShiftMatcher
-    from matcha_amd.synthetic import Workload, euler_zyz_to_matrix, geodesic_deg
+    from matcha_amd.synthetic import Workload
 
     w = dict(WORKLOADS[args.workload])
-    if args.batch:
-        w["batch"] = args.batch
-    B = w["batch"]
+    Bp = args.batch or w["batch"]
+    Bt = args.tuned_batch or w["batch"]
+    Bmax = max(Bp, Bt)
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     t_setup = time.perf_counter()
     wl = Workload(w["N"], w["n_blobs"], w["sigma"], args.seed, dev, mask_radius=w["radius"])
-    ccm = CrossCorrelationMatcher(w["N"], dev, batchsize=B, bandlimit=w["bandlimit"], radius=w["radius"])
+    ccm = CrossCorrelationMatcher(w["N"], dev, batchsize=Bmax, bandlimit=w["bandlimit"], radius=w["radius"])
     ccm.set_template(wl.template)
-    matcher = Matcha(B, dev, w["L_max"], {"Li": w["Li"], "candidates": w["C"], "num_steps": w["steps"],
-                                          "oversampling_factor_K": w["K"]})
-    sm = ShiftMatcher({"N": w["N"], "num_subtomograms_per_batch": B, "shift_upsample_factor": 1,
+    matcher = Matcha(Bmax, dev, w["L_max"], {"Li": w["Li"], "candidates": w["C"], "num_steps": w["steps"],
+                                             "oversampling_factor_K": w["K"]})
+    sm = ShiftMatcher({"N": w["N"], "num_subtomograms_per_batch": Bmax, "shift_upsample_factor": 1,
                        "shift_search_radius": w["max_shift"], "voxel_size": 1.0}, dev)
     sm.set_reference(wl.template.cpu().numpy())
-    pipe = Alternation(ccm, matcher, sm, num_alternations=args.alternations)
-    pool = [wl.particles(B, w["max_shift"], args.snr) for _ in range(max(1, args.pool))]
+    P = max(args.pool, Bmax)
+    chunks = [wl.particles(min(64, P - i), w["max_shift"], args.snr) for i in range(0, P, 64)]
+    pool_vols = torch.cat([c[0] for c in chunks])
+    pool_R = np.concatenate([c[1] for c in chunks])
+    pool_t = np.concatenate([c[2] for c in chunks])
+    ccm.fle.set_compute_precision(args.tuned_precision)  # builds the 16-bit tables (setup)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
 
-    def step(i):
-        return pipe.run(pool[i % len(pool)][0])
+    tuned = run_path("tuned", torch, dev, w, args, pool_vols, pool_R, pool_t, args.tuned_precision, Bt,
+                     ccm, matcher, sm)
+    plain = None
+    if not args.no_plain:
+        plain = run_path("plain", torch, dev, w, args, pool_vols, pool_R, pool_t, "fp32", Bp, ccm, matcher, sm)
 
-    for i in range(args.warmup):
-        step(i)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step(i)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    vps = args.steps * B / elapsed
-
-    # accuracy over the pool (untimed)
-    errs, shift_ok = [], []
-    for vols, R, t in pool:
-        a, b, g, ns, sh, ss = pipe.run(vols)
-        a, b, g = (x.cpu().numpy().astype(np.float64) for x in (a, b, g))
-        errs.append(geodesic_deg(R, euler_zyz_to_matrix(a, b, g)))
-        shift_ok.append(np.all(sh.cpu().numpy() == t[:, ::-1], axis=1))
-    errs = np.concatenate(errs)
-    shift_ok = np.concatenate(shift_ok)
-
-    # per-stage time of one alternation (HIP events on the compute stream)
-    stream = torch.cuda.current_stream(dev)
-    vols0 = pool[0][0]
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    ev[0].record(stream)
-    sigma, norms = ccm.get_sigma(vols0)
-    ev[1].record(stream)
-    a0, b0, g0, s0 = matcher.search_orientations(sigma)
-    ev[2].record(stream)
-    from matcha_amd.core.ShiftMatcher import euler_to_shift_rotations
-    sm.search_shifts(subtomos=vols0, rotation_tracker=euler_to_shift_rotations(a0, b0, g0))
-    ev[3].record(stream)
-    torch.cuda.synchronize()
-    stage_ms = {"get_sigma": ev[0].elapsed_time(ev[1]), "search_orientations": ev[1].elapsed_time(ev[2]),
-                "search_shifts": ev[2].elapsed_time(ev[3])}
-
-    # dominant kernel: the projection GEMM, timed alone with HIP events on its stream
+    # dominant kernel of the tuned path: the projection GEMM.  It streams the
+    # 16-bit W^T hi/lo once per launch, so HBM bounds it: algorithmic bytes =
+    # W^T hi + lo (4 B per element) + the FP32 X operand, each read once.
     fle_ = ccm.fle
-    ccm.fle.evaluate_t(pool[0][0])
-    reps = 20
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        _lib.check(_lib.lib().mch_fle_gemm_only(fle_._plan, B, _lib.stream_handle(dev)), "gemm")
-    e1.record(stream)
-    torch.cuda.synchronize()
-    gemm_s = e0.elapsed_time(e1) / 1e3 / reps
-    flops = 2.0 * (2 * B) * (2 * fle_.V0) * sum(fle_.class_cols)
-    achieved = flops / gemm_s / 1e12
-    peak = 157.3
+    gemm_t = time_gemm(torch, dev, ccm, pool_vols[:Bt], args.tuned_precision)
+    gemm_plain = time_gemm(torch, dev, ccm, pool_vols[:Bp], "fp32") if plain else None
+    w_bytes = 4.0 * fle_.Ntot * fle_.Kpad          # W^T hi + lo, 2 + 2 B per element
+    x_bytes = 4.0 * 8 * (2 * Bt) * fle_.Kpad       # FP32 X operand of the 8 classes
+    alg_bytes = w_bytes + x_bytes
+    achieved = alg_bytes / gemm_t / 1e9
+    flops16 = 3 * 2.0 * (2 * Bt) * (2 * fle_.V0) * sum(fle_.class_cols)
+    fle_.set_compute_precision(args.tuned_precision)
+
+    dtype = {"bf16x3": "bf16x3+f32acc", "fp16x3": "fp16x3+f32acc"}[args.tuned_precision]
     out = {
         "metric": "aligned volumes/s, box 96 L=32, at <=1 deg median error: tuned vs plain FP32",
-        "value": vps,
+        "value": tuned["volumes_per_s"],
         "unit": "volumes/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": tuned["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": dtype,
         "data": "synthetic (Gaussian-blob template, Haar rotations, integer shifts, AWGN)",
         "config": {
             "workload": (f"{args.workload}: box {w['N']}, {w['n_blobs']} blobs, FLE bandlimit "
                          f"{w['bandlimit']} radius {w['radius']}, march {w['Li']}->{w['L_max']}, "
                          f"grid {2 * min(w['Li']) * w['K']}^3, {w['C']} candidates, {w['steps']} Newton "
-                         f"steps, batch {B}, SNR {args.snr} dB, plain FP32 path"),
+                         f"steps, SNR {args.snr} dB, {P} particles cycled"),
             "path": f"{args.alternations} x (get_sigma + search_orientations + search_shifts)",
-            "stage_ms_per_batch": stage_ms,
-            "median_err_deg": float(np.median(errs)),
-            "shift_exact_frac": float(shift_ok.mean()),
-            "p90_err_deg": float(np.percentile(errs, 90)),
-            "particles_checked": int(errs.size),
+            "tuned": tuned,
+            "plain": plain,
+            "tuned_over_plain": (tuned["volumes_per_s"] / plain["volumes_per_s"]) if plain else None,
+            "meets_accuracy": tuned["median_err_deg"] <= 1.0 and (plain is None or plain["median_err_deg"] <= 1.0),
+            "gemm_ms": {"tuned": gemm_t * 1e3, "plain_fp32": gemm_plain * 1e3 if gemm_plain else None},
+            "gemm_pipeline_depth": int(_lib_depth(fle_)),
             "setup_s": t_setup,
         },
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": "fle_gemm_kernel", "avg_ms": gemm_s * 1e3,
-                     "flops_per_launch": flops},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": None,
+                     "kernel": f"fle_gemm16_kernel<{args.tuned_precision}>", "avg_ms": gemm_t * 1e3,
+                     "bytes_per_launch": alg_bytes,
+                     "mfma_tflops": flops16 / gemm_t / 1e12, "mfma_peak_tflops": 2500.0},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args)

@@ -111,6 +111,20 @@ int mch_fle_project(void* plan, const float* vols, long long ld_vol, int B, void
  * (timing aid; no reference counterpart). */
 int mch_fle_gemm_only(void* plan, int B, hipStream_t stream);
 
+/* Compute precision of the projection GEMM: 0 = FP32 MFMA (the plain path,
+ * the default), 1 = BF16 hi/lo split operands, 2 = FP16 hi/lo split with
+ * power-of-2 row/column scales; the 16-bit modes take three MFMA products
+ * (hi*hi + hi*lo + lo*hi) with FP32 accumulation.  The reference runs this
+ * projection in FP32/complex64 (FLEBasis3D.py:945-976); the reference's own
+ * precision_mode (:39-109) selects the expansion accuracy, not this. */
+int mch_fle_set_precision(void* plan, int precision);
+
+/* Tile configuration the 16-bit GEMM was autotuned to when the plan first
+ * switched to a 16-bit mode: 10 * (waves per workgroup, 2 or 4) + pipeline
+ * depth (2 or 3 LDS stages); 0 before that.  No reference counterpart
+ * (reporting aid). */
+int mch_fle_tuned_depth(void* plan);
+
 /* Index table (lmax+1, 2*lmax+1, kdim) of coefficient positions for
  * (l, m, k) (-1 = absent).  Replaces CrossCorrelationMatcher._precompute_indices
  * (src/matcha/core/CrossCorrelationMatcher.py:133-181). */

@@ -42,7 +42,8 @@ class CrossCorrelationMatcher:
     def __init__(self, N: int, device: torch.device, expansion_epsilon: float = 1e-4,
                  batchsize: int = 1, reduce_memory: bool = True, bandlimit: int = None,
                  micro_batch_split: int = 2, dtype=torch.float32, radius: int = None,
-                 jl_zeros_path: str = None, cs_path: str = None, precision_mode: str = "accurate"):
+                 jl_zeros_path: str = None, cs_path: str = None, precision_mode: str = "accurate",
+                 compute_precision: str = "fp32"):
         self.N = N
         self.fle_basis_eps = expansion_epsilon
         self.batchsize = batchsize
@@ -65,7 +66,7 @@ class CrossCorrelationMatcher:
                               reduce_memory=reduce_memory, batchsize=self.batchsize,
                               radius=self.radius, device=self.device, dtype=self.dtype,
                               jl_zeros_path=jl_zeros_path, cs_path=cs_path,
-                              precision_mode=precision_mode)
+                              precision_mode=precision_mode, compute_precision=compute_precision)
         self.lmax = self.fle.lmax
         self._make_corr_plan()
         self.sigma = torch.zeros((self.batchsize, compute_size(self.lmax + 1)), dtype=self.dtype,

@@ -34,6 +34,7 @@ from scipy import optimize, special
 from .. import _lib
 
 N_CLASSES = 8  # (l+m) mod 2 in {0,1} x m mod 4 in {0,1,2,3}
+COL_PAD = 128  # class column counts are padded to the GEMM column tile
 
 
 # ---------------------------------------------------------------------------
@@ -195,7 +196,7 @@ def projection_tables(N: int, radius: float, spectrum, lmax: int, BK: int = 16):
     K = 2 * V0
     Kpad = ((K + BK - 1) // BK) * BK
     cols = [len(c) for c in classes]
-    colpad = [((n + 63) // 64) * 64 for n in cols]
+    colpad = [((n + COL_PAD - 1) // COL_PAD) * COL_PAD for n in cols]
     Ntot = int(sum(colpad))
     Wb = np.zeros((Kpad, Ntot), dtype=np.float32)
     chan_plus = np.full(Ntot, -1, dtype=np.int32)
@@ -229,15 +230,21 @@ def projection_tables(N: int, radius: float, spectrum, lmax: int, BK: int = 16):
                 chan_minus=chan_minus, chan_m=chan_m)
 
 
-class FLEBasis3D:
-    """Same constructor as the reference (FLEBasis3D.py:39-109)."""
+# compute precision of the projection GEMM (mch_fle_set_precision)
+COMPUTE_PRECISIONS = {"fp32": 0, "bf16x3": 1, "fp16x3": 2}
 
-    BK = 16  # K padding of the GEMM
+
+class FLEBasis3D:
+    """Same constructor as the reference (FLEBasis3D.py:39-109), plus
+    ``compute_precision``: "fp32" (plain FP32 MFMA GEMM), "bf16x3" or "fp16x3"
+    (16-bit hi/lo split operands, three MFMA products, FP32 accumulation)."""
+
+    BK = 64  # K padding of the GEMM (the 16-bit path splits W^T in 64 x 64 tiles)
 
     def __init__(self, N, bandlimit, eps, maxitr=None, maxfun=None, max_l=None, mode="complex",
                  sph_harm_solver="nvidia_torch", reduce_memory=True, batchsize=1, radius=1,
                  device="cuda", dtype=torch.complex128, jl_zeros_path=None, cs_path=None,
-                 precision_mode="accurate"):
+                 precision_mode="accurate", compute_precision="fp32"):
         if mode != "complex":
             raise ValueError("only mode='complex' is supported")
         if maxfun or max_l:
@@ -267,6 +274,17 @@ class FLEBasis3D:
         self._build_tables()
         self._plan = None
         self._make_plan()
+        self.compute_precision = "fp32"
+        self.set_compute_precision(compute_precision)
+
+    def set_compute_precision(self, mode: str):
+        """Switch the projection GEMM between "fp32", "bf16x3" and "fp16x3"."""
+        if mode not in COMPUTE_PRECISIONS:
+            raise ValueError(f"compute_precision must be one of {sorted(COMPUTE_PRECISIONS)}, got {mode!r}")
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().mch_fle_set_precision(self._plan, COMPUTE_PRECISIONS[mode]),
+                       "mch_fle_set_precision")
+        self.compute_precision = mode
 
     # -- tables ---------------------------------------------------------------
     def _build_tables(self):

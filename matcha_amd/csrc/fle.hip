@@ -4,17 +4,24 @@
 //   FLEBasis3D.evaluate_t      src/matcha/core/FLEBasis3D.py:945-976,1026-1267
 //   compute_sigma / get_sigma  src/matcha/core/CrossCorrelationMatcher.py:183-249
 //
-// Pipeline per batch (three launches):
+// Pipeline per batch:
 //   1. fle_symmetrize_kernel: each fundamental voxel's 16 orbit values ->
 //      4 combinations per class (8 classes), written as the GEMM A operand
 //      X[class][row][k] with rows (b: real part, B+b: imaginary part) and
 //      k = 2*v0 + {0: cos-weighted, 1: sin-weighted}.
-//   2. fle_gemm_kernel: grouped FP32 MFMA GEMM (v_mfma_f32_16x16x4f32),
-//      Out[row][col] = sum_k X[class(col)][row][k] W[k][col], split over K.
-//   3. fle_finalize_kernel: sum the K splits, a_{lkm} = Re + i Im and
-//      a_{lk,-m} = (-1)^m conj(a_{lkm}), scattered to the reference order.
+//   2. the projection GEMM Out[row][col] = sum_k X[class(col)][row][k] W[k][col],
+//      split over K, in the plan's precision:
+//        FP32 (plain path): fle_gemm_kernel, v_mfma_f32_16x16x4f32;
+//        BF16x3 / FP16x3 (tuned path): fle_gemm16_kernel on 16-bit hi/lo
+//        operands, Xhi Whi + Xhi Wlo + Xlo Whi with FP32 accumulation
+//        (v_mfma_f32_16x16x32_{bf16,f16}); BF16 X comes split out of step 1,
+//        FP16 X is split by fle_xsplit_kernel with power-of-2 row scales.
+//   3. fle_finalize_kernel: sum the K splits (undo the FP16 scales),
+//      a_{lkm} = Re + i Im and a_{lk,-m} = (-1)^m conj(a_{lkm}), scattered
+//      to the reference order.
 #include "common.hpp"
 
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -22,6 +29,12 @@ namespace {
 constexpr int kClasses = 8;
 constexpr int BM = 128, BN = 64, BK = 16;
 constexpr int kGemmThreads = 256;
+
+// 16-bit split GEMM (tuned path): tiles of 128 rows x 128 columns x 32 k
+constexpr int BM16 = 128, BN16 = 128, BK16 = 32, LDA16 = BK16 + 8;
+
+// compute precision of the projection GEMM
+enum FlePrecision { kFp32 = 0, kBf16x3 = 1, kFp16x3 = 2 };
 
 struct FlePlan {
   int N, V0, Kpad, Ntot, ne, max_batch;
@@ -34,6 +47,18 @@ struct FlePlan {
   float* X;                   // (kClasses, Mpad, Kpad)
   float* part;                // (splits, Mpad, Ntot)
   int Mpad, splits;
+  // tuned path: W^T split into 16-bit hi + lo with a power-of-2 scale per column
+  int precision;              // FlePrecision
+  int table_precision;        // precision the 16-bit tables below were built for (0: none)
+  uint16_t* Xhi;              // (kClasses, Mpad, Kpad) 16-bit split X operand
+  uint16_t* Xlo;              // (kClasses, Mpad, Kpad)
+  int nst16;                  // pipeline depth of the 16-bit GEMM (autotuned: 2 or 3)
+  int nw16;                   // waves per workgroup of the 16-bit GEMM (autotuned: 2 or 4)
+  uint16_t* Wt16;             // W^T hi/lo in DMA order, Ntot * Kpad * 2 (see fle_wsplit_kernel)
+  float* colscale;            // (Ntot)
+  float* rowmax;              // (Mpad) max |X| of each GEMM row (FP16 scaling)
+  float* part16;              // (splits16, Mpad, Ntot)
+  int splits16;
 };
 
 struct CorrPlan {
@@ -42,6 +67,47 @@ struct CorrPlan {
 };
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+// v = hi + lo with hi, lo 16-bit (round to nearest even), and the 16x16x32 MFMA
+// of that format.  BF16 keeps the FP32 exponent range; FP16 needs scaled data.
+template <bool FP16>
+struct Split16;
+template <>
+struct Split16<false> {
+  static __device__ __forceinline__ void split(float v, uint16_t& hi, uint16_t& lo) {
+    const __bf16 h = (__bf16)v;
+    hi = __builtin_bit_cast(uint16_t, h);
+    lo = __builtin_bit_cast(uint16_t, (__bf16)(v - (float)h));
+  }
+  static __device__ __forceinline__ floatx4 mfma(u32x4 a, u32x4 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+  }
+};
+template <>
+struct Split16<true> {
+  static __device__ __forceinline__ void split(float v, uint16_t& hi, uint16_t& lo) {
+    const _Float16 h = (_Float16)v;
+    hi = __builtin_bit_cast(uint16_t, h);
+    lo = __builtin_bit_cast(uint16_t, (_Float16)(v - (float)h));
+  }
+  static __device__ __forceinline__ floatx4 mfma(u32x4 a, u32x4 b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a),
+                                                  __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+  }
+};
+
+// Power-of-2 scale that brings max |x| = m just below 2^14 (FP16 range with
+// room for the FP32 accumulation of the products); 1 for m == 0.
+__host__ __device__ __forceinline__ float pow2_scale(float m) {
+  if (!(m > 0.f)) return 1.f;
+  int e;
+  frexpf(m, &e);
+  return ldexpf(1.f, 14 - e);
+}
 
 }  // namespace
 
@@ -50,59 +116,111 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // ---------------------------------------------------------------------------
 // For class (eps, rho): G_t = sum_q (-1)^{q eps} sum_p (-i)^{rho p} f[p,q,t],
 // X1 = A0+A1, X2 = B0-B1, X3 = B0+B1, X4 = A1-A0 with G_t = A_t + i B_t.
+// SPLIT = false: X in FP32; rowmax (Mpad floats, zeroed by the caller; NULL to
+// skip) receives max |X| of each GEMM row, for the power-of-2 row scale of the
+// FP16 split GEMM.  SPLIT = true: X written directly as BF16 hi/lo (Xhi, Xlo).
+template <bool SPLIT>
 __global__ void fle_symmetrize_kernel(const float* __restrict__ vols, long long ld_vol, int B,
                                       const int* __restrict__ orbit, int V0, int Kpad, int Mpad,
-                                      float* __restrict__ X) {
+                                      float* __restrict__ X, float* __restrict__ rowmax,
+                                      uint16_t* __restrict__ Xhi, uint16_t* __restrict__ Xlo) {
   const long long tot = (long long)B * V0;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < tot;
-       t += (long long)gridDim.x * blockDim.x) {
-    const int b = (int)(t / V0);
-    const int v = (int)(t - (long long)b * V0);
-    const float* f = vols + (long long)b * ld_vol;
-    const int4* o4 = reinterpret_cast<const int4*>(orbit + (long long)v * 16);
-    float g[16];
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  // block-uniform trip count so that the wave reductions see every lane
+  for (long long t0 = (long long)blockIdx.x * blockDim.x; t0 < tot; t0 += stride) {
+    const long long t = t0 + threadIdx.x;
+    const bool active = t < tot;
+    const int b = active ? (int)(t / V0) : 0;
+    const int v = active ? (int)(t - (long long)b * V0) : 0;
+    float mre = 0.f, mim = 0.f;
+    if (active) {
+      const float* f = vols + (long long)b * ld_vol;
+      const int4* o4 = reinterpret_cast<const int4*>(orbit + (long long)v * 16);
+      float g[16];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int4 id = o4[i];
-      g[4 * i + 0] = f[id.x];
-      g[4 * i + 1] = f[id.y];
-      g[4 * i + 2] = f[id.z];
-      g[4 * i + 3] = f[id.w];
-    }
-    // g[p + 4q + 8t]
-#pragma unroll
-    for (int eps = 0; eps < 2; ++eps) {
-      // H_t(q) sums over p for each rho, then combine q with (-1)^{q eps}
-      float A[2][4], Bi[2][4];  // [t][rho]
-#pragma unroll
-      for (int tt = 0; tt < 2; ++tt) {
-        float s0 = 0.f, s2 = 0.f, s1r = 0.f, s1i = 0.f;
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const float sg = (eps && q) ? -1.f : 1.f;
-          const float* h = g + 4 * q + 8 * tt;
-          // rho = 0: sum_p f ; rho = 2: sum (-1)^p f ; rho = 1: sum (-i)^p f
-          s0 += sg * (h[0] + h[1] + h[2] + h[3]);
-          s2 += sg * (h[0] - h[1] + h[2] - h[3]);
-          s1r += sg * (h[0] - h[2]);
-          s1i += sg * (h[3] - h[1]);
-        }
-        A[tt][0] = s0;  Bi[tt][0] = 0.f;
-        A[tt][1] = s1r; Bi[tt][1] = s1i;
-        A[tt][2] = s2;  Bi[tt][2] = 0.f;
-        A[tt][3] = s1r; Bi[tt][3] = -s1i;  // rho = 3: conjugate of rho = 1
+      for (int i = 0; i < 4; ++i) {
+        const int4 id = o4[i];
+        g[4 * i + 0] = f[id.x];
+        g[4 * i + 1] = f[id.y];
+        g[4 * i + 2] = f[id.z];
+        g[4 * i + 3] = f[id.w];
       }
+      // g[p + 4q + 8t]
 #pragma unroll
-      for (int rho = 0; rho < 4; ++rho) {
-        const int cls = eps * 4 + rho;
-        const float X1 = A[0][rho] + A[1][rho];
-        const float X2 = Bi[0][rho] - Bi[1][rho];
-        const float X3 = Bi[0][rho] + Bi[1][rho];
-        const float X4 = A[1][rho] - A[0][rho];
-        float* xr = X + ((long long)cls * Mpad + b) * Kpad + 2 * v;
-        float* xi = X + ((long long)cls * Mpad + B + b) * Kpad + 2 * v;
-        *reinterpret_cast<float2*>(xr) = make_float2(X1, X2);
-        *reinterpret_cast<float2*>(xi) = make_float2(X3, X4);
+      for (int eps = 0; eps < 2; ++eps) {
+        // H_t(q) sums over p for each rho, then combine q with (-1)^{q eps}
+        float A[2][4], Bi[2][4];  // [t][rho]
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt) {
+          float s0 = 0.f, s2 = 0.f, s1r = 0.f, s1i = 0.f;
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const float sg = (eps && q) ? -1.f : 1.f;
+            const float* h = g + 4 * q + 8 * tt;
+            // rho = 0: sum_p f ; rho = 2: sum (-1)^p f ; rho = 1: sum (-i)^p f
+            s0 += sg * (h[0] + h[1] + h[2] + h[3]);
+            s2 += sg * (h[0] - h[1] + h[2] - h[3]);
+            s1r += sg * (h[0] - h[2]);
+            s1i += sg * (h[3] - h[1]);
+          }
+          A[tt][0] = s0;  Bi[tt][0] = 0.f;
+          A[tt][1] = s1r; Bi[tt][1] = s1i;
+          A[tt][2] = s2;  Bi[tt][2] = 0.f;
+          A[tt][3] = s1r; Bi[tt][3] = -s1i;  // rho = 3: conjugate of rho = 1
+        }
+#pragma unroll
+        for (int rho = 0; rho < 4; ++rho) {
+          const int cls = eps * 4 + rho;
+          const float X1 = A[0][rho] + A[1][rho];
+          const float X2 = Bi[0][rho] - Bi[1][rho];
+          const float X3 = Bi[0][rho] + Bi[1][rho];
+          const float X4 = A[1][rho] - A[0][rho];
+          const long long orow = ((long long)cls * Mpad + b) * Kpad + 2 * v;
+          const long long oim = ((long long)cls * Mpad + B + b) * Kpad + 2 * v;
+          if constexpr (SPLIT) {
+            uint16_t h1, l1, h2, l2, h3, l3, h4, l4;
+            Split16<false>::split(X1, h1, l1);
+            Split16<false>::split(X2, h2, l2);
+            Split16<false>::split(X3, h3, l3);
+            Split16<false>::split(X4, h4, l4);
+            *reinterpret_cast<unsigned*>(Xhi + orow) = (unsigned)h1 | ((unsigned)h2 << 16);
+            *reinterpret_cast<unsigned*>(Xlo + orow) = (unsigned)l1 | ((unsigned)l2 << 16);
+            *reinterpret_cast<unsigned*>(Xhi + oim) = (unsigned)h3 | ((unsigned)h4 << 16);
+            *reinterpret_cast<unsigned*>(Xlo + oim) = (unsigned)l3 | ((unsigned)l4 << 16);
+          } else {
+            *reinterpret_cast<float2*>(X + orow) = make_float2(X1, X2);
+            *reinterpret_cast<float2*>(X + oim) = make_float2(X3, X4);
+          }
+          mre = fmaxf(mre, fmaxf(fabsf(X1), fabsf(X2)));
+          mim = fmaxf(mim, fmaxf(fabsf(X3), fabsf(X4)));
+        }
+      }
+    }
+    if (rowmax) {
+      // consecutive t: a wave spans at most two particles (V0 >= 64)
+      const int bfirst = __shfl(b, 0, 64);
+      const bool first = b == bfirst;
+      float r0 = first ? mre : 0.f, i0 = first ? mim : 0.f;
+      float r1 = first ? 0.f : mre, i1 = first ? 0.f : mim;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) {
+        r0 = fmaxf(r0, __shfl_xor(r0, off, 64));
+        i0 = fmaxf(i0, __shfl_xor(i0, off, 64));
+        r1 = fmaxf(r1, __shfl_xor(r1, off, 64));
+        i1 = fmaxf(i1, __shfl_xor(i1, off, 64));
+      }
+      // the second particle of the wave (if any) is the largest active b; lane 0
+      // is active whenever any lane is (t grows with the lane)
+      int blast = active ? b : bfirst;
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) blast = max(blast, __shfl_xor(blast, off, 64));
+      if ((threadIdx.x & 63) == 0 && active) {
+        atomicMax(reinterpret_cast<unsigned*>(rowmax) + bfirst, __float_as_uint(r0));
+        atomicMax(reinterpret_cast<unsigned*>(rowmax) + B + bfirst, __float_as_uint(i0));
+        if (blast != bfirst) {
+          atomicMax(reinterpret_cast<unsigned*>(rowmax) + blast, __float_as_uint(r1));
+          atomicMax(reinterpret_cast<unsigned*>(rowmax) + B + blast, __float_as_uint(i1));
+        }
       }
     }
   }
@@ -193,13 +311,234 @@ __global__ __launch_bounds__(kGemmThreads) void fle_gemm_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// 2b. tuned path: 16-bit split GEMM, C = Xhi Whi + Xhi Wlo + Xlo Whi (FP32 acc)
+// ---------------------------------------------------------------------------
+// 1-D grid over (column tile, row tile, K split), XCD-aware so that the column
+// tiles of one class and K range (which read the same X rows) share one L2.
+// Tile 128 rows x 128 columns x 32 k per stage; wave w owns columns 32w..32w+31
+// (2 column blocks) and all 8 row blocks.  Both operands arrive by LDS-DMA
+// (global_load_lds_dwordx4: lane i's 16 bytes land at slot + 16 i), laid out
+// in MFMA fragment order, so every lane reads its own 16-byte fragment back:
+//   X hi/lo: 8 row blocks x 64 lanes (row rb*16 + lane%16, k 8*(lane/16)),
+//            shared by the 4 waves (each wave fetches 2 row blocks);
+//   W^T hi/lo: per wave 2 column blocks x 64 lanes, private to the wave.
+// NST stages rotate through LDS: while stage st is multiplied, stages
+// st + 1 .. st + NST - 1 are in flight (HBM latency x bandwidth), and no
+// operand passes through registers on its way in.
+constexpr int kStageU4 = 2048;   // u32x4 per stage: A hi 512, A lo 512, W 4 waves x 4 x 64
+
+// s_waitcnt immediates (gfx9 encoding: vmcnt bits [3:0] and [15:14], expcnt
+// [6:4], lgkmcnt [11:8]); the counters not named are left unconstrained
+constexpr int waitcnt_vm(int n) { return (n & 0xF) | ((n >> 4) << 14) | (7 << 4) | (0xF << 8); }
+
+// Execution barrier of the workgroup without the memory fence of
+// __syncthreads (which waits for every outstanding load, LDS-DMA included).
+// Callers make their own data visible first: an explicit vmcnt wait for their
+// LDS-DMA, and LDS reads that have been consumed.  The empty asm statements
+// keep the compiler from moving memory accesses across the barrier.
+__device__ __forceinline__ void wg_barrier_keep_loads() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
+__device__ __forceinline__ void lds_dma16(const void* g, u32x4* lds_wave_base) {
+  __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds_wave_base, 16, 0, 0);
+}
+
+template <bool FP16, int NST, int NW>
+__global__ __launch_bounds__(64 * NW) void fle_gemm16_kernel(
+    const uint16_t* __restrict__ Xhi, const uint16_t* __restrict__ Xlo, const u32x4* __restrict__ Wt16,
+    float* __restrict__ part, int Mpad, int Kpad, int Ntot, int kchunk, int ntn, int nmt, FlePlan cols) {
+  static_assert(NST >= 2 && NST <= 3, "pipeline depth");
+  static_assert(NW == 2 || NW == 4, "waves per workgroup");
+  constexpr int G = 4 / NW;        // 32-column groups per wave
+  constexpr int CB = 2 * G;        // 16-column blocks per wave
+  constexpr int AP = 16 / NW;      // A (row block, hi/lo) parts fetched per wave
+  extern __shared__ u32x4 lds[];   // NST x kStageU4
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lb = mch::xcd_remap(blockIdx.x, gridDim.x);
+  const int tn = lb % ntn, rest = lb / ntn;
+  const int mt = rest % nmt, ks = rest / nmt;
+  const int col0 = tn * BN16, row0 = mt * BM16;
+  int cls = 0;
+  while (cls < kClasses - 1 && col0 >= cols.col_off[cls + 1]) ++cls;
+  const int kbeg = ks * kchunk;
+  const int kend = min(kbeg + kchunk, Kpad);
+  const int nstage = kend > kbeg ? (kend - kbeg) / BK16 : 0;
+
+  // this lane's DMA sources: A parts (row block rb, hi/lo) of the row-major X,
+  // and this wave's G x 4 KB of the DMA-ordered W^T per stage
+  const long long xbase = ((long long)cls * Mpad + row0) * Kpad + (lane >> 4) * 8;
+  const uint16_t* asrc[AP];
+  int adst[AP];
+#pragma unroll
+  for (int t = 0; t < AP; ++t) {
+    const int part = wave * AP + t;            // 0..15: row block part / 2, hi/lo part % 2
+    const int rb = part >> 1, hl = part & 1;
+    asrc[t] = (hl ? Xlo : Xhi) + xbase + (long long)(rb * 16 + (lane & 15)) * Kpad;
+    adst[t] = hl * 512 + rb * 64;
+  }
+  const int nK = Kpad / BK16;
+  const u32x4* wsrc = Wt16 + ((long long)tn * nK + kbeg / BK16) * 1024 + wave * G * 256 + lane;
+  // stage layout (u32x4): A hi row block rb at rb*64, A lo at 512 + rb*64,
+  // W of 32-column group g, part q = {j * 2 + hi/lo} at 1024 + g*256 + q*64
+  auto issue = [&](int stage) {
+    u32x4* base = lds + (stage % NST) * kStageU4;
+    const int k0 = kbeg + stage * BK16;
+#pragma unroll
+    for (int t = 0; t < AP; ++t) lds_dma16(asrc[t] + k0, base + adst[t]);
+#pragma unroll
+    for (int q = 0; q < 4 * G; ++q)
+      lds_dma16(wsrc + (long long)stage * 1024 + q * 64, base + 1024 + wave * G * 256 + q * 64);
+  };
+  constexpr int PER_STAGE = AP + 4 * G;  // DMAs per wave per stage
+  floatx4 acc[8][CB];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int c = 0; c < CB; ++c) acc[i][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll
+  for (int s = 0; s < NST - 1; ++s)
+    if (s < nstage) issue(s);
+  for (int st = 0; st < nstage; ++st) {
+    // this wave's DMAs of stage st are done once at most those of the later
+    // stages already issued are pending
+    const int later = min(st + NST - 2, nstage - 1) - st;
+    if (NST == 3 && later >= 1)
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(PER_STAGE));
+    else
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    // every wave's part of stage st has landed, and every wave has finished
+    // stage st - 1, whose slot is refilled next
+    wg_barrier_keep_loads();
+    if (st + NST - 1 < nstage) issue(st + NST - 1);
+    const u32x4* base = lds + (st % NST) * kStageU4;
+    u32x4 w[4 * G];
+#pragma unroll
+    for (int q = 0; q < 4 * G; ++q) w[q] = base[1024 + wave * G * 256 + q * 64 + lane];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const u32x4 ah = base[i * 64 + lane];
+      const u32x4 al = base[512 + i * 64 + lane];
+#pragma unroll
+      for (int c = 0; c < CB; ++c) {
+        // column block c = group c / 2, block c % 2: parts q = 4 (c / 2) + 2 (c % 2) + {0, 1}
+        acc[i][c] = Split16<FP16>::mfma(ah, w[2 * c + 0], acc[i][c]);
+        acc[i][c] = Split16<FP16>::mfma(ah, w[2 * c + 1], acc[i][c]);
+        acc[i][c] = Split16<FP16>::mfma(al, w[2 * c + 0], acc[i][c]);
+      }
+    }
+  }
+  // D layout of the 16x16 MFMAs: lane holds rows 4 (lane / 16) + r, column lane % 16
+  float* P = part + (long long)ks * Mpad * Ntot;
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = row0 + i * 16 + 4 * (lane >> 4) + r;
+        const int col = col0 + wave * G * 32 + c * 16 + (lane & 15);
+        P[(long long)row * Ntot + col] = acc[i][c][r];
+      }
+}
+
+// FP16 mode: X (FP32) -> Xhi/Xlo of X * rowscale (rows of every class)
+__global__ void fle_xsplit_kernel(const float* __restrict__ X, const float* __restrict__ rowmax, int Mpad,
+                                  int Kpad, int rows, uint16_t* __restrict__ Xhi, uint16_t* __restrict__ Xlo) {
+  const long long n4 = (long long)kClasses * Mpad * Kpad / 4;
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n4;
+       q += (long long)gridDim.x * blockDim.x) {
+    const long long e = q * 4;
+    const int row = (int)((e / Kpad) % Mpad);
+    if (row >= rows) continue;
+    const float rs = pow2_scale(rowmax[row]);
+    const float4 v = reinterpret_cast<const float4*>(X)[q];
+    uint16_t h[4], l[4];
+    Split16<true>::split(v.x * rs, h[0], l[0]);
+    Split16<true>::split(v.y * rs, h[1], l[1]);
+    Split16<true>::split(v.z * rs, h[2], l[2]);
+    Split16<true>::split(v.w * rs, h[3], l[3]);
+    reinterpret_cast<uint2*>(Xhi)[q] = make_uint2((unsigned)h[0] | ((unsigned)h[1] << 16),
+                                                  (unsigned)h[2] | ((unsigned)h[3] << 16));
+    reinterpret_cast<uint2*>(Xlo)[q] = make_uint2((unsigned)l[0] | ((unsigned)l[1] << 16),
+                                                  (unsigned)l[2] | ((unsigned)l[3] << 16));
+  }
+}
+
+__global__ void fle_fill_kernel(float* __restrict__ p, int n, float v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// max |W| of each column over a K range (atomic over K ranges; values >= 0)
+__global__ void fle_colmax_kernel(const float* __restrict__ W, int Kpad, int Ntot, int kchunk,
+                                  float* __restrict__ colmax) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= Ntot) return;
+  const int k0 = blockIdx.y * kchunk, k1 = min(k0 + kchunk, Kpad);
+  float m = 0.f;
+  for (int k = k0; k < k1; ++k) m = fmaxf(m, fabsf(W[(long long)k * Ntot + c]));
+  atomicMax(reinterpret_cast<unsigned*>(colmax) + c, __float_as_uint(m));
+}
+
+__global__ void fle_colscale_kernel(float* __restrict__ cs, int Ntot, int fp16) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < Ntot) cs[c] = fp16 ? pow2_scale(cs[c]) : 1.f;
+}
+
+// W (Kpad, Ntot) FP32 -> 16-bit hi/lo of colscale * W, stored in the exact
+// order the GEMM's LDS-DMA streams it, in 16-byte chunks of 8 consecutive k:
+//   chunk ((((tn * nK + S) * 4 + w) * 4 + j * 2 + h) * 64 + lane)
+// for column tn * 128 + 32 w + 16 j + lane % 16, k = 32 S + 8 (lane / 16),
+// h = 0 (hi) / 1 (lo), nK = Kpad / 32: each workgroup reads one contiguous
+// 16 KB block per stage.  One workgroup per 64 x 64 (k, column) tile.
+template <bool FP16>
+__global__ __launch_bounds__(256) void fle_wsplit_kernel(const float* __restrict__ W, int Kpad, int Ntot,
+                                                         const float* __restrict__ colscale,
+                                                         u32x4* __restrict__ Wt16) {
+  __shared__ float tile[64][65];
+  const int c0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+    const int kk = i >> 6, cc = i & 63;
+    tile[kk][cc] = W[(long long)(k0 + kk) * Ntot + c0 + cc];
+  }
+  __syncthreads();
+  const int nK = Kpad / BK16;
+  // thread: column cc, 8 consecutive k (kg-th group of the 64)
+  for (int t = threadIdx.x; t < 64 * 8; t += 256) {
+    const int cc = t & 63, kg = t >> 6;
+    const int col = c0 + cc, k = k0 + kg * 8;
+    const float cs = colscale[col];
+    u32x4 hv, lv;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      uint16_t h0, l0, h1, l1;
+      Split16<FP16>::split(tile[kg * 8 + 2 * e][cc] * cs, h0, l0);
+      Split16<FP16>::split(tile[kg * 8 + 2 * e + 1][cc] * cs, h1, l1);
+      hv[e] = (unsigned)h0 | ((unsigned)h1 << 16);
+      lv[e] = (unsigned)l0 | ((unsigned)l1 << 16);
+    }
+    const int tn = col / BN16, w = (col % BN16) / 32, j = (col % 32) / 16;
+    const int S = k / BK16, lane = (col % 16) + 16 * ((k % BK16) / 8);
+    const long long chunk = ((((long long)tn * nK + S) * 4 + w) * 4 + j * 2) * 64 + lane;
+    Wt16[chunk] = hv;
+    Wt16[chunk + 64] = lv;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 3. split-K reduction and scatter to the reference coefficient order
 // ---------------------------------------------------------------------------
+// colscale / rowmax (NULL: 1) undo the power-of-2 scales of the FP16 split GEMM.
 __global__ void fle_finalize_kernel(const float* __restrict__ part, int splits, int Mpad, int Ntot,
                                     int B, const int* __restrict__ chan_plus,
                                     const int* __restrict__ chan_minus,
                                     const int* __restrict__ chan_m, float2* __restrict__ a,
-                                    long long lda) {
+                                    long long lda, const float* __restrict__ colscale,
+                                    const float* __restrict__ rowmax) {
   const long long tot = (long long)B * Ntot;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < tot;
        t += (long long)gridDim.x * blockDim.x) {
@@ -212,6 +551,10 @@ __global__ void fle_finalize_kernel(const float* __restrict__ part, int splits, 
       const float* P = part + (long long)s * Mpad * Ntot;
       re += P[(long long)b * Ntot + c];
       im += P[(long long)(B + b) * Ntot + c];
+    }
+    if (colscale) {
+      re /= colscale[c] * pow2_scale(rowmax[b]);
+      im /= colscale[c] * pow2_scale(rowmax[B + b]);
     }
     a[(long long)b * lda + ip] = make_float2(re, im);
     const int imn = chan_minus[c];
@@ -274,6 +617,60 @@ __global__ __launch_bounds__(256) void norms_kernel(const float2* __restrict__ a
     norms[b] = sqrtf(t);
   }
 }
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+namespace {
+
+void free_fle_plan(FlePlan* p) {
+  if (!p) return;
+  (void)hipFree(p->orbit); (void)hipFree(p->W); (void)hipFree(p->chan_plus);
+  (void)hipFree(p->chan_minus); (void)hipFree(p->chan_m); (void)hipFree(p->X); (void)hipFree(p->part);
+  (void)hipFree(p->Wt16); (void)hipFree(p->colscale); (void)hipFree(p->rowmax);
+  (void)hipFree(p->part16); (void)hipFree(p->Xhi); (void)hipFree(p->Xlo);
+  delete p;
+}
+
+// the projection GEMM of the plan's precision on its X operand
+int launch_gemm(FlePlan* p, int B, hipStream_t stream, int* splits_out) {
+  const int mt = (2 * B + BM - 1) / BM;
+  if (p->precision == kFp32) {
+    const int kblocks = p->Kpad / BK;
+    const int kchunk = ((kblocks + p->splits - 1) / p->splits) * BK;
+    const int splits = (p->Kpad + kchunk - 1) / kchunk;
+    hipLaunchKernelGGL(fle_gemm_kernel, dim3(p->Ntot / BN, mt, splits), dim3(kGemmThreads), 0,
+                       stream, p->X, p->W, p->part, p->Mpad, p->Kpad, p->Ntot, kchunk, *p);
+    *splits_out = splits;
+    return mch::hip_check(hipGetLastError(), "fle_gemm_kernel launch");
+  }
+  const int kblocks = p->Kpad / BK16;
+  const int kchunk = ((kblocks + p->splits16 - 1) / p->splits16) * BK16;
+  const int splits = (p->Kpad + kchunk - 1) / kchunk;
+  const int ntn = p->Ntot / BN16;
+  const dim3 grid(ntn * mt * splits);
+  const size_t lds = (size_t)p->nst16 * kStageU4 * sizeof(u32x4);
+  const bool fp16 = p->precision == kFp16x3;
+#define MCH_GEMM16(F, S, W)                                                                        \
+  hipLaunchKernelGGL((fle_gemm16_kernel<F, S, W>), grid, dim3(64 * W), lds, stream, p->Xhi, p->Xlo, \
+                     (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, kchunk, ntn, mt, *p)
+  if (p->nw16 == 2) {
+    if (fp16 && p->nst16 == 3) MCH_GEMM16(true, 3, 2);
+    else if (fp16) MCH_GEMM16(true, 2, 2);
+    else if (p->nst16 == 3) MCH_GEMM16(false, 3, 2);
+    else MCH_GEMM16(false, 2, 2);
+  } else {
+    if (fp16 && p->nst16 == 3) MCH_GEMM16(true, 3, 4);
+    else if (fp16) MCH_GEMM16(true, 2, 4);
+    else if (p->nst16 == 3) MCH_GEMM16(false, 3, 4);
+    else MCH_GEMM16(false, 2, 4);
+  }
+#undef MCH_GEMM16
+  *splits_out = splits;
+  return mch::hip_check(hipGetLastError(), "fle_gemm16_kernel launch");
+}
+
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -346,19 +743,135 @@ int mch_fle_plan_create(int N, int V0, int Kpad, int Ntot, const int* class_colp
   *out_plan = p;
   return MCH_OK;
 fail:
-  (void)hipFree(p->orbit); (void)hipFree(p->W); (void)hipFree(p->chan_plus);
-  (void)hipFree(p->chan_minus); (void)hipFree(p->chan_m); (void)hipFree(p->X); (void)hipFree(p->part);
-  delete p;
+  free_fle_plan(p);
   return rc;
 }
 
 int mch_fle_plan_destroy(void* plan) {
-  FlePlan* p = (FlePlan*)plan;
-  if (!p) return MCH_OK;
-  (void)hipFree(p->orbit); (void)hipFree(p->W); (void)hipFree(p->chan_plus);
-  (void)hipFree(p->chan_minus); (void)hipFree(p->chan_m); (void)hipFree(p->X); (void)hipFree(p->part);
-  delete p;
+  free_fle_plan((FlePlan*)plan);
   return MCH_OK;
+}
+
+// Selects the compute precision of the projection GEMM: 0 = FP32 (the plain
+// path), 1 = BF16 hi/lo split (three MFMA products, FP32 accumulation),
+// 2 = FP16 hi/lo split with power-of-2 row and column scales.  The first
+// switch to a 16-bit mode builds W^T hi/lo on the device (synchronises).
+int mch_fle_set_precision(void* plan, int precision) {
+  FlePlan* p = (FlePlan*)plan;
+  if (!p || precision < kFp32 || precision > kFp16x3) {
+    mch::set_error("mch_fle_set_precision: bad arguments (precision %d)", precision);
+    return MCH_ERR_ARG;
+  }
+  if (precision == kFp32) {
+    p->precision = kFp32;
+    return MCH_OK;
+  }
+  if (p->Kpad % BK16 || p->Ntot % BN16) {
+    mch::set_error("mch_fle_set_precision: 16-bit path needs Kpad %% %d == 0 and class widths %% %d == 0",
+                   BK16, BN16);
+    return MCH_ERR_UNSUPPORTED;
+  }
+  for (int c = 0; c < kClasses; ++c) {
+    if ((p->col_off[c + 1] - p->col_off[c]) % BN16) {
+      mch::set_error("mch_fle_set_precision: class column counts must be multiples of %d", BN16);
+      return MCH_ERR_UNSUPPORTED;
+    }
+  }
+  int rc = MCH_OK;
+  const size_t nw = (size_t)p->Ntot * p->Kpad;
+#define MCH_TRY(expr, what) do { if ((rc = mch::hip_check((expr), what)) != MCH_OK) return rc; } while (0)
+  if (!p->Wt16) {
+    MCH_TRY(hipMalloc(&p->Wt16, 2 * nw * sizeof(uint16_t)), "hipMalloc(Wt16)");
+    MCH_TRY(hipMalloc(&p->colscale, (size_t)p->Ntot * sizeof(float)), "hipMalloc(colscale)");
+    MCH_TRY(hipMalloc(&p->rowmax, (size_t)p->Mpad * sizeof(float)), "hipMalloc(rowmax)");
+    // split K so that the grid is ~two full rounds of resident workgroups
+    // (2 per CU on 256 CUs)
+    const int ntiles = p->Ntot / BN16 * (p->Mpad / BM16);
+    int splits = (1024 + ntiles - 1) / ntiles;
+    const int kblocks = p->Kpad / BK16;
+    if (splits > kblocks) splits = kblocks;
+    if (splits < 1) splits = 1;
+    p->splits16 = splits;
+    MCH_TRY(hipMalloc(&p->part16, (size_t)splits * p->Mpad * p->Ntot * sizeof(float)), "hipMalloc(part16)");
+    const size_t nx = (size_t)kClasses * p->Mpad * p->Kpad;
+    MCH_TRY(hipMalloc(&p->Xhi, nx * sizeof(uint16_t)), "hipMalloc(Xhi)");
+    MCH_TRY(hipMalloc(&p->Xlo, nx * sizeof(uint16_t)), "hipMalloc(Xlo)");
+    MCH_TRY(hipMemset(p->Xhi, 0, nx * sizeof(uint16_t)), "memset(Xhi)");
+    MCH_TRY(hipMemset(p->Xlo, 0, nx * sizeof(uint16_t)), "memset(Xlo)");
+    const int lds3 = 3 * kStageU4 * (int)sizeof(u32x4);
+    MCH_TRY(hipFuncSetAttribute((const void*)fle_gemm16_kernel<false, 3, 4>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds3), "hipFuncSetAttribute");
+    MCH_TRY(hipFuncSetAttribute((const void*)fle_gemm16_kernel<true, 3, 4>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds3), "hipFuncSetAttribute");
+    MCH_TRY(hipFuncSetAttribute((const void*)fle_gemm16_kernel<false, 3, 2>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds3), "hipFuncSetAttribute");
+    MCH_TRY(hipFuncSetAttribute((const void*)fle_gemm16_kernel<true, 3, 2>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds3), "hipFuncSetAttribute");
+    p->nst16 = 2;
+    p->nw16 = 4;
+  }
+  if (p->table_precision != precision) {
+    const bool fp16 = precision == kFp16x3;
+    const int nb = (p->Ntot + 255) / 256;
+    hipLaunchKernelGGL(fle_fill_kernel, dim3(nb), dim3(256), 0, 0, p->colscale, p->Ntot, 0.f);
+    if (fp16) {
+      const int kchunk = (p->Kpad + 63) / 64;
+      hipLaunchKernelGGL(fle_colmax_kernel, dim3(nb, 64), dim3(256), 0, 0, p->W, p->Kpad, p->Ntot, kchunk,
+                         p->colscale);
+    }
+    hipLaunchKernelGGL(fle_colscale_kernel, dim3(nb), dim3(256), 0, 0, p->colscale, p->Ntot, fp16 ? 1 : 0);
+    if (fp16)
+      hipLaunchKernelGGL(fle_wsplit_kernel<true>, dim3(p->Ntot / 64, p->Kpad / 64), dim3(256), 0, 0, p->W,
+                         p->Kpad, p->Ntot, p->colscale, (u32x4*)p->Wt16);
+    else
+      hipLaunchKernelGGL(fle_wsplit_kernel<false>, dim3(p->Ntot / 64, p->Kpad / 64), dim3(256), 0, 0, p->W,
+                         p->Kpad, p->Ntot, p->colscale, (u32x4*)p->Wt16);
+    MCH_TRY(hipGetLastError(), "fle_wsplit_kernel launch");
+    MCH_TRY(hipDeviceSynchronize(), "building the 16-bit projection tables");
+    p->table_precision = precision;
+    p->precision = precision;
+    // autotune the pipeline depth on the full batch (timing does not depend on
+    // the operand values): 2 stages fit two workgroups per CU, 3 stages keep
+    // more HBM traffic in flight per workgroup
+    hipEvent_t e0, e1;
+    MCH_TRY(hipEventCreate(&e0), "hipEventCreate");
+    MCH_TRY(hipEventCreate(&e1), "hipEventCreate");
+    float best = 1e30f;
+    int best_nst = 2, best_nw = 4;
+    for (int cand = 0; cand < 4; ++cand) {
+      const int nst = 2 + (cand & 1), nw = (cand & 2) ? 2 : 4;
+      p->nst16 = nst;
+      p->nw16 = nw;
+      int splits = 0;
+      for (int rep = 0; rep < 2; ++rep)
+        if ((rc = launch_gemm(p, p->max_batch, 0, &splits))) break;
+      if (rc) break;
+      (void)hipEventRecord(e0, 0);
+      for (int rep = 0; rep < 5; ++rep)
+        if ((rc = launch_gemm(p, p->max_batch, 0, &splits))) break;
+      if (rc) break;
+      (void)hipEventRecord(e1, 0);
+      MCH_TRY(hipEventSynchronize(e1), "autotune");
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) { best = ms; best_nst = nst; best_nw = nw; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    p->nst16 = best_nst;
+    p->nw16 = best_nw;
+  }
+#undef MCH_TRY
+  p->precision = precision;
+  return MCH_OK;
+}
+
+// Tile configuration the 16-bit GEMM was tuned to: 10 * waves per workgroup
+// + pipeline depth (0 before any 16-bit mode).
+int mch_fle_tuned_depth(void* plan) {
+  FlePlan* p = (FlePlan*)plan;
+  return p ? 10 * p->nw16 + p->nst16 : 0;
 }
 
 // a (B, ne) complex64 rows of stride lda = B^H f for volumes (B, N^3) float32.
@@ -374,24 +887,37 @@ int mch_fle_project(void* plan, const float* vols, long long ld_vol, int B, void
   const long long tot = (long long)B * p->V0;
   int nb = (int)((tot + 255) / 256);
   if (nb > 8192) nb = 8192;
-  hipLaunchKernelGGL(fle_symmetrize_kernel, dim3(nb), dim3(256), 0, stream, vols, ld_vol, B,
-                     p->orbit, p->V0, p->Kpad, p->Mpad, p->X);
-  int rc = mch::hip_check(hipGetLastError(), "fle_symmetrize_kernel launch");
+  const bool fp16 = p->precision == kFp16x3;
+  int rc;
+  if (fp16) {
+    hipLaunchKernelGGL(fle_fill_kernel, dim3((p->Mpad + 255) / 256), dim3(256), 0, stream, p->rowmax,
+                       p->Mpad, 0.f);
+    if ((rc = mch::hip_check(hipGetLastError(), "fle_fill_kernel launch"))) return rc;
+  }
+  if (p->precision == kBf16x3) {
+    hipLaunchKernelGGL(fle_symmetrize_kernel<true>, dim3(nb), dim3(256), 0, stream, vols, ld_vol, B,
+                       p->orbit, p->V0, p->Kpad, p->Mpad, p->X, nullptr, p->Xhi, p->Xlo);
+  } else {
+    hipLaunchKernelGGL(fle_symmetrize_kernel<false>, dim3(nb), dim3(256), 0, stream, vols, ld_vol, B,
+                       p->orbit, p->V0, p->Kpad, p->Mpad, p->X, fp16 ? p->rowmax : nullptr, nullptr,
+                       nullptr);
+  }
+  rc = mch::hip_check(hipGetLastError(), "fle_symmetrize_kernel launch");
   if (rc) return rc;
-  const int mt = (2 * B + BM - 1) / BM;
-  const int kblocks = p->Kpad / BK;
-  const int kchunk = ((kblocks + p->splits - 1) / p->splits) * BK;
-  const int splits = (p->Kpad + kchunk - 1) / kchunk;
-  hipLaunchKernelGGL(fle_gemm_kernel, dim3(p->Ntot / BN, mt, splits), dim3(kGemmThreads), 0,
-                     stream, p->X, p->W, p->part, p->Mpad, p->Kpad, p->Ntot, kchunk, *p);
-  rc = mch::hip_check(hipGetLastError(), "fle_gemm_kernel launch");
-  if (rc) return rc;
+  if (fp16) {
+    hipLaunchKernelGGL(fle_xsplit_kernel, dim3(4096), dim3(256), 0, stream, p->X, p->rowmax, p->Mpad,
+                       p->Kpad, 2 * B, p->Xhi, p->Xlo);
+    if ((rc = mch::hip_check(hipGetLastError(), "fle_xsplit_kernel launch"))) return rc;
+  }
+  int splits = 0;
+  if ((rc = launch_gemm(p, B, stream, &splits))) return rc;
   const long long ftot = (long long)B * p->Ntot;
   int fb = (int)((ftot + 255) / 256);
   if (fb > 8192) fb = 8192;
-  hipLaunchKernelGGL(fle_finalize_kernel, dim3(fb), dim3(256), 0, stream, p->part, splits,
-                     p->Mpad, p->Ntot, B, p->chan_plus, p->chan_minus, p->chan_m,
-                     (float2*)coeffs, lda);
+  hipLaunchKernelGGL(fle_finalize_kernel, dim3(fb), dim3(256), 0, stream,
+                     p->precision == kFp32 ? p->part : p->part16, splits, p->Mpad, p->Ntot, B,
+                     p->chan_plus, p->chan_minus, p->chan_m, (float2*)coeffs, lda,
+                     fp16 ? p->colscale : nullptr, fp16 ? p->rowmax : nullptr);
   return mch::hip_check(hipGetLastError(), "fle_finalize_kernel launch");
 }
 
@@ -400,13 +926,8 @@ int mch_fle_project(void* plan, const float* vols, long long ld_vol, int B, void
 int mch_fle_gemm_only(void* plan, int B, hipStream_t stream) {
   FlePlan* p = (FlePlan*)plan;
   if (!p || B < 1 || B > p->max_batch) { mch::set_error("mch_fle_gemm_only: bad arguments"); return MCH_ERR_ARG; }
-  const int mt = (2 * B + BM - 1) / BM;
-  const int kblocks = p->Kpad / BK;
-  const int kchunk = ((kblocks + p->splits - 1) / p->splits) * BK;
-  const int splits = (p->Kpad + kchunk - 1) / kchunk;
-  hipLaunchKernelGGL(fle_gemm_kernel, dim3(p->Ntot / BN, mt, splits), dim3(kGemmThreads), 0,
-                     stream, p->X, p->W, p->part, p->Mpad, p->Kpad, p->Ntot, kchunk, *p);
-  return mch::hip_check(hipGetLastError(), "fle_gemm_kernel launch");
+  int splits = 0;
+  return launch_gemm(p, B, stream, &splits);
 }
 
 int mch_corr_plan_create(int lmax, int kdim, const int* idx_host, int max_batch, void** out_plan) {

@@ -107,3 +107,63 @@ def test_get_sigma_end_to_end_matches_oracle(gpu_device):
     # the unrotated copy correlates best at the identity rotation
     s_id = so3.score(np.array([0.0]), np.array([0.0]), np.array([0.0]), ref[1], ccm.lmax)[0]
     assert s_id == pytest.approx(np.linalg.norm(aT) ** 2, rel=1e-6)
+
+
+# --- tuned path: 16-bit hi/lo split GEMM (three MFMA products, FP32 accumulation) ---
+# Bars against the FP64 oracle, relative to max |a|: FP16x3 keeps ~22 mantissa bits per
+# operand (same bar as FP32, 2e-5); BF16x3 keeps ~16 (8e-5).
+SPLIT_TOL = {"fp16x3": 2e-5, "bf16x3": 8e-5}
+
+
+@pytest.mark.parametrize("mode", ["bf16x3", "fp16x3"])
+@pytest.mark.parametrize("N,bl,radius,B", [(24, 8, 11, 3), (32, 12, 15, 5)])
+def test_projection_split_precision_exact(gpu_device, mode, N, bl, radius, B):
+    from matcha_amd.core.FLEBasis3D import FLEBasis3D
+    rng = np.random.default_rng(N + bl + len(mode))
+    vols = rng.standard_normal((B, N, N, N)).astype(np.float32)
+    f = FLEBasis3D(N, bl, 1e-4, batchsize=B, radius=radius, device=gpu_device, compute_precision=mode)
+    got = f.evaluate_t(torch.tensor(vols, device=gpu_device)).cpu().numpy()
+    ref = fle.evaluate_t(vols.astype(np.float64), N, bl, radius)
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    assert err < SPLIT_TOL[mode], err
+
+
+@pytest.mark.parametrize("mode", ["bf16x3", "fp16x3"])
+@pytest.mark.parametrize("amp", [1e4, 1e-4])
+def test_projection_split_precision_scaling(gpu_device, mode, amp):
+    # amplitudes far outside FP16's range without the power-of-2 row/column scales;
+    # one smaller batch after a larger one (stale rows of the operand must not leak)
+    from matcha_amd.core.FLEBasis3D import FLEBasis3D
+    N, bl, radius = 24, 8, 11
+    rng = np.random.default_rng(int(amp * 7) % 1000)
+    f = FLEBasis3D(N, bl, 1e-4, batchsize=4, radius=radius, device=gpu_device, compute_precision=mode)
+    big = (amp * rng.standard_normal((4, N, N, N))).astype(np.float32)
+    f.evaluate_t(torch.tensor(big, device=gpu_device))
+    vols = (amp * rng.standard_normal((2, N, N, N))).astype(np.float32)
+    got = f.evaluate_t(torch.tensor(vols, device=gpu_device)).cpu().numpy()
+    ref = fle.evaluate_t(vols.astype(np.float64), N, bl, radius)
+    assert np.all(np.isfinite(got))
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    assert err < SPLIT_TOL[mode], err
+
+
+@pytest.mark.parametrize("mode", ["bf16x3", "fp16x3"])
+def test_projection_split_precision_full_size(gpu_device, mode):
+    from matcha_amd.core.FLEBasis3D import FLEBasis3D
+    N, bl, radius = 96, 32, 45
+    rng = np.random.default_rng(96)
+    vol = fle.blob_volume(rng, N, 30, sigma_range=(1.0, 5.0)) + 0.3 * rng.standard_normal((N, N, N))
+    vols = vol[None].astype(np.float32)
+    f = FLEBasis3D(N, bl, 1e-4, batchsize=2, radius=radius, device=gpu_device)
+    plain = f.evaluate_t(torch.tensor(vols, device=gpu_device)).cpu().numpy()[0]
+    f.set_compute_precision(mode)
+    got = f.evaluate_t(torch.tensor(vols, device=gpu_device)).cpu().numpy()[0]
+    pick = sorted(set([0, 1, f.ne - 1, int(np.argmax(f.ls))] + list(rng.choice(f.ne, 12, replace=False))))
+    ref = fle.evaluate_t_subset(vols.astype(np.float64), N, bl, radius, pick)[0]
+    scale = np.linalg.norm(plain)
+    assert np.max(np.abs(got[pick] - ref)) < SPLIT_TOL[mode] * scale
+    # whole vector against the FP32 path (itself within 2e-5 of the oracle)
+    assert np.max(np.abs(got - plain)) < (SPLIT_TOL[mode] + 2e-5) * scale
+    f.set_compute_precision("fp32")
+    again = f.evaluate_t(torch.tensor(vols, device=gpu_device)).cpu().numpy()[0]
+    assert np.array_equal(again, plain)
