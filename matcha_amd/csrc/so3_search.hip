@@ -129,10 +129,13 @@ __global__ void so3_repack_kernel(const float2* __restrict__ sig, long long ldsi
     v2.x *= sg; v2.y *= sg;
     if (m == mp || m == -mp) { v2 = make_float2(0.f, 0.f); v3 = v2; }
     if (mp == 0) v1 = make_float2(0.f, 0.f);
+    // images (mp, m) and (-mp, -m) carry conjugate phases e^{+-i(mp a + m g)},
+    // and every derivative weight (u, v, u^2, uv, v^2) of the second equals
+    // that of the first after conjugation: they enter as sigma_0 + conj sigma_1
+    // (likewise (m, mp) and (-m, -mp)).  16 bytes per (l, pair).
     float4* o = reinterpret_cast<float4*>(pk + (long long)b * mch::pk_size(Lmax) +
-                                          mch::pk_offset(l) + 4 * p);
-    o[0] = make_float4(v0.x, v0.y, v1.x, v1.y);
-    o[1] = make_float4(v2.x, v2.y, v3.x, v3.y);
+                                          mch::pk_offset(l) / 2 + 2 * p);
+    o[0] = make_float4(v0.x + v1.x, v0.y - v1.y, v2.x + v3.x, v2.y - v3.y);
   }
 }
 
@@ -335,11 +338,30 @@ struct EvalOut {
   float v[10];  // s, ga, gb, gg, Haa, Hab, Hag, Hbb, Hbg, Hgg
 };
 
+// Per-l constants of the three-term l-recurrence of d^l_{m'm}:
+// (l^2, l (2l - 1), 1 / ((l - 1) l) (0 for l <= 1), 1 / (l (2l + 1))),
+// rounded from FP64; built once per workgroup in LDS.
+__device__ void fill_lrec(float4* lrec, int L) {
+  for (int l = threadIdx.x; l <= L + 1; l += blockDim.x) {
+    const double dl = (double)l;
+    lrec[l] = make_float4((float)(dl * dl), (float)(dl * (2.0 * dl - 1.0)),
+                          l > 1 ? (float)(1.0 / ((dl - 1.0) * dl)) : 0.f,
+                          l > 0 ? (float)(1.0 / (dl * (2.0 * dl + 1.0))) : 0.f);
+  }
+}
+
 // Evaluate score, gradient and Hessian of Re sum_{l<=Li} D(a,b,g) sigma.
-__device__ void march_eval(const float2* __restrict__ sp, int Li, float beta,
+// Lanes own fundamental pairs p = mp^2 + mp + m (0 <= |m| <= mp <= Li) and
+// march l upwards from the task's smallest mp; the repacked sigma of shell l
+// starts at byte 4 pk_offset(l), so one wave-uniform offset (advanced by
+// 16 (l + 1)^2 bytes) plus the lane's 16 p addresses both merged image pairs.
+__device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int Li, float beta,
                            const float2* __restrict__ za, const float2* __restrict__ zg,
-                           const float* __restrict__ lgf, float* red, float out[10]) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+                           const float* __restrict__ lgf, const float4* __restrict__ lrec, float* red,
+                           float out[10]) {
+  // the wave index through readfirstlane: the compiler then keeps every
+  // per-task quantity (t, lstart, l, the sigma offset) in scalar registers
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   float acc[10];
 #pragma unroll
   for (int i = 0; i < 10; ++i) acc[i] = 0.f;
@@ -347,9 +369,14 @@ __device__ void march_eval(const float2* __restrict__ sp, int Li, float beta,
   const float hc = cosf(0.5f * beta), hs = sinf(0.5f * beta);
   const float lc = log2f(fabsf(hc)), ls = log2f(fabsf(hs));
   const bool cneg = hc < 0.f, sneg = hs < 0.f;
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(sp), (short)0, (int)sp_bytes, 0x00020000);
   const int P = (Li + 1) * (Li + 1);
   const int ntasks = (P + 63) / 64;
-  for (int t = wave; t < ntasks; t += kMarchWaves) {
+  // snake order over the waves: early tasks (small mp) march the longest
+  for (int k = 0; k * kMarchWaves < ntasks; ++k) {
+    const int t = k * kMarchWaves + ((k & 1) ? (kMarchWaves - 1 - wave) : wave);
+    if (t >= ntasks) continue;
     const int p = t * 64 + lane;
     const bool valid = p < P;
     const int pc = valid ? p : P - 1;
@@ -380,47 +407,51 @@ __device__ void march_eval(const float2* __restrict__ sp, int Li, float beta,
                  (float)(bb * (a + 1) + a * (bb + 1)) * term(a, bb) +
                  (float)(a * (a - 1)) * term(a - 2, bb + 2));
     float pd = 0.f, pdp = 0.f, pdpp = 0.f, Cprev = 0.f;
-    float2 U[4], U1[4], U2[4];
+    // U[i] = sum_l sigma'_i d, U1 with d', U2 with d'' (i: merged images (mp, m), (m, mp))
+    float2 U[2], U1[2], U2[2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 2; ++i) {
       U[i] = make_float2(0.f, 0.f);
       U1[i] = U[i];
       U2[i] = U[i];
     }
     const float fm = (float)m, fmp = (float)mp;
+    const float fm2 = fm * fm, fmp2 = fmp * fmp, mmp = fm * fmp;
+    const unsigned voff = 16u * (unsigned)pc;
+    unsigned soff = 4u * (unsigned)mch::pk_offset(lstart);
     for (int l = lstart; l <= Li; ++l) {
       if (l > mp) {
-        const float fl = (float)l;
-        const float Pl = (fl * fl - fm * fm) * (fl * fl - fmp * fmp);
-        const float r = rsqrtf(Pl);
-        const float A = fl * (2.f * fl - 1.f) * r;
-        const float Bc = (l > 1) ? fm * fmp / ((fl - 1.f) * fl) : 0.f;
-        const float xb = x - Bc;
+        const float4 c = lrec[l];
+        const float Pl = (c.x - fm2) * (c.x - fmp2);  // >= 1 for l > mp >= |m|
+        const float r = __builtin_amdgcn_rsqf(Pl);
+        const float A = c.y * r;
+        const float xb = x - mmp * c.z;
         const float nd = A * (xb * d - Cprev * pd);
         const float ndp = A * (xb * dp - sx * d - Cprev * pdp);
         const float ndpp = A * (xb * dpp - 2.f * sx * dp - x * d - Cprev * pdpp);
         pd = d; pdp = dp; pdpp = dpp;
         d = nd; dp = ndp; dpp = ndpp;
-        Cprev = Pl * r / (fl * (2.f * fl + 1.f));
+        Cprev = Pl * r * c.w;
       }
       if (valid && l >= mp) {
-        const float4* q = reinterpret_cast<const float4*>(sp + mch::pk_offset(l) + 4 * p);
-        const float4 s01 = q[0], s23 = q[1];
-        const float2 sv[4] = {make_float2(s01.x, s01.y), make_float2(s01.z, s01.w),
-                              make_float2(s23.x, s23.y), make_float2(s23.z, s23.w)};
+        typedef unsigned v4u __attribute__((ext_vector_type(4)));
+        const v4u q = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
+        const float2 sv[2] = {make_float2(__uint_as_float(q.x), __uint_as_float(q.y)),
+                              make_float2(__uint_as_float(q.z), __uint_as_float(q.w))};
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < 2; ++i) {
           U[i].x += sv[i].x * d;   U[i].y += sv[i].y * d;
           U1[i].x += sv[i].x * dp; U1[i].y += sv[i].y * dp;
           U2[i].x += sv[i].x * dpp; U2[i].y += sv[i].y * dpp;
         }
       }
+      soff += 16u * (unsigned)((l + 1) * (l + 1));
     }
     if (valid) {
-      const int us[4] = {mp, -mp, m, -m};
-      const int vs[4] = {m, -m, mp, -mp};
+      const int us[2] = {mp, m};
+      const int vs[2] = {m, mp};
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < 2; ++i) {
         const int u = us[i], v = vs[i];
         float2 pa = za[abs(u)];
         if (u < 0) pa.y = -pa.y;
@@ -509,11 +540,14 @@ __global__ __launch_bounds__(kMarchThreads) void so3_march_kernel(MarchArgs args
   __shared__ float lgf[2 * kMaxL + 2];
   __shared__ float red[10 * kMarchWaves];
   __shared__ float ang[4];  // trial alpha, beta, gamma
+  __shared__ float4 lrec[kMaxL + 2];
   const int nblk = args.B * args.C;
   const int q = mch::xcd_remap(blockIdx.x, nblk);
   const int b = q / args.C;
   const int tid = threadIdx.x;
   for (int k = tid; k <= 2 * args.Lmax + 1; k += blockDim.x) lgf[k] = lgf_g[k];
+  fill_lrec(lrec, args.Lmax);
+  const unsigned sp_bytes = (unsigned)args.pk_stride * 8u;
   const float2* sp = args.sig_pk + (long long)b * args.pk_stride;
   // thread 0 owns the optimiser state
   float ca = args.alpha[q], cb = args.beta[q], cg = args.gamma[q];
@@ -544,7 +578,7 @@ __global__ __launch_bounds__(kMarchThreads) void so3_march_kernel(MarchArgs args
       phase_table((double)ta, Li, za);
       phase_table((double)tg, Li, zg);
       __syncthreads();
-      march_eval(sp, Li, tb, za, zg, lgf, red, ev);
+      march_eval(sp, sp_bytes, Li, tb, za, zg, lgf, lrec, red, ev);
       if (tid == 0) {
         const bool first = (st == 0);
         const bool acc = first || (ev[0] > best);
@@ -580,14 +614,17 @@ __global__ __launch_bounds__(kMarchThreads) void so3_eval_kernel(
   __shared__ float2 za[kMaxL + 1], zg[kMaxL + 1];
   __shared__ float lgf[2 * kMaxL + 2];
   __shared__ float red[10 * kMarchWaves];
+  __shared__ float4 lrec[kMaxL + 2];
   const int q = blockIdx.x;
   const int b = q / C;
   for (int k = threadIdx.x; k <= 2 * L + 1; k += blockDim.x) lgf[k] = lgf_g[k];
+  fill_lrec(lrec, L);
   phase_table((double)al[q], L, za);
   phase_table((double)ga[q], L, zg);
   __syncthreads();
   float ev[10];
-  march_eval(sig_pk + (long long)b * pk_stride, L, be[q], za, zg, lgf, red, ev);
+  march_eval(sig_pk + (long long)b * pk_stride, (unsigned)pk_stride * 8u, L, be[q], za, zg, lgf, lrec, red,
+             ev);
   if (threadIdx.x == 0)
     for (int k = 0; k < 10; ++k) out[(long long)q * 10 + k] = ev[k];
 }
