@@ -247,6 +247,126 @@ __global__ __launch_bounds__(256) void sofft_slice_kernel(
   }
 }
 
+// Same slice as sofft_slice_kernel for n = 16 R: the phase table
+// E[u][x] = e^{2 pi i u x / n} (|u| < L0) is built once in LDS and serves both
+// DFT stages, and each thread owns an R x R block of (alpha, gamma) points of
+// stage B, so every pair of E / G loads feeds R outputs.
+template <int R>
+__global__ __launch_bounds__(256) void sofft_slice_blk_kernel(
+    const float2* __restrict__ sig, long long ldsig, const float* __restrict__ dgrid,
+    const float2* __restrict__ twiddle, int L0, int nsz, int C, float* __restrict__ part_val,
+    int* __restrict__ part_idx, float* __restrict__ grid_out) {
+  constexpr int n = 16 * R, nn = n * n;
+  extern __shared__ float4 smem4[];
+  float2* smem = reinterpret_cast<float2*>(smem4);
+  const int j = blockIdx.x;
+  const int b = blockIdx.y;
+  const int W = 2 * L0 - 1;
+  float2* F = smem;            // W*W
+  float2* G = F + W * W;       // W*n   G[ui][g]
+  float2* E = G + W * n;       // W*n   E[ui][x]
+  __shared__ float red_v[4];
+  __shared__ int red_i[4];
+  const int tid = threadIdx.x;
+  for (int q = tid; q < W * n; q += 256) {
+    const int ui = q / n, xx = q - (q / n) * n;
+    int k = ((ui - (L0 - 1)) * xx) % n;
+    if (k < 0) k += n;
+    E[q] = twiddle[k];
+  }
+  const float2* s = sig + (long long)b * ldsig;
+  const float* dj = dgrid + (long long)j * nsz;
+  for (int q = tid; q < W * W; q += 256) {
+    const int u = q / W - (L0 - 1), v = q % W - (L0 - 1);
+    const int l0 = abs(u) > abs(v) ? abs(u) : abs(v);
+    float ax = 0.f, ay = 0.f;
+    for (int l = l0; l < L0; ++l) {
+      const int id = mch::dindex(l, u, v);
+      const float dv = dj[id];
+      const float2 sv = s[id];
+      ax += dv * sv.x;
+      ay += dv * sv.y;
+    }
+    F[q] = make_float2(ax, ay);
+  }
+  __syncthreads();
+  // stage A: G(u, g) = sum_v F(u, v) e^{2 pi i v g / n}
+  for (int q = tid; q < W * n; q += 256) {
+    const int ui = q / n, g = q - (q / n) * n;
+    float ax = 0.f, ay = 0.f;
+    for (int vi = 0; vi < W; ++vi) {
+      const float2 f = F[ui * W + vi], t = E[vi * n + g];
+      ax += f.x * t.x - f.y * t.y;
+      ay += f.x * t.y + f.y * t.x;
+    }
+    G[q] = make_float2(ax, ay);
+  }
+  __syncthreads();
+  // stage B: f(a, g) = Re sum_u e^{2 pi i u a / n} G(u, g) / n^2 on the thread's block
+  const int a0 = (tid >> 4) * R, g0 = (tid & 15) * R;
+  float vals[R * R];
+#pragma unroll
+  for (int i = 0; i < R * R; ++i) vals[i] = 0.f;
+  for (int ui = 0; ui < W; ++ui) {
+    float2 e[R], gv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      e[r] = E[ui * n + a0 + r];
+      gv[r] = G[ui * n + g0 + r];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int c = 0; c < R; ++c) vals[r * R + c] += e[r].x * gv[c].x - e[r].y * gv[c].y;
+  }
+  const float inv = 1.0f / ((float)n * (float)n);
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+#pragma unroll
+    for (int c = 0; c < R; ++c) {
+      vals[r * R + c] *= inv;
+      if (grid_out != nullptr)
+        grid_out[((long long)b * n + j) * nn + (a0 + r) * n + g0 + c] = vals[r * R + c];
+    }
+  // C rounds of block argmax (value desc, flat index asc)
+  const int lane = tid & 63, wave = tid >> 6;
+  for (int cc = 0; cc < C; ++cc) {
+    float bv = -INFINITY;
+    int bi = 0x7fffffff;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int c = 0; c < R; ++c) {
+        const int q = (a0 + r) * n + g0 + c;
+        const float v = vals[r * R + c];
+        if (v > bv || (v == bv && q < bi)) { bv = v; bi = q; }
+      }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const float ov = __shfl_xor(bv, off, 64);
+      const int oi = __shfl_xor(bi, off, 64);
+      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
+    __syncthreads();
+    bv = red_v[0]; bi = red_i[0];
+    for (int w = 1; w < 4; ++w) {
+      if (red_v[w] > bv || (red_v[w] == bv && red_i[w] < bi)) { bv = red_v[w]; bi = red_i[w]; }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+      for (int c = 0; c < R; ++c)
+        if ((a0 + r) * n + g0 + c == bi) vals[r * R + c] = -INFINITY;  // taken
+    if (tid == 0) {
+      const long long o = ((long long)b * n + j) * C + cc;
+      part_val[o] = bv;
+      part_idx[o] = (bi == 0x7fffffff) ? -1 : j * nn + bi;
+    }
+  }
+}
+
 // Merge the per-row candidates into the global top-C per particle and convert
 // flat ids to Euler angles (SOFFT.ids_to_angles).
 __global__ __launch_bounds__(256) void sofft_merge_kernel(
@@ -730,6 +850,41 @@ int mch_so3_plan_destroy(void* plan) {
 }
 
 // Top-C grid candidates: writes (B, C) alpha/beta/gamma/score.
+// One launch of the per-beta-row slice kernel (grid_out: full grid or NULL).
+static int launch_sofft_slice(So3Plan* p, const void* sigma, long long ld_sigma, int B, int C,
+                              float* grid_out, hipStream_t stream) {
+  const int W = 2 * p->L0 - 1;
+  const int n = p->n;
+  const size_t lds_blk = (size_t)(W * W + 2 * W * n) * sizeof(float2);
+  if (n % 16 == 0 && n <= 64 && lds_blk <= 64 * 1024) {
+    const dim3 grid(n, B);
+#define MCH_BLK(R)                                                                               \
+  hipLaunchKernelGGL(sofft_slice_blk_kernel<R>, grid, dim3(256), lds_blk, stream,                \
+                     (const float2*)sigma, ld_sigma, p->d_grid, p->twiddle, p->L0, p->nsz, C,     \
+                     p->part_val, p->part_idx, grid_out)
+    switch (n / 16) {
+      case 1: MCH_BLK(1); break;
+      case 2: MCH_BLK(2); break;
+      case 3: MCH_BLK(3); break;
+      default: MCH_BLK(4); break;
+    }
+#undef MCH_BLK
+    return mch::hip_check(hipGetLastError(), "sofft_slice_blk_kernel launch");
+  }
+  const size_t lds = (size_t)(W * W + W * n + n) * sizeof(float2);
+  const int per = (n * n + 255) / 256;
+  if (per <= 16) {
+    hipLaunchKernelGGL(sofft_slice_kernel<16>, dim3(n, B), dim3(256), lds, stream,
+                       (const float2*)sigma, ld_sigma, p->d_grid, p->twiddle, p->L0, n, p->nsz,
+                       C, p->part_val, p->part_idx, grid_out);
+  } else {
+    hipLaunchKernelGGL(sofft_slice_kernel<64>, dim3(n, B), dim3(256), lds, stream,
+                       (const float2*)sigma, ld_sigma, p->d_grid, p->twiddle, p->L0, n, p->nsz,
+                       C, p->part_val, p->part_idx, grid_out);
+  }
+  return mch::hip_check(hipGetLastError(), "sofft_slice_kernel launch");
+}
+
 int mch_sofft_topk(void* plan, const void* sigma, long long ld_sigma, int B, int C,
                    float* alpha, float* beta, float* gamma, float* score, hipStream_t stream) {
   So3Plan* p = (So3Plan*)plan;
@@ -742,19 +897,7 @@ int mch_sofft_topk(void* plan, const void* sigma, long long ld_sigma, int B, int
                    ld_sigma, p->max_batch, p->max_cand, p->nsz);
     return MCH_ERR_SIZE;
   }
-  const int W = 2 * p->L0 - 1;
-  const size_t lds = (size_t)(W * W + W * p->n + p->n) * sizeof(float2);
-  const int per = (p->n * p->n + 255) / 256;
-  if (per <= 16) {
-    hipLaunchKernelGGL(sofft_slice_kernel<16>, dim3(p->n, B), dim3(256), lds, stream,
-                       (const float2*)sigma, ld_sigma, p->d_grid, p->twiddle, p->L0, p->n, p->nsz,
-                       C, p->part_val, p->part_idx, (float*)nullptr);
-  } else {
-    hipLaunchKernelGGL(sofft_slice_kernel<64>, dim3(p->n, B), dim3(256), lds, stream,
-                       (const float2*)sigma, ld_sigma, p->d_grid, p->twiddle, p->L0, p->n, p->nsz,
-                       C, p->part_val, p->part_idx, (float*)nullptr);
-  }
-  int rc = mch::hip_check(hipGetLastError(), "sofft_slice_kernel launch");
+  int rc = launch_sofft_slice(p, sigma, ld_sigma, B, C, nullptr, stream);
   if (rc) return rc;
   hipLaunchKernelGGL(sofft_merge_kernel, dim3(B), dim3(256), (size_t)p->n * C, stream,
                      p->part_val, p->part_idx, p->n, C, alpha, beta, gamma, score, C);
@@ -770,19 +913,7 @@ int mch_sofft_eval(void* plan, const void* sigma, long long ld_sigma, int B, flo
     mch::set_error("mch_sofft_eval: bad sizes (B=%d ld=%lld)", B, ld_sigma);
     return MCH_ERR_SIZE;
   }
-  const int W = 2 * p->L0 - 1;
-  const size_t lds = (size_t)(W * W + W * p->n + p->n) * sizeof(float2);
-  const int per = (p->n * p->n + 255) / 256;
-  if (per <= 16) {
-    hipLaunchKernelGGL(sofft_slice_kernel<16>, dim3(p->n, B), dim3(256), lds, stream,
-                       (const float2*)sigma, ld_sigma, p->d_grid, p->twiddle, p->L0, p->n, p->nsz,
-                       0, p->part_val, p->part_idx, grid);
-  } else {
-    hipLaunchKernelGGL(sofft_slice_kernel<64>, dim3(p->n, B), dim3(256), lds, stream,
-                       (const float2*)sigma, ld_sigma, p->d_grid, p->twiddle, p->L0, p->n, p->nsz,
-                       0, p->part_val, p->part_idx, grid);
-  }
-  return mch::hip_check(hipGetLastError(), "sofft_slice_kernel launch");
+  return launch_sofft_slice(p, sigma, ld_sigma, B, 0, grid, stream);
 }
 
 // Newton march over the level schedule for (B, C) candidates (in/out).
