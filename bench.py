@@ -52,7 +52,7 @@ def parse():
     p.add_argument("--workload", default="box96_L32", choices=sorted(WORKLOADS))
     p.add_argument("--batch", type=int, default=None, help="plain-path batch (default: the workload's)")
     p.add_argument("--tuned-batch", type=int, default=None, help="tuned-path batch (default: the workload's)")
-    p.add_argument("--tuned-precision", default="fp16x3", choices=["bf16x3", "fp16x3"])
+    p.add_argument("--tuned-precision", default="bf16x3", choices=["bf16x3", "fp16x3"])
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP32 path")
     p.add_argument("--snr", type=float, default=0.0)
     p.add_argument("--pool", type=int, default=256, help="distinct particles cycled through")
