@@ -149,6 +149,15 @@ int mch_shift_plan_create(int N, const void* ref_host, float box_radius, int max
                           void** out_plan);
 int mch_shift_plan_destroy(void* plan);
 
+/* Sub-voxel refinement: upsample factor `up` (> 1) over a window of
+ * 2 * radius + 1 voxels, evaluated as the reference's local matrix DFT around
+ * zero (the coarse lag is discarded, as compute_shift does in mode "local");
+ * up == 1 selects integer lags over the whole box.  Replaces the
+ * upsample_factor > 1 branch of compute_shift / _upsampled_dft
+ * (src/matcha/core/ShiftMatcher.py:46-92,241-282) called from
+ * estimate_shift_like_relion (:658-670). */
+int mch_shift_plan_set_upsample(void* plan, int up, int radius);
+
 /* Integer-lag shift search (shift_upsample_factor 1): rotate the reference by
  * rot (B, 3x3 row-major acting on (x, y, z) = array axes (2, 1, 0)), extract
  * the patch at the prior shift_zyx (B, 3) (NULL = 0), cosine-masked

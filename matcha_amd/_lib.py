@@ -51,6 +51,7 @@ _SIGNATURES = [
                                   c_void_p, c_void_p]),
     ("mch_shift_plan_create", c_int, [c_int, c_void_p, c_float, c_int, ctypes.POINTER(c_void_p)]),
     ("mch_shift_plan_destroy", c_int, [c_void_p]),
+    ("mch_shift_plan_set_upsample", c_int, [c_void_p, c_int, c_int]),
     ("mch_shift_search", c_int, [c_void_p, c_void_p, c_ll, c_int, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),
     ("mch_extract_patches", c_int, [c_void_p, c_ll, c_int, c_int, c_void_p, c_void_p, c_ll, c_void_p]),

@@ -9,9 +9,10 @@ the device as complex64; the per-batch search (rotation of the Fourier
 reference, shifted-patch extraction, cosine-masked background, rFFT, cross
 power, inverse FFT, argmax) runs in ``shift.hip``.
 
-Supported: ``shift_upsample_factor`` 1 (integer lags over the whole box, the
-reference's behaviour there).  Sub-voxel refinement (factor > 1) is not
-implemented yet and raises.
+``shift_upsample_factor`` 1 gives integer lags over the whole box; a factor
+up > 1 gives the reference's local sub-voxel refinement (``compute_shift``,
+mode "local": the coarse lag is zeroed and a matrix DFT is evaluated on
+ceil(up * (2 * shift_search_radius + 1))^3 points around zero, 1/up apart).
 """
 from __future__ import annotations
 
@@ -116,6 +117,8 @@ class ShiftMatcher:
                                                   self.max_batch, ctypes.byref(h))
         _lib.check(rc, "mch_shift_plan_create")
         self._plan = h
+        rc = _lib.lib().mch_shift_plan_set_upsample(self._plan, self.upsample, self.radius)
+        _lib.check(rc, "mch_shift_plan_set_upsample")
 
     def __del__(self):
         h = getattr(self, "_plan", None)
@@ -147,8 +150,6 @@ class ShiftMatcher:
         """(shift (B, 3), scores (B,)) float32 on the device."""
         if self._plan is None:
             raise RuntimeError("ShiftMatcher reference is not initialized. Call `set_reference` first.")
-        if self.upsample != 1:
-            raise NotImplementedError("shift_upsample_factor > 1 is not implemented in matcha_amd yet")
         _lib.require_gpu_tensor(subtomos, "subtomos", device=self.device)
         vols = subtomos.float().contiguous()
         B = vols.shape[0]

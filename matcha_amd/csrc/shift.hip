@@ -55,6 +55,13 @@ struct ShiftPlan {
   float* amax_val;    // (B, N)
   int* amax_idx;      // (B, N)
   int ntile_x;
+  // sub-voxel refinement (upsample factor > 1): R = ceil(up (2 r + 1))
+  int up, R;
+  float2* t01;        // (R, N) e^{2 pi i (u - floor(R/2)) fftfreq(N, up)[k]}
+  float2* t2;         // (R, H) the same for the half axis, fftfreq(H, up)
+  float2* Qg;         // (B, H, R, R)
+  float* up_val;      // (B, R)
+  int* up_idx;        // (B, R)
 };
 
 // Raw buffer resource over nbytes at p: loads at byte offsets >= nbytes return 0
@@ -249,32 +256,25 @@ __global__ __launch_bounds__(row_threads<N>()) void shift_fwd_x_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// one (b, kx) plane: FFT y, z; cross power with the rotated reference; inverse
+// one (b, kx) plane: FFT y, z; cross power with the rotated reference
 // ---------------------------------------------------------------------------
+// Leaves P = conj(R) S of plane kx of particle b in LDS, slot (sz, sy) holding
+// frequency (freq(sz), freq(sy)), the plane's shares of ||S||^2 and ||R||^2 in
+// snorm_part / rnorm_part, and ends on a barrier.  `red` holds 2 * NT / 64 floats.
 template <int N>
-__global__ __launch_bounds__(plane_threads<N>()) void shift_yz_kernel(
-    float2* __restrict__ T, const float2* __restrict__ XM, const float* __restrict__ bg_part, float wsum,
+__device__ __forceinline__ void plane_cross_power(
+    float2* __restrict__ plane, float2* __restrict__ tw, float* __restrict__ red, int kx, int b,
+    const float2* __restrict__ T, const float2* __restrict__ XM, const float* __restrict__ bg_part, float wsum,
     const float4* __restrict__ refpair, const float* __restrict__ rot, const float* __restrict__ ctf,
     const float2* __restrict__ twg, float* __restrict__ rnorm_part, float* __restrict__ snorm_part) {
   constexpr int NT = plane_threads<N>(), S = N + 1, H = N / 2 + 1, ZSTEP = NT / N;
-  static_assert(NT % 64 == 0 && NT % N == 0, "whole waves, column split");
-  extern __shared__ float4 smem4[];
-  float2* plane = reinterpret_cast<float2*>(smem4);  // [z][y], stride S
-  float2* tw = plane + N * S;
-  __shared__ float red[2][NT / 64];
-  // XCD-aware order: the 8 XCDs take consecutive ranges of (b, kx) planes, so
-  // neighbouring planes of one particle (which sample neighbouring planes of
-  // the reference) share one L2
-  const int nplanes = gridDim.x;
-  const int lb = mch::xcd_remap(blockIdx.x, nplanes);
-  const int kx = lb % H, b = lb / H;
   for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
   // background average of particle b; every wave sums the partials in the same order
   float bgs = 0.f;
   for (int zz = threadIdx.x & 63; zz < N; zz += 64) bgs += bg_part[b * N + zz];
   bgs = mch::wave_sum(bgs);
   const float bg = wsum > 0.f ? bgs / wsum : 0.f;
-  float2* P = T + ((long long)b * H + kx) * N * N;
+  const float2* P = T + ((long long)b * H + kx) * N * N;
   const float2* M = XM + (long long)kx * N * N;
   // thread column y (fixed), rows z = zb, zb + ZSTEP, ...
   const int cy = threadIdx.x % N, zb = threadIdx.x / N;
@@ -364,19 +364,226 @@ __global__ __launch_bounds__(plane_threads<N>()) void shift_yz_kernel(
   ns = mch::wave_sum(ns);
   nr = mch::wave_sum(nr);
   if ((threadIdx.x & 63) == 0) {
-    red[0][threadIdx.x >> 6] = ns;
-    red[1][threadIdx.x >> 6] = nr;
+    red[threadIdx.x >> 6] = ns;
+    red[NT / 64 + (threadIdx.x >> 6)] = nr;
   }
   __syncthreads();  // also publishes the cross power to every thread
   if (threadIdx.x == 0) {
     float a = 0.f, bsum = 0.f;
-    for (int w = 0; w < NT / 64; ++w) { a += red[0][w]; bsum += red[1][w]; }
+    for (int w = 0; w < NT / 64; ++w) { a += red[w]; bsum += red[NT / 64 + w]; }
     snorm_part[b * H + kx] = a;
     rnorm_part[b * H + kx] = bsum;
   }
+}
+
+// ---------------------------------------------------------------------------
+// integer lags: inverse FFT of the plane along z and y, back to HBM
+// ---------------------------------------------------------------------------
+template <int N>
+__global__ __launch_bounds__(plane_threads<N>()) void shift_yz_kernel(
+    float2* __restrict__ T, const float2* __restrict__ XM, const float* __restrict__ bg_part, float wsum,
+    const float4* __restrict__ refpair, const float* __restrict__ rot, const float* __restrict__ ctf,
+    const float2* __restrict__ twg, float* __restrict__ rnorm_part, float* __restrict__ snorm_part) {
+  constexpr int NT = plane_threads<N>(), S = N + 1, H = N / 2 + 1, ZSTEP = NT / N;
+  static_assert(NT % 64 == 0 && NT % N == 0, "whole waves, column split");
+  extern __shared__ float4 smem4[];
+  float2* plane = reinterpret_cast<float2*>(smem4);  // [z][y], stride S
+  float2* tw = plane + N * S;
+  __shared__ float red[2 * NT / 64];
+  // XCD-aware order: the 8 XCDs take consecutive ranges of (b, kx) planes, so
+  // neighbouring planes of one particle (which sample neighbouring planes of
+  // the reference) share one L2
+  const int lb = mch::xcd_remap(blockIdx.x, gridDim.x);
+  const int kx = lb % H, b = lb / H;
+  plane_cross_power<N>(plane, tw, red, kx, b, T, XM, bg_part, wsum, refpair, rot, ctf, twg, rnorm_part,
+                       snorm_part);
   mch::fft::inverse<N, N, NT, 1>(plane, 1, S, tw);  // along z
   mch::fft::inverse<N, N, NT, 1>(plane, S, 1, tw);  // along y
+  float2* P = T + ((long long)b * H + kx) * N * N;
+  const int cy = threadIdx.x % N, zb = threadIdx.x / N;
   for (int zz = zb; zz < N; zz += ZSTEP) P[zz * N + cy] = plane[zz * S + cy];
+}
+
+// ---------------------------------------------------------------------------
+// sub-voxel lags (shift_upsample_factor > 1): local matrix DFT of the cross power
+// ---------------------------------------------------------------------------
+// The reference (compute_shift, mode "local") zeroes the coarse shift and
+// evaluates Re sum_k P(k) prod_d e^{2 pi i (u_d - R/2) f_d(k_d)} on an R^3 grid
+// (R = ceil(up (2 r + 1)), f_d = fftfreq(n_d, up), n = N, N, N/2+1).  Here the
+// plane kernel contracts y and z of each kx plane to R x R (Q), and
+// shift_up_kx_kernel contracts kx and takes the first argmax.
+// t01: (R, N) phases of the y / z axes in natural frequency order.
+template <int N>
+__global__ __launch_bounds__(plane_threads<N>()) void shift_yz_up_kernel(
+    const float2* __restrict__ T, const float2* __restrict__ XM, const float* __restrict__ bg_part,
+    float wsum, const float4* __restrict__ refpair, const float* __restrict__ rot,
+    const float* __restrict__ ctf, const float2* __restrict__ twg, const float2* __restrict__ t01, int R,
+    float2* __restrict__ Qg, float* __restrict__ rnorm_part, float* __restrict__ snorm_part) {
+  constexpr int NT = plane_threads<N>(), S = N + 1, H = N / 2 + 1;
+  static_assert(NT % 64 == 0 && NT % N == 0, "whole waves, column split");
+  extern __shared__ float4 smem4[];
+  float2* plane = reinterpret_cast<float2*>(smem4);  // [z][y], stride S
+  float2* tw = plane + N * S;
+  float2* ts = tw + N;                                // (R, N): phases in slot order
+  __shared__ float red[2 * NT / 64];
+  const int lb = mch::xcd_remap(blockIdx.x, gridDim.x);
+  const int kx = lb % H, b = lb / H;
+  for (int q = threadIdx.x; q < R * N; q += NT) {
+    const int u = q / N, k = q - (q / N) * N;
+    ts[u * N + mch::fft::slot_of_freq<N>(k)] = t01[q];
+  }
+  plane_cross_power<N>(plane, tw, red, kx, b, T, XM, bg_part, wsum, refpair, rot, ctf, twg, rnorm_part,
+                       snorm_part);
+  // Y[sz][u1] = sum_sy P[sz][sy] e[u1][sy], 4 x 4 register blocks
+  const int nbz = N / 4, nbu = (R + 3) / 4;
+  constexpr int MAXB = 2;  // blocks per thread (N / 4 * ceil(R / 4) <= 2 NT for R <= 2 N)
+  float2 y[MAXB][16];
+#pragma unroll
+  for (int bi = 0; bi < MAXB; ++bi) {
+    const int blk = threadIdx.x + bi * NT;
+    if (blk >= nbz * nbu) continue;
+    const int z4 = (blk / nbu) * 4, u4 = (blk - (blk / nbu) * nbu) * 4;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) y[bi][e] = make_float2(0.f, 0.f);
+    for (int sy = 0; sy < N; ++sy) {
+      float2 pv[4], ev[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        pv[r] = plane[(z4 + r) * S + sy];
+        ev[r] = (u4 + r < R) ? ts[(u4 + r) * N + sy] : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          y[bi][r * 4 + c].x += pv[r].x * ev[c].x - pv[r].y * ev[c].y;
+          y[bi][r * 4 + c].y += pv[r].x * ev[c].y + pv[r].y * ev[c].x;
+        }
+    }
+  }
+  __syncthreads();
+  // Y replaces P in LDS: [sz][u1], stride R + 1 (R <= N)
+  const int YS = R + 1;
+#pragma unroll
+  for (int bi = 0; bi < MAXB; ++bi) {
+    const int blk = threadIdx.x + bi * NT;
+    if (blk >= nbz * nbu) continue;
+    const int z4 = (blk / nbu) * 4, u4 = (blk - (blk / nbu) * nbu) * 4;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (u4 + c < R) plane[(z4 + r) * YS + u4 + c] = y[bi][r * 4 + c];
+  }
+  __syncthreads();
+  // Q[u0][u1] = sum_sz e[u0][sz] Y[sz][u1], 2 x 2 register blocks
+  const int nb2 = (R + 1) / 2;
+  float2* Q = Qg + ((long long)b * H + kx) * R * R;
+  for (int blk = threadIdx.x; blk < nb2 * nb2; blk += NT) {
+    const int u0 = (blk / nb2) * 2, u1 = (blk - (blk / nb2) * nb2) * 2;
+    float2 q[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f),
+                   make_float2(0.f, 0.f)};
+    for (int sz = 0; sz < N; ++sz) {
+      const float2 e0 = u0 < R ? ts[u0 * N + sz] : make_float2(0.f, 0.f);
+      const float2 e1 = u0 + 1 < R ? ts[(u0 + 1) * N + sz] : make_float2(0.f, 0.f);
+      const float2 y0 = plane[sz * YS + u1];
+      const float2 y1 = u1 + 1 < R ? plane[sz * YS + u1 + 1] : make_float2(0.f, 0.f);
+      q[0].x += e0.x * y0.x - e0.y * y0.y; q[0].y += e0.x * y0.y + e0.y * y0.x;
+      q[1].x += e0.x * y1.x - e0.y * y1.y; q[1].y += e0.x * y1.y + e0.y * y1.x;
+      q[2].x += e1.x * y0.x - e1.y * y0.y; q[2].y += e1.x * y0.y + e1.y * y0.x;
+      q[3].x += e1.x * y1.x - e1.y * y1.y; q[3].y += e1.x * y1.y + e1.y * y1.x;
+    }
+    if (u0 < R && u1 < R) Q[u0 * R + u1] = q[0];
+    if (u0 < R && u1 + 1 < R) Q[u0 * R + u1 + 1] = q[1];
+    if (u0 + 1 < R && u1 < R) Q[(u0 + 1) * R + u1] = q[2];
+    if (u0 + 1 < R && u1 + 1 < R) Q[(u0 + 1) * R + u1 + 1] = q[3];
+  }
+}
+
+// One workgroup per (particle, u0): Re sum_kx e2[u2][kx] Q[kx][u0][u1] over the
+// R x R (u1, u2) slab, first argmax (flat index (u0 R + u1) R + u2).
+__global__ __launch_bounds__(256) void shift_up_kx_kernel(const float2* __restrict__ Qg, int H, int R,
+                                                          const float2* __restrict__ t2,
+                                                          float* __restrict__ part_val,
+                                                          int* __restrict__ part_idx) {
+  extern __shared__ float4 smem4[];
+  float2* qs = reinterpret_cast<float2*>(smem4);  // (H, R): Q[kx][u0][.]
+  float2* es = qs + H * R;                        // (R, H)
+  __shared__ float rv[4];
+  __shared__ int ri[4];
+  const int u0 = blockIdx.x, b = blockIdx.y;
+  for (int q = threadIdx.x; q < H * R; q += 256) {
+    const int kx = q / R, u1 = q - (q / R) * R;
+    qs[q] = Qg[(((long long)b * H + kx) * R + u0) * R + u1];
+  }
+  for (int q = threadIdx.x; q < R * H; q += 256) es[q] = t2[q];
+  __syncthreads();
+  float bv = -INFINITY;
+  int bi = 0x7fffffff;
+  for (int q = threadIdx.x; q < R * R; q += 256) {
+    const int u1 = q / R, u2 = q - (q / R) * R;
+    float acc = 0.f;
+    for (int kx = 0; kx < H; ++kx) {
+      const float2 e = es[u2 * H + kx], v = qs[kx * R + u1];
+      acc += e.x * v.x - e.y * v.y;
+    }
+    const int flat = (u0 * R + u1) * R + u2;
+    if (acc > bv || (acc == bv && flat < bi)) { bv = acc; bi = flat; }
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const float ov = __shfl_xor(bv, off, 64);
+    const int oi = __shfl_xor(bi, off, 64);
+    if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+  }
+  if ((threadIdx.x & 63) == 0) { rv[threadIdx.x >> 6] = bv; ri[threadIdx.x >> 6] = bi; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w)
+      if (rv[w] > rv[0] || (rv[w] == rv[0] && ri[w] < ri[0])) { rv[0] = rv[w]; ri[0] = ri[w]; }
+    part_val[(long long)b * R + u0] = rv[0];
+    part_idx[(long long)b * R + u0] = ri[0];
+  }
+}
+
+// argmax over the u0 partials; shift = (index - floor(R / 2)) / up per axis,
+// flipped to the reference's (z, y, x) labels; score / (||S|| ||R||)
+__global__ __launch_bounds__(64) void shift_up_final_kernel(const float* __restrict__ part_val,
+                                                            const int* __restrict__ part_idx, int R,
+                                                            float up, const float* __restrict__ rn,
+                                                            const float* __restrict__ sn, int H,
+                                                            float* __restrict__ out_shift,
+                                                            float* __restrict__ out_score) {
+  const int b = blockIdx.x;
+  float bv = -INFINITY;
+  int bi = 0x7fffffff;
+  float r2 = 0.f, s2 = 0.f;
+  for (int t = threadIdx.x; t < R; t += 64) {
+    const float v = part_val[(long long)b * R + t];
+    const int i = part_idx[(long long)b * R + t];
+    if (v > bv || (v == bv && i < bi)) { bv = v; bi = i; }
+  }
+  for (int t = threadIdx.x; t < H; t += 64) {
+    r2 += rn[(long long)b * H + t];
+    s2 += sn[(long long)b * H + t];
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const float ov = __shfl_xor(bv, off, 64);
+    const int oi = __shfl_xor(bi, off, 64);
+    if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+  }
+  r2 = mch::wave_sum(r2);
+  s2 = mch::wave_sum(s2);
+  if (threadIdx.x == 0) {
+    const int u0 = bi / (R * R), u1 = (bi / R) % R, u2 = bi % R;
+    const float half = floorf((float)R * 0.5f);
+    // the reference returns flip(shift) of the array-axis shift
+    out_shift[b * 3 + 0] = ((float)u2 - half) / up;
+    out_shift[b * 3 + 1] = ((float)u1 - half) / up;
+    out_shift[b * 3 + 2] = ((float)u0 - half) / up;
+    out_score[b] = bv / (sqrtf(s2) * sqrtf(r2));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -602,7 +809,31 @@ void free_plan(ShiftPlan* p) {
   (void)hipFree(p->refpair); (void)hipFree(p->twid); (void)hipFree(p->lut); (void)hipFree(p->XM);
   (void)hipFree(p->T); (void)hipFree(p->rnorm_part); (void)hipFree(p->snorm_part);
   (void)hipFree(p->bg_part); (void)hipFree(p->amax_val); (void)hipFree(p->amax_idx);
+  (void)hipFree(p->t01); (void)hipFree(p->t2); (void)hipFree(p->Qg); (void)hipFree(p->up_val);
+  (void)hipFree(p->up_idx);
   delete p;
+}
+
+template <int N> size_t plane_up_lds(int R) { return plane_lds<N>() + (size_t)R * N * sizeof(float2); }
+
+// LDS of the refinement plane kernel for R against the device, attribute raised
+template <int N>
+int configure_up(int R) {
+  int dev = 0, maxlds = 0;
+  int rc = mch::hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  rc = mch::hip_check(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
+                      "hipDeviceGetAttribute");
+  if (rc) return rc;
+  if (R > N || plane_up_lds<N>(R) + 1024 > (size_t)maxlds) {
+    mch::set_error("mch_shift_plan_set_upsample: refinement grid R=%d too large for box %d (R <= N and "
+                   "%zu B of LDS)", R, N, plane_up_lds<N>(R));
+    return MCH_ERR_UNSUPPORTED;
+  }
+  return mch::hip_check(hipFuncSetAttribute((const void*)shift_yz_up_kernel<N>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)plane_up_lds<N>(R)),
+                        "hipFuncSetAttribute(shift_yz_up_kernel)");
 }
 
 struct SearchArgs {
@@ -625,6 +856,19 @@ int launch_search(ShiftPlan* p, const SearchArgs& a, hipStream_t stream) {
                      row_lds<N>() + (size_t)p->lut_n * sizeof(float), stream, a.subtomos, a.ld_vol,
                      a.shift_zyx, p->lut, p->lut_lo, p->lut_n, p->twid, p->T, p->bg_part);
   if ((rc = mch::hip_check(hipGetLastError(), "shift_fwd_x_kernel"))) return rc;
+  if (p->up > 1) {
+    const int R = p->R;
+    hipLaunchKernelGGL(shift_yz_up_kernel<N>, dim3(H * a.B), dim3(plane_threads<N>()), plane_up_lds<N>(R),
+                       stream, p->T, p->XM, p->bg_part, p->wsum, p->refpair, a.rot, a.ctf, p->twid, p->t01, R,
+                       p->Qg, p->rnorm_part, p->snorm_part);
+    if ((rc = mch::hip_check(hipGetLastError(), "shift_yz_up_kernel"))) return rc;
+    hipLaunchKernelGGL(shift_up_kx_kernel, dim3(R, a.B), dim3(256), (size_t)2 * H * R * sizeof(float2), stream,
+                       p->Qg, H, R, p->t2, p->up_val, p->up_idx);
+    if ((rc = mch::hip_check(hipGetLastError(), "shift_up_kx_kernel"))) return rc;
+    hipLaunchKernelGGL(shift_up_final_kernel, dim3(a.B), dim3(64), 0, stream, p->up_val, p->up_idx, R,
+                       (float)p->up, p->rnorm_part, p->snorm_part, H, a.out_shift, a.out_score);
+    return mch::hip_check(hipGetLastError(), "shift_up_final_kernel");
+  }
   hipLaunchKernelGGL(shift_yz_kernel<N>, dim3(H * a.B), dim3(plane_threads<N>()), plane_lds<N>(), stream,
                      p->T, p->XM, p->bg_part, p->wsum, p->refpair, a.rot, a.ctf, p->twid, p->rnorm_part,
                      p->snorm_part);
@@ -696,6 +940,7 @@ int mch_shift_plan_create(int N, const void* ref_host /* complex64 (N,N,N) */, f
   p->width = 5.0f;
   p->radius_p = box_radius + 5.0f;
   p->ntile_x = N;
+  p->up = 1;
   const int H = p->H;
   // background weight per integer r2 (monotone: 0, shell, 1) and its table
   const int r2max = 3 * (N / 2) * (N / 2);
@@ -792,6 +1037,68 @@ fail:
 
 int mch_shift_plan_destroy(void* plan) {
   free_plan((ShiftPlan*)plan);
+  return MCH_OK;
+}
+
+// Sub-voxel refinement of the shift search: the reference's local matrix-DFT
+// with upsample factor `up` over a window of 2 * radius + 1 voxels (up == 1:
+// integer lags of the full cross-correlation, the default).
+int mch_shift_plan_set_upsample(void* plan, int up, int radius) {
+  ShiftPlan* p = (ShiftPlan*)plan;
+  if (!p || up < 1 || radius < 0) {
+    mch::set_error("mch_shift_plan_set_upsample: bad arguments (up=%d radius=%d)", up, radius);
+    return MCH_ERR_ARG;
+  }
+  if (up == 1) {
+    p->up = 1;
+    return MCH_OK;
+  }
+  const int N = p->N, H = p->H;
+  const int R = (int)ceil((double)up * (double)(2 * radius + 1));
+  int rc = MCH_OK;
+  switch (N) {
+    case 16: rc = configure_up<16>(R); break;
+    case 32: rc = configure_up<32>(R); break;
+    case 48: rc = configure_up<48>(R); break;
+    case 64: rc = configure_up<64>(R); break;
+    case 96: rc = configure_up<96>(R); break;
+    case 128: rc = configure_up<128>(R); break;
+    default: rc = MCH_ERR_UNSUPPORTED; break;
+  }
+  if (rc) return rc;
+  // phases in FP64: e^{2 pi i (u - floor(R/2)) f}, f = fftfreq(n, d=up)
+  auto fftfreq = [](int k, int n, int d) {
+    const int kk = (k <= (n - 1) / 2) ? k : k - n;
+    return (double)kk / ((double)n * (double)d);
+  };
+  const double off = floor(R / 2.0);
+  std::vector<float2> t01((size_t)R * N), t2((size_t)R * H);
+  for (int u = 0; u < R; ++u) {
+    for (int k = 0; k < N; ++k) {
+      const double a = 2.0 * M_PI * ((double)u - off) * fftfreq(k, N, up);
+      t01[(size_t)u * N + k] = make_float2((float)cos(a), (float)sin(a));
+    }
+    for (int k = 0; k < H; ++k) {
+      const double a = 2.0 * M_PI * ((double)u - off) * fftfreq(k, H, up);
+      t2[(size_t)u * H + k] = make_float2((float)cos(a), (float)sin(a));
+    }
+  }
+  (void)hipFree(p->t01); (void)hipFree(p->t2); (void)hipFree(p->Qg); (void)hipFree(p->up_val);
+  (void)hipFree(p->up_idx);
+  p->t01 = p->t2 = p->Qg = nullptr;
+  p->up_val = nullptr;
+  p->up_idx = nullptr;
+#define MCH_TRY(expr, what) do { if ((rc = mch::hip_check((expr), what)) != MCH_OK) return rc; } while (0)
+  MCH_TRY(hipMalloc(&p->t01, t01.size() * sizeof(float2)), "hipMalloc(t01)");
+  MCH_TRY(hipMemcpy(p->t01, t01.data(), t01.size() * sizeof(float2), hipMemcpyHostToDevice), "copy t01");
+  MCH_TRY(hipMalloc(&p->t2, t2.size() * sizeof(float2)), "hipMalloc(t2)");
+  MCH_TRY(hipMemcpy(p->t2, t2.data(), t2.size() * sizeof(float2), hipMemcpyHostToDevice), "copy t2");
+  MCH_TRY(hipMalloc(&p->Qg, (size_t)p->max_batch * H * R * R * sizeof(float2)), "hipMalloc(Qg)");
+  MCH_TRY(hipMalloc(&p->up_val, (size_t)p->max_batch * R * sizeof(float)), "hipMalloc(up_val)");
+  MCH_TRY(hipMalloc(&p->up_idx, (size_t)p->max_batch * R * sizeof(int)), "hipMalloc(up_idx)");
+#undef MCH_TRY
+  p->up = up;
+  p->R = R;
   return MCH_OK;
 }
 

@@ -145,3 +145,37 @@ def test_alternation_recovers_rotation_and_shift(gpu_device):
     assert np.max(err) < 1.0, err
     # particles were moved by +t along array axes (0,1,2); the shift comes back flipped
     assert np.array_equal(shift.cpu().numpy().astype(int), t[:, ::-1])
+
+
+@pytest.mark.parametrize("N,B,up,radius", [(64, 3, 8, 2), (32, 2, 4, 2), (48, 2, 2, 3)])
+def test_upsampled_refinement_matches_oracle(gpu_device, N, B, up, radius):
+    # sub-voxel refinement (the reference's local matrix DFT, its production setting is
+    # up 8, radius 2): shifts in multiples of 1/up must match the FP64 oracle exactly
+    rng = np.random.default_rng(200 + N + up)
+    tmpl = fle.blob_volume(rng, N, 15, sigma_range=(1.5, 3.5))
+    from matcha_amd.core.ShiftMatcher import tracker_to_matrices
+    q = _rand_quats(rng, B)
+    rot = tracker_to_matrices(q)
+    vols = np.stack([np.roll(tmpl, tuple(rng.integers(-radius, radius + 1, 3)), axis=(0, 1, 2))
+                     + 0.4 * rng.standard_normal(tmpl.shape) for _ in range(B)]).astype(np.float32)
+    prior = np.zeros((B, 1, 3))
+    prior[0, 0] = (0.5, -1.0, 0.0)
+    sm = _matcher(gpu_device, N, B, shift_upsample_factor=up, shift_search_radius=radius)
+    sm.set_reference(tmpl)
+    sh, sc = sm.search_shifts(subtomos=torch.tensor(vols, device=gpu_device), ctfs=None,
+                              shift_zyx=torch.tensor(prior, device=gpu_device), rotation_tracker=q)
+    refF = S.load_reference_full(tmpl, N, 1.0).astype(np.complex64).astype(np.complex128)
+    rsh, rsc = S.estimate_shift(refF, vols.astype(np.float64), prior, rot, N=N, upsample=up, radius=radius)
+    got = sh.cpu().numpy()
+    assert np.array_equal(got, rsh.astype(np.float32)), (got, rsh)
+    assert np.all(got * up == np.round(got * up))
+    assert np.max(np.abs(sc.cpu().numpy() - rsc) / np.abs(rsc)) < 1e-4
+
+
+def test_upsampled_refinement_unsupported_grid_raises(gpu_device):
+    # the refinement grid R = ceil(up (2 r + 1)) must fit the box (R <= N)
+    from matcha_amd.core.ShiftMatcher import ShiftMatcher
+    sm = ShiftMatcher({"N": 32, "num_subtomograms_per_batch": 1, "shift_upsample_factor": 16,
+                       "shift_search_radius": 2}, gpu_device)
+    with pytest.raises(RuntimeError):
+        sm.set_reference(np.zeros((32, 32, 32), dtype=np.float32))

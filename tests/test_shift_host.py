@@ -1,5 +1,6 @@
 """Host-side shift preparation checked on the CPU."""
 import numpy as np
+import pytest
 
 from oracle import fle, shift as S
 
@@ -51,3 +52,23 @@ def test_shift_frame_of_search_orientations():
         t = rng.integers(-3, 4, 3)
         sh, sc = S.estimate_shift(refF, render(c @ R.T + t)[None], np.zeros((1, 1, 3)), (P @ R.T @ P)[None], N=N)
         assert np.array_equal(sh[0], t[::-1].astype(float)) and sc[0] > 0.95
+
+
+@pytest.mark.parametrize("up,s0,s1,s2", [(8, 0.375, -1.125, 0.0), (4, -0.5, 0.75, 0.25), (2, 1.0, 0.0, -0.5)])
+def test_oracle_upsampled_refinement_closed_form(up, s0, s1, s2):
+    # ref = 1 and sub = e^{-2 pi i (k0' s0 + k1' s1) / N} e^{-2 pi i k2' s2 / H}, with signed
+    # frequencies (numpy fftfreq order) and, along the half axis, the reference's own
+    # convention fftfreq(N/2 + 1): the local matrix DFT is then a product of Dirichlet kernels
+    # whose real part peaks exactly at u = floor(R/2) + up * s, so the refinement returns
+    # (s0, s1, s2) with score 1.  A wrong sign, offset or frequency convention moves the peak.
+    from oracle import shift as S
+    N = 16
+    H = N // 2 + 1
+    kf = np.fft.fftfreq(N) * N
+    ramp = np.exp(-2j * np.pi * (kf[:, None] * s0 + kf[None, :] * s1) / N)
+    ramp2 = np.exp(-2j * np.pi * np.fft.fftfreq(H) * s2)
+    sub = ramp[:, :, None] * ramp2[None, None, :]
+    ref = np.ones((N, N, H), dtype=np.complex128)
+    shift, score = S.compute_shift_local(ref, sub, upsample=up, window=2 * 2 + 1)
+    assert np.array_equal(shift, np.array([s0, s1, s2]))
+    assert score == pytest.approx(1.0, rel=1e-12)
