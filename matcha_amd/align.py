@@ -24,10 +24,14 @@ class Alternation:
         self.num_alternations = int(num_alternations)
         self.final_shift = final_shift
         self._patches = None
+        self.history = []
 
     @torch.no_grad()
     def run(self, subtomos: torch.Tensor):
-        """(alpha, beta, gamma, norm_score, shift_zyx (B,3), shift_score) for the batch."""
+        """(alpha, beta, gamma, norm_score, shift_zyx (B,3), shift_score) for the batch.
+
+        ``self.history`` keeps that tuple for every alternation, in order: the
+        per-alternation records the STAR output is joined from."""
         B = subtomos.shape[0]
         dev = subtomos.device
         shift = torch.zeros((B, 3), dtype=torch.float32, device=dev)
@@ -35,12 +39,13 @@ class Alternation:
             self._patches = torch.empty_like(subtomos, dtype=torch.float32)
         vols = subtomos
         out = None
+        self.history = []
         for it in range(self.num_alternations):
             if it > 0:
                 vols = extract_shifted_patches(subtomos, shift, out=self._patches)
             sigma, norms = self.ccm.get_sigma(vols)
             a, b, g, s = self.rot.search_orientations(sigma)
-            nscore = s / (norms[:B] * self.ccm.bh_norm_template)
+            nscore = s / (norms[:B] * self.ccm.bh_norm_template).clamp_min(1e-12)
             last = it == self.num_alternations - 1
             sscore = None
             if not last or self.final_shift:
@@ -49,4 +54,5 @@ class Alternation:
                                                    shift_zyx=None, rotation_tracker=rb)
                 shift = shift + ds
             out = (a, b, g, nscore, shift, sscore)
+            self.history.append(out)
         return out

@@ -3,6 +3,7 @@
 Bar: integer shifts bit-exact; scores within 1e-4 relative (FP32 path).
 """
 import numpy as np
+import pandas as pd
 import pytest
 import torch
 
@@ -123,7 +124,9 @@ def test_extract_patches_matches_oracle(gpu_device):
         assert np.max(np.abs(got[b] - ref)) < 1e-5
 
 
-def test_alternation_recovers_rotation_and_shift(gpu_device):
+def test_alternation_recovers_rotation_and_shift(gpu_device, tmp_path):
+    from scipy.spatial.transform import Rotation
+    from matcha_amd import relion
     from matcha_amd.align import Alternation
     from matcha_amd.core.CrossCorrelationMatcher import CrossCorrelationMatcher
     from matcha_amd.core.Matcha import Matcha
@@ -140,11 +143,27 @@ def test_alternation_recovers_rotation_and_shift(gpu_device):
     sm.set_reference(wl.template.cpu().numpy())
     # three alternations, the reference default (num_alternations: 3): at L = 16 the first
     # orientation is too coarse for every shift, the second fixes the shifts
-    a, b, g, ns, shift, ss = Alternation(ccm, m, sm, num_alternations=3).run(vols)
+    pipe = Alternation(ccm, m, sm, num_alternations=3)
+    a, b, g, ns, shift, ss = pipe.run(vols)
     err = geodesic_deg(R, euler_zyz_to_matrix(*(x.cpu().numpy().astype(np.float64) for x in (a, b, g))))
     assert np.max(err) < 1.0, err
     # particles were moved by +t along array axes (0,1,2); the shift comes back flipped
     assert np.array_equal(shift.cpu().numpy().astype(int), t[:, ::-1])
+    # RELION STAR output of the same run: Rz(psi) Ry(tilt) Rz(rot) = Rx(pi) P R P Rx(pi)
+    # with P the x<->z exchange, origins = -shift * pixel size
+    names = [f"TS_01/{i}" for i in range(B)]
+    star_in = {"optics": pd.DataFrame({"rlnImagePixelSize": [1.5]}),
+               "particles": pd.DataFrame({"rlnTomoParticleName": names})}
+    rec = relion.records_from_history(names, pipe.history)
+    assert relion.join_to_star(star_in, rec, tmp_path / "out.star", num_alternations=3)
+    df = relion.read_star(tmp_path / "out.star")["particles"]
+    P = np.array([[0.0, 0, 1], [0, 1, 0], [1, 0, 0]])
+    RX = np.diag([1.0, -1.0, -1.0])
+    M = Rotation.from_euler("ZYZ", df[["rlnAnglePsi", "rlnAngleTilt", "rlnAngleRot"]].to_numpy(),
+                            degrees=True).as_matrix()
+    assert np.max(geodesic_deg(M, RX @ P @ R @ P @ RX)) < 1.0
+    org = df[["rlnOriginXAngst", "rlnOriginYAngst", "rlnOriginZAngst"]].to_numpy()
+    np.testing.assert_allclose(org, -1.5 * t[:, ::-1], atol=1e-6)
 
 
 @pytest.mark.parametrize("N,B,up,radius", [(64, 3, 8, 2), (32, 2, 4, 2), (48, 2, 2, 3)])
