@@ -32,6 +32,13 @@ import time
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+
+# HBM bytes the tuned projection GEMM fetches per launch, measured with
+# rocprofv3 --pmc FETCH_SIZE and scaled by 2.0 (FETCH_SIZE of a device copy of
+# 512 MiB reads 256 MiB); keyed by (workload, batch).  Its writes are the K-split
+# partials, splits x rows x Ntot x 4 B (WRITE_SIZE reads exactly that), added
+# per run for the split the autotuner chose.
+GEMM16_FETCH_BYTES = {("box96_L32", 64): 584368 * 1024 * 2}
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
@@ -59,6 +66,8 @@ def parse():
     p.add_argument("--alternations", type=int, default=2)
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--copy-calibration-mib", type=int, default=0,
+                   help="untimed device copy of this size before the run (HBM counter calibration)")
     p.add_argument("--cpu-sample-coeffs", type=int, default=100)
     return p.parse_args()
 
@@ -185,6 +194,13 @@ def main():
     Bmax = max(Bp, Bt)
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.copy_calibration_mib > 0:
+        src = torch.empty(args.copy_calibration_mib << 18, dtype=torch.float32, device=dev).normal_()
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     t_setup = time.perf_counter()
     wl = Workload(w["N"], w["n_blobs"], w["sigma"], args.seed, dev, mask_radius=w["radius"])
     ccm = CrossCorrelationMatcher(w["N"], dev, batchsize=Bmax, bandlimit=w["bandlimit"], radius=w["radius"])
@@ -221,6 +237,10 @@ def main():
     achieved = alg_bytes / gemm_t / 1e9
     flops16 = 3 * 2.0 * (2 * Bt) * (2 * fle_.V0) * sum(fle_.class_cols)
     fle_.set_compute_precision(args.tuned_precision)
+    tiles = int(_lib_depth(fle_))
+    fetch = GEMM16_FETCH_BYTES.get((args.workload, Bt))
+    rows = -(-2 * Bt // 128) * 128
+    traffic = fetch + (tiles // 100) * rows * fle_.Ntot * 4.0 if fetch is not None else None
 
     dtype = {"bf16x3": "bf16x3+f32acc", "fp16x3": "fp16x3+f32acc"}[args.tuned_precision]
     out = {
@@ -247,11 +267,12 @@ def main():
             "tuned_over_plain": (tuned["volumes_per_s"] / plain["volumes_per_s"]) if plain else None,
             "meets_accuracy": tuned["median_err_deg"] <= 1.0 and (plain is None or plain["median_err_deg"] <= 1.0),
             "gemm_ms": {"tuned": gemm_t * 1e3, "plain_fp32": gemm_plain * 1e3 if gemm_plain else None},
-            "gemm_pipeline_depth": int(_lib_depth(fle_)),
+            "gemm_tiles": {"k_splits": tiles // 100, "waves_per_workgroup": tiles % 100 // 10,
+                           "lds_stages": tiles % 10},
             "setup_s": t_setup,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": None,
+                     "frac": achieved / 8000.0, "traffic": traffic,
                      "kernel": f"fle_gemm16_kernel<{args.tuned_precision}>", "avg_ms": gemm_t * 1e3,
                      "bytes_per_launch": alg_bytes,
                      "mfma_tflops": flops16 / gemm_t / 1e12, "mfma_peak_tflops": 2500.0},

@@ -120,9 +120,9 @@ int mch_fle_gemm_only(void* plan, int B, hipStream_t stream);
 int mch_fle_set_precision(void* plan, int precision);
 
 /* Tile configuration the 16-bit GEMM was autotuned to when the plan first
- * switched to a 16-bit mode: 10 * (waves per workgroup, 2 or 4) + pipeline
- * depth (2 or 3 LDS stages); 0 before that.  No reference counterpart
- * (reporting aid). */
+ * switched to a 16-bit mode: 100 * (K splits) + 10 * (waves per workgroup,
+ * 2 or 4) + pipeline depth (2 or 3 LDS stages); 0 before that.  No reference
+ * counterpart (reporting aid). */
 int mch_fle_tuned_depth(void* plan);
 
 /* Index table (lmax+1, 2*lmax+1, kdim) of coefficient positions for

@@ -50,9 +50,12 @@ __host__ __device__ __forceinline__ int pk_size(int L) { return pk_offset(L + 1)
 // XCD-aware block remap: the hardware hands consecutive workgroups to the 8
 // XCDs round-robin; map them so that a contiguous range of logical blocks
 // (e.g. all candidates of one particle) lands on one XCD and shares its L2.
+// Bijective for any grid size: XCDs 0..r-1 take q+1 logical blocks, the rest
+// q (n = 8q + r).
 __device__ __forceinline__ int xcd_remap(int bid, int nblocks) {
-  if (nblocks % 8 != 0) return bid;
-  return (bid % 8) * (nblocks / 8) + bid / 8;
+  const int q = nblocks >> 3, r = nblocks & 7;
+  const int x = bid & 7, i = bid >> 3;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
 }
 
 }  // namespace mch

@@ -21,6 +21,7 @@
 //      to the reference order.
 #include "common.hpp"
 
+#include <algorithm>
 #include <type_traits>
 #include <vector>
 
@@ -58,7 +59,9 @@ struct FlePlan {
   float* colscale;            // (Ntot)
   float* rowmax;              // (Mpad) max |X| of each GEMM row (FP16 scaling)
   float* part16;              // (splits16, Mpad, Ntot)
-  int splits16;
+  int splits16;               // K splits part16 holds (the largest autotune candidate)
+  int ksplit16;               // K splits in use (autotuned, <= splits16)
+  int cus;                    // compute units of the plan's device
 };
 
 struct CorrPlan {
@@ -342,8 +345,12 @@ __device__ __forceinline__ void wg_barrier_keep_loads() {
   asm volatile("" ::: "memory");
 }
 
+// 16 B per lane into LDS at lds_wave_base + 16 * lane.  CPOL is the cache
+// policy: 0 = default, 2 = non-temporal (the streamed W^T, read once, so that
+// it does not evict the X slices the neighbouring column tiles re-read).
+template <int CPOL = 0>
 __device__ __forceinline__ void lds_dma16(const void* g, u32x4* lds_wave_base) {
-  __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds_wave_base, 16, 0, 0);
+  __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds_wave_base, 16, 0, CPOL);
 }
 
 template <bool FP16, int NST, int NW>
@@ -390,7 +397,7 @@ __global__ __launch_bounds__(64 * NW) void fle_gemm16_kernel(
     for (int t = 0; t < AP; ++t) lds_dma16(asrc[t] + k0, base + adst[t]);
 #pragma unroll
     for (int q = 0; q < 4 * G; ++q)
-      lds_dma16(wsrc + (long long)stage * 1024 + q * 64, base + 1024 + wave * G * 256 + q * 64);
+      lds_dma16<2>(wsrc + (long long)stage * 1024 + q * 64, base + 1024 + wave * G * 256 + q * 64);
   };
   constexpr int PER_STAGE = AP + 4 * G;  // DMAs per wave per stage
   floatx4 acc[8][CB];
@@ -633,6 +640,38 @@ void free_fle_plan(FlePlan* p) {
 }
 
 // the projection GEMM of the plan's precision on its X operand
+// Autotune candidates of the 16-bit GEMM: pipeline depth x waves per
+// workgroup x rounds of resident workgroups the K split fills.
+constexpr int kGemm16Candidates = 12;
+inline int gemm16_nst(int cand) { return 2 + (cand & 1); }
+inline int gemm16_nw(int cand) { return (cand & 2) ? 2 : 4; }
+inline int gemm16_rounds(int cand) { return 1 + (cand >> 2); }
+
+// Resident workgroups per CU of one tile configuration.
+int gemm16_occupancy(int nst, int nw, int* occ) {
+  const size_t lds = (size_t)nst * kStageU4 * sizeof(u32x4);
+  const void* fn = nullptr;
+  if (nw == 4) fn = nst == 3 ? (const void*)fle_gemm16_kernel<false, 3, 4> : (const void*)fle_gemm16_kernel<false, 2, 4>;
+  else fn = nst == 3 ? (const void*)fle_gemm16_kernel<false, 3, 2> : (const void*)fle_gemm16_kernel<false, 2, 2>;
+  int n = 0;
+  const int rc = mch::hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64 * nw, lds),
+                                "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  *occ = n > 0 ? n : 1;
+  return rc;
+}
+
+// K split that fills `rounds` rounds of `slots` resident workgroups with
+// the (column tile x row tile) grid, rounded so that every split but the
+// last has the same number of K blocks.
+int gemm16_splits(const FlePlan* p, int slots, int rounds) {
+  const int ntiles = p->Ntot / BN16 * (p->Mpad / BM16);
+  const int kblocks = p->Kpad / BK16;
+  int s = rounds * slots / ntiles;
+  s = std::min(std::max(s, 1), kblocks);
+  const int per = (kblocks + s - 1) / s;
+  return (kblocks + per - 1) / per;
+}
+
 int launch_gemm(FlePlan* p, int B, hipStream_t stream, int* splits_out) {
   const int mt = (2 * B + BM - 1) / BM;
   if (p->precision == kFp32) {
@@ -645,7 +684,7 @@ int launch_gemm(FlePlan* p, int B, hipStream_t stream, int* splits_out) {
     return mch::hip_check(hipGetLastError(), "fle_gemm_kernel launch");
   }
   const int kblocks = p->Kpad / BK16;
-  const int kchunk = ((kblocks + p->splits16 - 1) / p->splits16) * BK16;
+  const int kchunk = ((kblocks + p->ksplit16 - 1) / p->ksplit16) * BK16;
   const int splits = (p->Kpad + kchunk - 1) / kchunk;
   const int ntn = p->Ntot / BN16;
   const dim3 grid(ntn * mt * splits);
@@ -784,20 +823,6 @@ int mch_fle_set_precision(void* plan, int precision) {
     MCH_TRY(hipMalloc(&p->Wt16, 2 * nw * sizeof(uint16_t)), "hipMalloc(Wt16)");
     MCH_TRY(hipMalloc(&p->colscale, (size_t)p->Ntot * sizeof(float)), "hipMalloc(colscale)");
     MCH_TRY(hipMalloc(&p->rowmax, (size_t)p->Mpad * sizeof(float)), "hipMalloc(rowmax)");
-    // split K so that the grid is ~two full rounds of resident workgroups
-    // (2 per CU on 256 CUs)
-    const int ntiles = p->Ntot / BN16 * (p->Mpad / BM16);
-    int splits = (1024 + ntiles - 1) / ntiles;
-    const int kblocks = p->Kpad / BK16;
-    if (splits > kblocks) splits = kblocks;
-    if (splits < 1) splits = 1;
-    p->splits16 = splits;
-    MCH_TRY(hipMalloc(&p->part16, (size_t)splits * p->Mpad * p->Ntot * sizeof(float)), "hipMalloc(part16)");
-    const size_t nx = (size_t)kClasses * p->Mpad * p->Kpad;
-    MCH_TRY(hipMalloc(&p->Xhi, nx * sizeof(uint16_t)), "hipMalloc(Xhi)");
-    MCH_TRY(hipMalloc(&p->Xlo, nx * sizeof(uint16_t)), "hipMalloc(Xlo)");
-    MCH_TRY(hipMemset(p->Xhi, 0, nx * sizeof(uint16_t)), "memset(Xhi)");
-    MCH_TRY(hipMemset(p->Xlo, 0, nx * sizeof(uint16_t)), "memset(Xlo)");
     const int lds3 = 3 * kStageU4 * (int)sizeof(u32x4);
     MCH_TRY(hipFuncSetAttribute((const void*)fle_gemm16_kernel<false, 3, 4>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds3), "hipFuncSetAttribute");
@@ -807,6 +832,25 @@ int mch_fle_set_precision(void* plan, int precision) {
                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds3), "hipFuncSetAttribute");
     MCH_TRY(hipFuncSetAttribute((const void*)fle_gemm16_kernel<true, 3, 2>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, lds3), "hipFuncSetAttribute");
+    // partials for the largest K split any autotune candidate uses
+    int cus = 0, dev = 0;
+    MCH_TRY(hipGetDevice(&dev), "hipGetDevice");
+    MCH_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
+    p->cus = cus > 0 ? cus : 256;
+    int smax = 1;
+    for (int cand = 0; cand < kGemm16Candidates; ++cand) {
+      int occ = 0;
+      if ((rc = gemm16_occupancy(gemm16_nst(cand), gemm16_nw(cand), &occ))) return rc;
+      smax = std::max(smax, gemm16_splits(p, occ * p->cus, gemm16_rounds(cand)));
+    }
+    p->splits16 = smax;
+    p->ksplit16 = smax;
+    MCH_TRY(hipMalloc(&p->part16, (size_t)smax * p->Mpad * p->Ntot * sizeof(float)), "hipMalloc(part16)");
+    const size_t nx = (size_t)kClasses * p->Mpad * p->Kpad;
+    MCH_TRY(hipMalloc(&p->Xhi, nx * sizeof(uint16_t)), "hipMalloc(Xhi)");
+    MCH_TRY(hipMalloc(&p->Xlo, nx * sizeof(uint16_t)), "hipMalloc(Xlo)");
+    MCH_TRY(hipMemset(p->Xhi, 0, nx * sizeof(uint16_t)), "memset(Xhi)");
+    MCH_TRY(hipMemset(p->Xlo, 0, nx * sizeof(uint16_t)), "memset(Xlo)");
     p->nst16 = 2;
     p->nw16 = 4;
   }
@@ -830,18 +874,23 @@ int mch_fle_set_precision(void* plan, int precision) {
     MCH_TRY(hipDeviceSynchronize(), "building the 16-bit projection tables");
     p->table_precision = precision;
     p->precision = precision;
-    // autotune the pipeline depth on the full batch (timing does not depend on
-    // the operand values): 2 stages fit two workgroups per CU, 3 stages keep
-    // more HBM traffic in flight per workgroup
+    // autotune on the full batch (timing does not depend on the operand
+    // values): pipeline depth (2 stages fit two workgroups per CU, 3 keep more
+    // HBM traffic in flight per workgroup), waves per workgroup, and the K
+    // split as 1, 2 or 3 full rounds of resident workgroups (a partial last
+    // round leaves most CUs idle while it runs)
     hipEvent_t e0, e1;
     MCH_TRY(hipEventCreate(&e0), "hipEventCreate");
     MCH_TRY(hipEventCreate(&e1), "hipEventCreate");
     float best = 1e30f;
-    int best_nst = 2, best_nw = 4;
-    for (int cand = 0; cand < 4; ++cand) {
-      const int nst = 2 + (cand & 1), nw = (cand & 2) ? 2 : 4;
+    int best_nst = 2, best_nw = 4, best_split = p->splits16;
+    for (int cand = 0; cand < kGemm16Candidates; ++cand) {
+      const int nst = gemm16_nst(cand), nw = gemm16_nw(cand);
+      int occ = 0;
+      if ((rc = gemm16_occupancy(nst, nw, &occ))) break;
       p->nst16 = nst;
       p->nw16 = nw;
+      p->ksplit16 = gemm16_splits(p, occ * p->cus, gemm16_rounds(cand));
       int splits = 0;
       for (int rep = 0; rep < 2; ++rep)
         if ((rc = launch_gemm(p, p->max_batch, 0, &splits))) break;
@@ -854,24 +903,25 @@ int mch_fle_set_precision(void* plan, int precision) {
       MCH_TRY(hipEventSynchronize(e1), "autotune");
       float ms = 0.f;
       (void)hipEventElapsedTime(&ms, e0, e1);
-      if (ms < best) { best = ms; best_nst = nst; best_nw = nw; }
+      if (ms < best) { best = ms; best_nst = nst; best_nw = nw; best_split = p->ksplit16; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     p->nst16 = best_nst;
     p->nw16 = best_nw;
+    p->ksplit16 = best_split;
   }
 #undef MCH_TRY
   p->precision = precision;
   return MCH_OK;
 }
 
-// Tile configuration the 16-bit GEMM was tuned to: 10 * waves per workgroup
-// + pipeline depth (0 before any 16-bit mode).
+// Tile configuration the 16-bit GEMM was tuned to: 100 * K splits + 10 *
+// waves per workgroup + pipeline depth (0 before any 16-bit mode).
 int mch_fle_tuned_depth(void* plan) {
   FlePlan* p = (FlePlan*)plan;
-  return p ? 10 * p->nw16 + p->nst16 : 0;
+  return p && p->nst16 ? 100 * p->ksplit16 + 10 * p->nw16 + p->nst16 : 0;
 }
 
 // a (B, ne) complex64 rows of stride lda = B^H f for volumes (B, N^3) float32.
