@@ -107,7 +107,7 @@ def run_path(name, torch, dev, w, args, pool_vols, pool_R, pool_t, precision, B,
     from matcha_amd.core.ShiftMatcher import euler_to_shift_rotations
     from matcha_amd.synthetic import euler_zyz_to_matrix, geodesic_deg
     ccm.fle.set_compute_precision(precision)
-    pipe = Alternation(ccm, matcher, sm, num_alternations=args.alternations)
+    pipe = Alternation(ccm, matcher, sm, num_alternations=args.alternations, mode="global")
     P = pool_vols.shape[0]
     nb = max(1, P // B)
     batches = [pool_vols[i * B:(i + 1) * B] for i in range(nb)]

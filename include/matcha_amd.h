@@ -185,6 +185,35 @@ int mch_extract_patches(const float* vols, long long ld_vol, int B, int N, const
 int mch_euler_to_shift_rot(const float* alpha, const float* beta, const float* gamma, int B,
                            float* rot, hipStream_t stream);
 
+/* One alternation's particles: the N^3 patch of each (Nv^3) particle centred
+ * on Nv/2 and displaced by shift (x, y, z) (trilinear, zeros outside),
+ * normalised by its mean and population standard deviation over the binary
+ * sphere |i - N/2| < radius, times that mask, then rotated: out[i] = that
+ * volume at c + M (i - c), c = N/2, (x, y, z) index coordinates with x the
+ * last axis, trilinear with zeros outside.  vols (B, Nv^3) float32 with
+ * stride ld_vol; shift (B, 3) float32 or NULL; rot (B, 9) row-major M = R(tracker);
+ * work B * N^3 floats (needed when shift is given; used for fractional
+ * shifts); partial B * N * 4 floats; out (B, N^3) with stride ld_out.
+ * Replaces BatchProcessor._extract_rotated_batch
+ * (src/matcha/align_subtomograms.py:240-252): extract_subtomogram_patch_batch
+ * with normalize=True (src/matcha/utils/io_utils.py:94-178) and rotate_volumes
+ * (src/matcha/utils/volume_rotation.py:9-125). */
+int mch_extract_rotated(const float* vols, long long ld_vol, int Nv, int B, int N, const float* shift,
+                        const float* rot, double radius, float* work, float* partial, float* out,
+                        long long ld_out, hipStream_t stream);
+
+/* Rotation trackers (B, 4) float32 (w, x, y, z), updated in place when
+ * alpha/beta/gamma are given: tracker <- normalised(tracker * conj(q~)), q~ the
+ * z-y-z quaternion of (alpha, beta, gamma) with its x and z components
+ * exchanged.  Then rot_particle = R(tracker) (the next alternation's particle
+ * rotation) and rot_reference = R(tracker)^T (the shift search's reference
+ * rotation), each (B, 9) row-major or NULL.  Replaces update_rotation_estimate
+ * (src/matcha/utils/rotation_ops.py:23-34) and the Euler round trips of
+ * rotate_volumes (src/matcha/utils/volume_rotation.py:30,95) and search_shifts
+ * (src/matcha/core/ShiftMatcher.py:913). */
+int mch_tracker_update(const float* alpha, const float* beta, const float* gamma, int B, float* tracker,
+                       float* rot_particle, float* rot_reference, hipStream_t stream);
+
 /* Host FP64 Wigner d^l_{m'm}(beta), 0 <= l <= L, D layout (host memory).
  * Replaces spherical.Wigner(L).d (used by src/matcha/core/SOFFT.py:31-39). */
 int mch_wigner_d_host(double beta, int L, double* out);

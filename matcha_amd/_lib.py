@@ -56,6 +56,10 @@ _SIGNATURES = [
                                  c_void_p, c_void_p, c_void_p]),
     ("mch_extract_patches", c_int, [c_void_p, c_ll, c_int, c_int, c_void_p, c_void_p, c_ll, c_void_p]),
     ("mch_euler_to_shift_rot", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    ("mch_extract_rotated", c_int, [c_void_p, c_ll, c_int, c_int, c_int, c_void_p, c_void_p, c_double,
+                                    c_void_p, c_void_p, c_void_p, c_ll, c_void_p]),
+    ("mch_tracker_update", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                   c_void_p]),
     ("mch_wigner_d_host", c_int, [c_double, c_int, ctypes.POINTER(c_double)]),
 ]
 

@@ -47,6 +47,29 @@ __host__ __device__ __forceinline__ int pk_offset(int l) {
 }
 __host__ __device__ __forceinline__ int pk_size(int L) { return pk_offset(L + 1); }
 
+// Trilinear sample of an N^3 volume (z, y, x index order) at index
+// coordinates, corners outside the volume counting as zero (grid_sample with
+// align_corners=True and zero padding, in index units).
+__device__ __forceinline__ float tri_sample_real(const float* __restrict__ v, int N, float z, float y,
+                                                 float x) {
+  const float fz0 = floorf(z), fy0 = floorf(y), fx0 = floorf(x);
+  const int z0 = (int)fz0, y0 = (int)fy0, x0 = (int)fx0;
+  const float fz = z - fz0, fy = y - fy0, fx = x - fx0;
+  float acc = 0.f;
+#pragma unroll
+  for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+      for (int dx = 0; dx < 2; ++dx) {
+        const int zi = z0 + dz, yi = y0 + dy, xi = x0 + dx;
+        if (zi < 0 || zi >= N || yi < 0 || yi >= N || xi < 0 || xi >= N) continue;
+        const float w = (dz ? fz : 1.f - fz) * (dy ? fy : 1.f - fy) * (dx ? fx : 1.f - fx);
+        acc += w * v[((long long)zi * N + yi) * N + xi];
+      }
+  return acc;
+}
+
 // XCD-aware block remap: the hardware hands consecutive workgroups to the 8
 // XCDs round-robin; map them so that a contiguous range of logical blocks
 // (e.g. all candidates of one particle) lands on one XCD and shares its L2.

@@ -79,26 +79,6 @@ __device__ __forceinline__ float load_f1(__amdgpu_buffer_rsrc_t r, unsigned off)
   return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
 }
 
-__device__ __forceinline__ float tri_sample_real(const float* __restrict__ v, int N, float z, float y,
-                                                 float x) {
-  const float fz0 = floorf(z), fy0 = floorf(y), fx0 = floorf(x);
-  const int z0 = (int)fz0, y0 = (int)fy0, x0 = (int)fx0;
-  const float fz = z - fz0, fy = y - fy0, fx = x - fx0;
-  float acc = 0.f;
-#pragma unroll
-  for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-    for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-      for (int dx = 0; dx < 2; ++dx) {
-        const int zi = z0 + dz, yi = y0 + dy, xi = x0 + dx;
-        if (zi < 0 || zi >= N || yi < 0 || yi >= N || xi < 0 || xi >= N) continue;
-        const float w = (dz ? fz : 1.f - fz) * (dy ? fy : 1.f - fy) * (dx ? fx : 1.f - fx);
-        acc += w * v[((long long)zi * N + yi) * N + xi];
-      }
-  return acc;
-}
-
 // Shifted patch value at array index (a0, a1, a2): vol[a + s] with trilinear
 // sampling for fractional s and the border slice zeroed as the reference does.
 struct ShiftSpec {
@@ -118,7 +98,7 @@ __device__ __forceinline__ float patch_value(const float* __restrict__ v, int N,
     if (z < 0 || z >= N || y < 0 || y >= N || x < 0 || x >= N) return 0.f;
     return v[((long long)z * N + y) * N + x];
   }
-  return tri_sample_real(v, N, a0 + sp.s[0], a1 + sp.s[1], a2 + sp.s[2]);
+  return mch::tri_sample_real(v, N, a0 + sp.s[0], a1 + sp.s[1], a2 + sp.s[2]);
 }
 
 __device__ void make_shift_spec(const float* __restrict__ shift_zyx, int b, int N, ShiftSpec& sp) {

@@ -163,14 +163,19 @@ class AlignmentRecords:
 def records_from_history(paths, history, half: str = "half1", rotation_tracker=None,
                          records: AlignmentRecords = None) -> AlignmentRecords:
     """One record per particle and alternation from an alternation history of
-    (alpha, beta, gamma, norm_score, shift, shift_score) tuples.  Every
-    orientation search is global, so each alternation's tracker is the prior
-    tracker (identity by default) updated by that alternation's angles
-    (align_subtomograms.py:330-361 with the particles re-rotated each time)."""
+    (alpha, beta, gamma, norm_score, shift, shift_score[, tracker]) tuples
+    (align_subtomograms.py:330-361).  With a 7th entry (the reference-mode
+    harness) that composed tracker is stored; without one (global searches) the
+    tracker is the prior tracker (identity by default) updated by that
+    alternation's angles."""
     records = AlignmentRecords() if records is None else records
     prior = identity_tracker(len(paths)) if rotation_tracker is None else np.asarray(rotation_tracker)
-    for k, (a, b, g, nscore, shift, sscore) in enumerate(history):
-        tracker, _ = update_rotation_estimate(_to_numpy(a), _to_numpy(b), _to_numpy(g), prior)
+    for k, entry in enumerate(history):
+        a, b, g, nscore, shift, sscore = entry[:6]
+        if len(entry) > 6:
+            tracker = np.asarray(_to_numpy(entry[6]), dtype=np.float64).reshape(len(paths), 4)
+        else:
+            tracker, _ = update_rotation_estimate(_to_numpy(a), _to_numpy(b), _to_numpy(g), prior)
         ss = np.full(len(paths), np.nan) if sscore is None else _to_numpy(sscore)
         records.add(paths, nscore, ss, shift, tracker, half=half, alternation_index=k)
     return records

@@ -124,7 +124,8 @@ def test_extract_patches_matches_oracle(gpu_device):
         assert np.max(np.abs(got[b] - ref)) < 1e-5
 
 
-def test_alternation_recovers_rotation_and_shift(gpu_device, tmp_path):
+@pytest.mark.parametrize("mode", ["reference", "global"])
+def test_alternation_recovers_rotation_and_shift(gpu_device, tmp_path, mode):
     from scipy.spatial.transform import Rotation
     from matcha_amd import relion
     from matcha_amd.align import Alternation
@@ -143,10 +144,23 @@ def test_alternation_recovers_rotation_and_shift(gpu_device, tmp_path):
     sm.set_reference(wl.template.cpu().numpy())
     # three alternations, the reference default (num_alternations: 3): at L = 16 the first
     # orientation is too coarse for every shift, the second fixes the shifts
-    pipe = Alternation(ccm, m, sm, num_alternations=3)
+    pipe = Alternation(ccm, m, sm, num_alternations=3, mode=mode)
     a, b, g, ns, shift, ss = pipe.run(vols)
-    err = geodesic_deg(R, euler_zyz_to_matrix(*(x.cpu().numpy().astype(np.float64) for x in (a, b, g))))
-    assert np.max(err) < 1.0, err
+    P = np.array([[0.0, 0, 1], [0, 1, 0], [1, 0, 0]])
+    if mode == "global":
+        est = euler_zyz_to_matrix(*(x.cpu().numpy().astype(np.float64) for x in (a, b, g)))
+    else:
+        # the composed tracker: R(tracker) = P (R_1 R_2 ...) P on the particles' axis-0-first frame
+        est = P @ relion.quat_to_matrix(pipe.tracker.q.cpu().numpy().astype(np.float64)) @ P
+        assert len(pipe.history[-1]) == 7
+    err = geodesic_deg(R, est)
+    if mode == "global":
+        assert np.max(err) < 1.0, err
+    else:
+        # the reference's loop searches the residual rotation of pre-rotated particles,
+        # i.e. near beta = 0 where the Euler Newton step is singular and skipped: its
+        # accuracy at L = 16 is grid-limited for some particles
+        assert np.max(err) < 3.0 and np.median(err) < 1.5, err
     # particles were moved by +t along array axes (0,1,2); the shift comes back flipped
     assert np.array_equal(shift.cpu().numpy().astype(int), t[:, ::-1])
     # RELION STAR output of the same run: Rz(psi) Ry(tilt) Rz(rot) = Rx(pi) P R P Rx(pi)
@@ -157,13 +171,37 @@ def test_alternation_recovers_rotation_and_shift(gpu_device, tmp_path):
     rec = relion.records_from_history(names, pipe.history)
     assert relion.join_to_star(star_in, rec, tmp_path / "out.star", num_alternations=3)
     df = relion.read_star(tmp_path / "out.star")["particles"]
-    P = np.array([[0.0, 0, 1], [0, 1, 0], [1, 0, 0]])
     RX = np.diag([1.0, -1.0, -1.0])
     M = Rotation.from_euler("ZYZ", df[["rlnAnglePsi", "rlnAngleTilt", "rlnAngleRot"]].to_numpy(),
                             degrees=True).as_matrix()
-    assert np.max(geodesic_deg(M, RX @ P @ R @ P @ RX)) < 1.0
+    # the STAR angles carry exactly the estimate (FP32 tracker -> FP64 host conversion)
+    assert np.max(geodesic_deg(M, RX @ P @ est @ P @ RX)) < 1e-3
     org = df[["rlnOriginXAngst", "rlnOriginYAngst", "rlnOriginZAngst"]].to_numpy()
     np.testing.assert_allclose(org, -1.5 * t[:, ::-1], atol=1e-6)
+
+
+def test_reference_mode_first_alternation_matches_global(gpu_device):
+    # identity tracker, zero shift: the resampled particle is the particle normalised over
+    # the mask, which rescales sigma and changes only l = 0, so the search agrees
+    from matcha_amd.align import Alternation
+    from matcha_amd.core.CrossCorrelationMatcher import CrossCorrelationMatcher
+    from matcha_amd.core.Matcha import Matcha
+    from matcha_amd.core.ShiftMatcher import ShiftMatcher
+    from matcha_amd.synthetic import Workload, euler_zyz_to_matrix, geodesic_deg
+    N, B = 64, 8
+    wl = Workload(N, 40, (1.5, 4.0), 4, gpu_device, mask_radius=28)
+    vols, R, t = wl.particles(B, 0, 0.0)
+    ccm = CrossCorrelationMatcher(N, gpu_device, batchsize=B, bandlimit=16, radius=28)
+    ccm.set_template(wl.template)
+    m = Matcha(B, gpu_device, 16, {"Li": [8, 12], "oversampling_factor_K": 2})
+    sm = ShiftMatcher({"N": N, "num_subtomograms_per_batch": B, "shift_upsample_factor": 1}, gpu_device)
+    sm.set_reference(wl.template.cpu().numpy())
+    outs = {}
+    for mode in ("reference", "global"):
+        a, b, g, ns, shift, ss = Alternation(ccm, m, sm, num_alternations=1, mode=mode).run(vols)
+        outs[mode] = euler_zyz_to_matrix(*(x.cpu().numpy().astype(np.float64) for x in (a, b, g)))
+        assert np.array_equal(shift.cpu().numpy().astype(int), np.zeros((B, 3), int))
+    assert np.max(geodesic_deg(outs["reference"], outs["global"])) < 0.25
 
 
 @pytest.mark.parametrize("N,B,up,radius", [(64, 3, 8, 2), (32, 2, 4, 2), (48, 2, 2, 3)])
