@@ -66,6 +66,10 @@ def parse():
     p.add_argument("--alternations", type=int, default=2)
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--diff", action="store_true",
+                   help="also measure the differentiable score (256 particles, box 128, L = 48)")
+    p.add_argument("--diff-batch", type=int, default=256)
+    p.add_argument("--diff-steps", type=int, default=3)
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="untimed device copy of this size before the run (HBM counter calibration)")
     p.add_argument("--cpu-sample-coeffs", type=int, default=100)
@@ -175,6 +179,72 @@ def _lib_depth(fle_):
     return _lib.lib().mch_fle_tuned_depth(fle_._plan)
 
 
+def diff_workload(torch, dev, args):
+    """Differentiable score at box 128, bandlimit 48, L = 48 with FP32
+    coefficients: time of score + gradients in (omega, t, aT) for the batch,
+    and a check of the rotation/shift gradient against FP32 central
+    differences (1e-3 rad / 1e-2 px) on a few particles."""
+    from matcha_amd.diff import ScoreEngine
+    from matcha_amd.synthetic import Workload
+    from scipy.spatial.transform import Rotation
+    N, bl, radius, L, B = 128, 48, 60, 48, args.diff_batch
+    t0 = time.perf_counter()
+    wl = Workload(N, 200, (1.0, 5.0), args.seed + 11, dev, mask_radius=radius)
+    eng = ScoreEngine(N, bl, radius, L, B, dev)
+    aT = eng.template_coefficients(wl.template)
+    chunks = [wl.particles(min(64, B - i), 6, -6.0) for i in range(0, B, 64)]
+    vols = torch.cat([c[0] for c in chunks])
+    R = np.concatenate([c[1] for c in chunks])
+    tt = np.concatenate([c[2] for c in chunks])
+    rng = np.random.default_rng(args.seed + 12)
+    # poses near the truth (a few degrees, a fraction of a voxel off)
+    omega_np = np.stack([(Rotation.from_rotvec(rng.normal(0, 0.03, 3)) * Rotation.from_matrix(r)).as_rotvec()
+                         for r in R])
+    shift_np = tt[:, ::-1] + rng.uniform(-0.4, 0.4, (B, 3))
+    omega = torch.tensor(omega_np, dtype=torch.float32, device=dev, requires_grad=True)
+    shift = torch.tensor(shift_np, dtype=torch.float32, device=dev, requires_grad=True)
+    a = aT.clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+
+    def step():
+        omega.grad = shift.grad = a.grad = None
+        s = eng.score(omega, shift, a, vols)
+        s.sum().backward()
+        return s
+
+    step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.diff_steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t1) / args.diff_steps * 1e3
+    # FP32 central differences on the first few particles
+    nchk = min(4, B)
+    with torch.no_grad():
+        base, _ = eng.score_and_gradients(omega[:nchk], shift[:nchk], aT, vols[:nchk])
+        fd = torch.zeros((nchk, 6), device=dev, dtype=torch.float64)
+        for j, h in ((0, 1e-3), (1, 1e-2)):
+            for i in range(3):
+                e = torch.zeros(3, device=dev)
+                e[i] = h
+                args_p = [omega[:nchk] + (e if j == 0 else 0), shift[:nchk] + (e if j == 1 else 0)]
+                args_m = [omega[:nchk] - (e if j == 0 else 0), shift[:nchk] - (e if j == 1 else 0)]
+                sp = eng.score_and_gradients(args_p[0], args_p[1], aT, vols[:nchk])[0][:, 0].double()
+                sm = eng.score_and_gradients(args_m[0], args_m[1], aT, vols[:nchk])[0][:, 0].double()
+                fd[:, 3 * j + i] = (sp - sm) / (2 * h)
+        an = base[:, 1:].double()
+        rel = ((an - fd).abs().amax(dim=1) / fd.abs().amax(dim=1)).max().item()
+    return {
+        "workload": f"{B} particles, box {N}, FLE bandlimit {bl} radius {radius}, L = {L}, -6 dB, FP32 coefficients",
+        "ms_per_step": ms, "particles_per_s": B / ms * 1e3,
+        "step": "score + d/d(axis-angle, shift, template coefficients), forward and backward",
+        "fd_check": {"particles": nchk, "steps": "1e-3 rad / 1e-2 px", "max_rel_err": rel},
+        "setup_s": setup,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -277,6 +347,8 @@ def main():
                      "bytes_per_launch": alg_bytes,
                      "mfma_tflops": flops16 / gemm_t / 1e12, "mfma_peak_tflops": 2500.0},
     }
+    if args.diff:
+        out["differentiation"] = diff_workload(torch, dev, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args)
     print(json.dumps(out), flush=True)

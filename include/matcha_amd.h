@@ -214,6 +214,43 @@ int mch_extract_rotated(const float* vols, long long ld_vol, int Nv, int B, int 
 int mch_tracker_update(const float* alpha, const float* beta, const float* gamma, int B, float* tracker,
                        float* rot_particle, float* rot_reference, hipStream_t stream);
 
+/* ------------------------------------------------- differentiable score --- */
+/* The score s = Re sum_{l<=L} D^l(R(omega)) sigma^l(aT, aV(t)) of the reference's
+ * search (src/matcha/core/Matcha.py:184,288) with R(omega) = exp([omega]x) and aV(t)
+ * the coefficients of the particle patch at shift t, differentiated with respect to
+ * omega, t and aT.  The reference has Euler-angle derivatives only
+ * (src/matcha/core/WignerDGradients.py:247-302); these entry points have no direct
+ * counterpart. */
+
+/* omega (B, 3) float32 -> z-y-z Euler angles (B, 3) FP64 of exp([omega]x) and the
+ * right Jacobian J_r (B, 9) FP64 of exp. */
+int mch_score_pose(const float* omega, int B, double* angles, double* jr, hipStream_t stream);
+
+/* Patch at shift t (B, 3) (reference shift order: t0 along the last axis) of each
+ * (Nv^3) particle, trilinear with zeros outside, and its derivatives in t0, t1, t2:
+ * rows b, B+b, 2B+b, 3B+b of out (4B, N^3) with row stride ld_out.  The patch is
+ * extract_subtomogram_patch_batch's (src/matcha/utils/io_utils.py:113-178). */
+int mch_shift_tangent(const float* vols, long long ld_vol, int Nv, int B, int N, const float* t, float* out,
+                      long long ld_out, hipStream_t stream);
+
+/* Score and gradients: sigma (4B rows, stride ld_sigma >= Dsize(lmax)) of the
+ * patch and of its three t-derivatives (as from mch_compute_sigma of their
+ * coefficients), idx the coefficient index table (lmax+1, 2lmax+1, kdim) of
+ * mch_corr_plan_create (device copy), angles/jr from mch_score_pose, partial
+ * B*(L+1)*8 floats of scratch.  out (B, 7) float32: s, ds/domega (3), ds/dt (3).
+ * With G (B rows, stride ldG >= ne, complex64) and aV (the patch coefficients,
+ * B rows, stride ld_aV): the per-particle template adjoint
+ * G_{l k,-v} = sum_u conj(D^l_uv) (-1)^{u+v} aV_{l k,-u} (zero above L). */
+int mch_score_gradients(const int* idx, int lmax, int kdim, const void* sigma, long long ld_sigma, int B,
+                        const void* aV, long long ld_aV, int L, const double* angles, const double* jr,
+                        float* partial, float* out, void* G, long long ldG, hipStream_t stream);
+
+/* Backward of the score under upstream weights (B,): d_omega (B, 3) and d_t (B, 3)
+ * (either may be NULL) from out of mch_score_gradients, and with G the template
+ * gradient d_aT (ne,) complex64 = sum_b weights[b] G_b (d/dRe + i d/dIm). */
+int mch_score_backward(const float* weights, const float* grads, int B, const void* G, long long ldG, int ne,
+                       float* d_omega, float* d_t, void* d_aT, hipStream_t stream);
+
 /* Host FP64 Wigner d^l_{m'm}(beta), 0 <= l <= L, D layout (host memory).
  * Replaces spherical.Wigner(L).d (used by src/matcha/core/SOFFT.py:31-39). */
 int mch_wigner_d_host(double beta, int L, double* out);

@@ -60,6 +60,12 @@ _SIGNATURES = [
                                     c_void_p, c_void_p, c_void_p, c_ll, c_void_p]),
     ("mch_tracker_update", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                                    c_void_p]),
+    ("mch_score_pose", c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    ("mch_shift_tangent", c_int, [c_void_p, c_ll, c_int, c_int, c_int, c_void_p, c_void_p, c_ll, c_void_p]),
+    ("mch_score_gradients", c_int, [c_void_p, c_int, c_int, c_void_p, c_ll, c_int, c_void_p, c_ll, c_int,
+                                    c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_ll, c_void_p]),
+    ("mch_score_backward", c_int, [c_void_p, c_void_p, c_int, c_void_p, c_ll, c_int, c_void_p, c_void_p,
+                                   c_void_p, c_void_p]),
     ("mch_wigner_d_host", c_int, [c_double, c_int, ctypes.POINTER(c_double)]),
 ]
 
