@@ -69,6 +69,10 @@ def parse():
     p.add_argument("--diff", action="store_true",
                    help="also measure the differentiable score (256 particles, box 128, L = 48)")
     p.add_argument("--diff-batch", type=int, default=256)
+    p.add_argument("--batching", action="store_true",
+                   help="also measure uneven batches (4096 particles, boxes 64/96/128, wedges, L 16/24/32)")
+    p.add_argument("--batching-particles", type=int, default=4096)
+    p.add_argument("--batching-batch", type=int, default=64)
     p.add_argument("--diff-steps", type=int, default=3)
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="untimed device copy of this size before the run (HBM counter calibration)")
@@ -245,6 +249,79 @@ def diff_workload(torch, dev, args):
     }
 
 
+def batching_workload(torch, dev, args):
+    """Uneven batches: particles of boxes 64/96/128 (2:2:1, one blob template per
+    box), missing wedges of +-U(45, 65) deg about a randomly offset y axis,
+    SNR U(-12, 0) dB, march ends L in {16, 24, 32}; bucketed per (box, L), each
+    full batch one HIP graph replay (bf16x3 projection, FLE bandlimit 32)."""
+    from matcha_amd.batching import UnevenAligner
+    from matcha_amd.synthetic import UnevenWorkload, euler_zyz_to_matrix, geodesic_deg
+    n = args.batching_particles
+    t0 = time.perf_counter()
+    wl = UnevenWorkload(seed=args.seed + 21, device=dev)
+    specs = wl.draw(n)
+    particles, ctfs = [None] * n, [None] * n
+    Rt = np.zeros((n, 3, 3))
+    tt = np.zeros((n, 3), dtype=int)
+    for N in wl.boxes:
+        idx = [i for i, sp in enumerate(specs) if sp[0] == N]
+        for c in range(0, len(idx), 64):
+            sel = idx[c:c + 64]
+            vols, wedges, R, t, snr, tilt = wl.particles(N, len(sel))
+            for j, i in enumerate(sel):
+                particles[i], ctfs[i], Rt[i], tt[i] = vols[j], wedges[j], R[j], t[j]
+    templates = {N: wl.work[N].template for N in wl.boxes}
+    al = UnevenAligner(templates, dev, radius=wl.radius, bandlimit=32, batch=args.batching_batch,
+                       L_ends=wl.L_ends, precision=args.tuned_precision, graphs=True, max_shift=wl.max_shift)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    # warm every bucket: plan set-up, autotune and graph capture stay out of the timed pass
+    from matcha_amd.batching import plan_buckets
+    for (N, L), chunks in plan_buckets(specs, al.batch).items():
+        full = [c for c in chunks if len(c) == al.batch]
+        if full:
+            al.bucket(N, L).run(torch.stack([particles[i] for i in full[0]]),
+                                torch.stack([ctfs[i] for i in full[0]]))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    res = al.align(specs, particles, ctfs)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t1
+    est = euler_zyz_to_matrix(res["alpha"].astype(np.float64), res["beta"].astype(np.float64),
+                              res["gamma"].astype(np.float64))
+    err = geodesic_deg(Rt, est)
+    ok = np.all(res["shift"].astype(int) == tt[:, ::-1], axis=1)
+    per_box = {}
+    for N in wl.boxes:
+        m = np.array([sp[0] == N for sp in specs])
+        per_box[str(N)] = {"particles": int(m.sum()), "median_err_deg": float(np.median(err[m])),
+                           "shift_exact_frac": float(ok[m].mean())}
+    # graph replay against eager launches of the same full batch (96^3, L = 32)
+    bk = al.bucket(96, 32)
+    sel = [i for i, sp in enumerate(specs) if sp == (96, 32)][:al.batch]
+    vb, cb = torch.stack([particles[i] for i in sel]), torch.stack([ctfs[i] for i in sel])
+    reps = 10
+    times = {}
+    for mode in ("graph", "eager"):
+        bk.graphs = mode == "graph"
+        bk.run(vb, cb)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for _ in range(reps):
+            bk.run(vb, cb)
+        torch.cuda.synchronize()
+        times[mode] = (time.perf_counter() - t2) / reps * 1e3
+    bk.graphs = True
+    return {
+        "workload": (f"{n} particles, boxes 64/96/128 at 2:2:1 (one template each), missing wedge "
+                     f"+-U(45,65) deg, SNR U(-12,0) dB, march end L in {{16,24,32}}, 2 alternations, "
+                     f"batch {al.batch}, {args.tuned_precision}"),
+        "particles_per_s": n / el, "seconds": el, "buckets": len(al.buckets),
+        "median_err_deg": float(np.median(err)), "shift_exact_frac": float(ok.mean()), "per_box": per_box,
+        "graph_vs_eager_ms_per_batch_96_L32": times, "setup_s": t_data,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -349,6 +426,8 @@ def main():
     }
     if args.diff:
         out["differentiation"] = diff_workload(torch, dev, args)
+    if args.batching:
+        out["batching"] = batching_workload(torch, dev, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args)
     print(json.dumps(out), flush=True)

@@ -104,7 +104,7 @@ class Alternation:
         self.tracker = None
 
     @torch.no_grad()
-    def run(self, subtomos: torch.Tensor, prior_tracker=None):
+    def run(self, subtomos: torch.Tensor, prior_tracker=None, ctfs: torch.Tensor | None = None):
         """(alpha, beta, gamma, norm_score, shift (B,3), shift_score) of the last
         alternation.
 
@@ -113,23 +113,24 @@ class Alternation:
         mode the angles are the last search's (relative to the particles rotated
         by the previous tracker) and ``self.tracker.q`` holds the composed
         rotation trackers, the reference's ``rotation_tracker``; in global mode
-        the angles are the full estimate."""
+        the angles are the full estimate.  ``ctfs`` (B, N, N, N//2+1): CTF or
+        missing-wedge weights of the shift search (``do_ctf_correction``)."""
         if self.mode == "reference":
-            return self._run_reference(subtomos, prior_tracker)
+            return self._run_reference(subtomos, prior_tracker, ctfs)
         if prior_tracker is not None:
             raise ValueError("a prior rotation needs mode='reference'")
-        return self._run_global(subtomos)
+        return self._run_global(subtomos, ctfs)
 
     def _buffers(self, subtomos):
         if self._patches is None or self._patches.shape != subtomos.shape:
             self._patches = torch.empty_like(subtomos, dtype=torch.float32)
             self._work = torch.empty_like(subtomos, dtype=torch.float32)
+            self._shift0 = torch.empty((subtomos.shape[0], 3), dtype=torch.float32, device=subtomos.device)
 
-    def _run_global(self, subtomos):
+    def _run_global(self, subtomos, ctfs=None):
         B = subtomos.shape[0]
-        dev = subtomos.device
-        shift = torch.zeros((B, 3), dtype=torch.float32, device=dev)
         self._buffers(subtomos)
+        shift = self._shift0.zero_()
         vols = subtomos
         out = None
         self.history = []
@@ -137,25 +138,25 @@ class Alternation:
             if it > 0:
                 vols = extract_shifted_patches(subtomos, shift, out=self._patches)
             sigma, norms = self.ccm.get_sigma(vols)
-            a, b, g, s = self.rot.search_orientations(sigma)
+            a, b, g, s = self.rot.search_orientations(sigma[:B])
             nscore = s / (norms[:B] * self.ccm.bh_norm_template).clamp_min(1e-12)
             last = it == self.num_alternations - 1
             sscore = None
             if not last or self.final_shift:
                 rb = euler_to_shift_rotations(a, b, g)
-                ds, sscore = self.sm.search_shifts(subtomos=vols, ctfs=None,
+                ds, sscore = self.sm.search_shifts(subtomos=vols, ctfs=ctfs,
                                                    shift_zyx=None, rotation_tracker=rb)
                 shift = shift + ds
             out = (a, b, g, nscore, shift, sscore)
             self.history.append(out)
         return out
 
-    def _run_reference(self, subtomos, prior_tracker):
+    def _run_reference(self, subtomos, prior_tracker, ctfs=None):
         B, N = subtomos.shape[0], subtomos.shape[1]
         dev = subtomos.device
         radius = self.mask_radius if self.mask_radius is not None else getattr(self.ccm, "radius", N // 2)
-        shift = torch.zeros((B, 3), dtype=torch.float32, device=dev)
         self._buffers(subtomos)
+        shift = self._shift0.zero_()
         self.tracker = RotationTracker(B, dev, prior_tracker)
         out = None
         self.history = []
@@ -163,13 +164,13 @@ class Alternation:
             vols = extract_rotated_batch(subtomos, shift, self.tracker.rot_particle, radius,
                                          out=self._patches, work=self._work)
             sigma, norms = self.ccm.get_sigma(vols)
-            a, b, g, s = self.rot.search_orientations(sigma)
+            a, b, g, s = self.rot.search_orientations(sigma[:B])
             nscore = s / (norms[:B] * self.ccm.bh_norm_template).clamp_min(1e-12)
             self.tracker.update(a, b, g)
             last = it == self.num_alternations - 1
             sscore = None
             if not last or self.final_shift:
-                ds, sscore = self.sm.search_shifts(subtomos=subtomos, ctfs=None, shift_zyx=shift,
+                ds, sscore = self.sm.search_shifts(subtomos=subtomos, ctfs=ctfs, shift_zyx=shift,
                                                    rotation_tracker=self.tracker.rot_reference)
                 shift = shift + ds
             out = (a, b, g, nscore, shift, sscore)

@@ -119,3 +119,66 @@ class Workload:
             noise = torch.randn(vols.shape, device=self.device, generator=self.gen)
             vols = vols + noise * std[:, None, None, None]
         return vols.contiguous(), R, t
+
+
+def wedge_mask_half(N: int, tilt_max_deg: float, axis_offset_deg: float, device) -> torch.Tensor:
+    """Missing-wedge weights (N, N, N//2 + 1) float32 in rfftn order of a (z, y, x)
+    volume (x the half axis): a tilt series about the in-plane axis y rotated by
+    ``axis_offset_deg`` about z, beam along z, tilts within +-``tilt_max_deg``,
+    samples the frequencies with |k_z| <= tan(tilt_max) |k_perp|, k_perp the
+    component perpendicular to the tilt axis in the x-y plane."""
+    kz = torch.fft.fftfreq(N, device=device)[:, None, None]
+    ky = torch.fft.fftfreq(N, device=device)[None, :, None]
+    kx = torch.fft.rfftfreq(N, device=device)[None, None, :]
+    psi = math.radians(axis_offset_deg)
+    kp = torch.abs(math.cos(psi) * kx - math.sin(psi) * ky)
+    keep = torch.abs(kz) <= math.tan(math.radians(tilt_max_deg)) * kp + 1e-12
+    keep = keep | ((kz == 0) & (kp == 0))
+    return keep.to(torch.float32).expand(N, N, N // 2 + 1).contiguous()
+
+
+class UnevenWorkload:
+    """Particles of several box sizes, each box with its own blob template; per
+    particle a Haar rotation, an integer shift, a missing wedge (tilt range
+    +-U(tilt_range) about y offset by U(-90, 90) degrees), white noise at
+    U(snr_range) dB (signal power over the box's spherical mask, noise added
+    before the wedge) and a march end L drawn from ``L_ends``."""
+
+    def __init__(self, boxes=(64, 96, 128), ratio=(2, 2, 1), L_ends=(16, 24, 32), seed: int = 0, device="cuda",
+                 n_blobs=None, sigma_range=(1.0, 5.0), tilt_range=(45.0, 65.0), snr_range=(-12.0, 0.0),
+                 max_shift: int = 4):
+        self.device = torch.device(device)
+        self.boxes = tuple(int(b) for b in boxes)
+        self.ratio = np.asarray(ratio, dtype=np.float64) / float(np.sum(ratio))
+        self.L_ends = tuple(int(x) for x in L_ends)
+        self.rng = np.random.default_rng(seed)
+        self.tilt_range, self.snr_range, self.max_shift = tilt_range, snr_range, int(max_shift)
+        self.radius = {N: float(min(N // 2, int(0.95 * N / 2)) - 2) for N in self.boxes}
+        nb = n_blobs or {N: max(20, int(120 * (N / 96) ** 2)) for N in self.boxes}
+        self.work = {N: Workload(N, nb[N], sigma_range, seed + 100 + N, self.device, mask_radius=self.radius[N])
+                     for N in self.boxes}
+
+    def draw(self, n: int):
+        """Per-particle specs: list of (box, L_end), in draw order."""
+        box = self.rng.choice(len(self.boxes), size=n, p=self.ratio)
+        le = self.rng.integers(0, len(self.L_ends), size=n)
+        return [(self.boxes[b], self.L_ends[l]) for b, l in zip(box, le)]
+
+    def particles(self, N: int, P: int):
+        """(vols (P,N,N,N), wedges (P,N,N,N//2+1), R (P,3,3), shifts (P,3), snr (P,), tilt (P,))."""
+        w = self.work[N]
+        R = haar_rotations(w.rng, P)
+        t = w.rng.integers(-self.max_shift, self.max_shift + 1, (P, 3))
+        cen = np.einsum('pij,bj->pbi', R, w.centres) + t[:, None, :]
+        vols = render_blobs(N, torch.tensor(cen, dtype=torch.float32, device=self.device), w.sig, w.amp,
+                            self.device)
+        snr = self.rng.uniform(*self.snr_range, P)
+        tilt = self.rng.uniform(*self.tilt_range, P)
+        off = self.rng.uniform(-90.0, 90.0, P)
+        m = w.mask
+        p = (vols[:, m] ** 2).mean(dim=1)
+        std = torch.sqrt(p / torch.tensor(10.0 ** (snr / 10.0), dtype=torch.float32, device=self.device))
+        vols = vols + torch.randn(vols.shape, device=self.device, generator=w.gen) * std[:, None, None, None]
+        wedges = torch.stack([wedge_mask_half(N, tilt[i], off[i], self.device) for i in range(P)])
+        vols = torch.fft.irfftn(torch.fft.rfftn(vols, dim=(1, 2, 3)) * wedges, s=(N, N, N), dim=(1, 2, 3))
+        return vols.contiguous(), wedges, R, t, snr, tilt
