@@ -538,8 +538,21 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
     const float fm = (float)m, fmp = (float)mp;
     const float fm2 = fm * fm, fmp2 = fmp * fmp, mmp = fm * fmp;
     const unsigned voff = 16u * (unsigned)pc;
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
+    // sigma of shells l and l + 1 in flight while shell l's recurrence runs
+    // (the offsets are wave-uniform; reads past the particle's sigma return 0)
     unsigned soff = 4u * (unsigned)mch::pk_offset(lstart);
+    v4u q0 = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
+    soff += 16u * (unsigned)((lstart + 1) * (lstart + 1));
+    v4u q1 = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
+    soff += 16u * (unsigned)((lstart + 2) * (lstart + 2));
     for (int l = lstart; l <= Li; ++l) {
+      const v4u q = q0;
+      q0 = q1;
+      if (l + 2 <= Li) {
+        q1 = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
+        soff += 16u * (unsigned)((l + 3) * (l + 3));
+      }
       if (l > mp) {
         const float4 c = lrec[l];
         const float Pl = (c.x - fm2) * (c.x - fmp2);  // >= 1 for l > mp >= |m|
@@ -554,8 +567,6 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
         Cprev = Pl * r * c.w;
       }
       if (valid && l >= mp) {
-        typedef unsigned v4u __attribute__((ext_vector_type(4)));
-        const v4u q = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
         const float2 sv[2] = {make_float2(__uint_as_float(q.x), __uint_as_float(q.y)),
                               make_float2(__uint_as_float(q.z), __uint_as_float(q.w))};
 #pragma unroll
@@ -565,7 +576,6 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
           U2[i].x += sv[i].x * dpp; U2[i].y += sv[i].y * dpp;
         }
       }
-      soff += 16u * (unsigned)((l + 1) * (l + 1));
     }
     if (valid) {
       const int us[2] = {mp, m};
@@ -669,23 +679,32 @@ __global__ __launch_bounds__(kMarchThreads) void so3_march_kernel(MarchArgs args
   fill_lrec(lrec, args.Lmax);
   const unsigned sp_bytes = (unsigned)args.pk_stride * 8u;
   const float2* sp = args.sig_pk + (long long)b * args.pk_stride;
-  // thread 0 owns the optimiser state
-  float ca = args.alpha[q], cb = args.beta[q], cg = args.gamma[q];
-  float best = 0.f, lr = 0.f;
-  double gcur[3] = {0, 0, 0}, Hcur[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // the optimiser state lives in LDS and only thread 0 touches it, so it
+  // holds no registers in the other lanes during the evaluation
+  __shared__ double gcur[3], Hcur[9];
+  __shared__ float opt[5];  // current alpha, beta, gamma, best score, lr
+  if (tid == 0) {
+    opt[0] = args.alpha[q]; opt[1] = args.beta[q]; opt[2] = args.gamma[q];
+    opt[3] = 0.f; opt[4] = 0.f;
+    for (int i = 0; i < 3; ++i) gcur[i] = 0.0;
+    for (int i = 0; i < 9; ++i) Hcur[i] = 0.0;
+  }
   float ev[10];
   for (int li = 0; li < args.n_levels; ++li) {
     const int Li = args.levels[li];
     for (int st = 0; st <= args.num_steps; ++st) {
       // thread 0 proposes the point to evaluate; everybody reads it from LDS
       if (tid == 0) {
+        const float ca = opt[0], cb = opt[1], cg = opt[2], lr = opt[4];
         float ta = ca, tb = cb, tg = cg;
         if (st > 0) {
-          double stepd[3];
+          double Hl[9], gl[3], stepd[3];
+          for (int i = 0; i < 9; ++i) Hl[i] = Hcur[i];
+          for (int i = 0; i < 3; ++i) gl[i] = gcur[i];
           if (args.newton) {
-            if (!solve3(Hcur, gcur, stepd)) { stepd[0] = stepd[1] = stepd[2] = 0.0; }
+            if (!solve3(Hl, gl, stepd)) { stepd[0] = stepd[1] = stepd[2] = 0.0; }
           } else {
-            stepd[0] = -gcur[0]; stepd[1] = -gcur[1]; stepd[2] = -gcur[2];
+            stepd[0] = -gl[0]; stepd[1] = -gl[1]; stepd[2] = -gl[2];
           }
           ta = ca - lr * (float)stepd[0];
           tb = cb - lr * (float)stepd[1];
@@ -701,27 +720,28 @@ __global__ __launch_bounds__(kMarchThreads) void so3_march_kernel(MarchArgs args
       march_eval(sp, sp_bytes, Li, tb, za, zg, lgf, lrec, red, ev);
       if (tid == 0) {
         const bool first = (st == 0);
-        const bool acc = first || (ev[0] > best);
+        const bool acc = first || (ev[0] > opt[3]);
+        float lr = opt[4];
         if (!first) lr = acc ? lr * 1.1f : lr * 0.5f;
         if (acc) {
-          ca = ta; cb = tb; cg = tg;
-          best = ev[0];
+          opt[0] = ta; opt[1] = tb; opt[2] = tg;
+          opt[3] = ev[0];
           gcur[0] = ev[1]; gcur[1] = ev[2]; gcur[2] = ev[3];
           Hcur[0] = ev[4]; Hcur[1] = ev[5]; Hcur[2] = ev[6];
           Hcur[3] = ev[5]; Hcur[4] = ev[7]; Hcur[5] = ev[8];
           Hcur[6] = ev[6]; Hcur[7] = ev[8]; Hcur[8] = ev[9];
         }
         if (first) lr = args.base_lr;
+        opt[4] = lr;
       }
-      // every thread keeps (ca,cb,cg) consistent via the next broadcast
       __syncthreads();
     }
   }
   if (tid == 0) {
-    args.alpha[q] = ca;
-    args.beta[q] = cb;
-    args.gamma[q] = cg;
-    args.score[q] = best;
+    args.alpha[q] = opt[0];
+    args.beta[q] = opt[1];
+    args.gamma[q] = opt[2];
+    args.score[q] = opt[3];
   }
 }
 
