@@ -1,9 +1,10 @@
 """Benchmark: aligned volumes/s for Matcha's orientation-search hot path on one MI355X.
 
-Headline (the metric's own definition): the tuned path -- projection GEMM on
-16-bit hi/lo split operands (three MFMA products, FP32 accumulation) -- against
-the plain FP32 path, both measured in this run on the same synthetic particles,
-each required to reach <= 1 deg median geodesic error.  ``value`` is the tuned
+Headline (the metric's own definition): the tuned path -- batched (512
+particles per batch) with the projection GEMM on 16-bit hi/lo split operands
+(three MFMA products, FP32 accumulation) -- against the plain FP32 path at
+batch 64, both measured in this run on the same synthetic particles, each
+required to reach <= 1 deg median geodesic error.  ``value`` is the tuned
 path's throughput; the plain path's throughput and the ratio are in ``config``.
 
 A step = one batch of particles aligned on the GPU by two rotation/shift
@@ -19,12 +20,17 @@ U(1,5) px, amplitude U(0.5,1), centres in a ball of 0.35*96), particles =
 Haar-random rotation + integer shift in [-6,6]^3 + white noise at 0 dB over
 the radius-45 mask; FLE bandlimit 32 (ne 8252, lmax 42), radius 45; march
 Li = 8, 16, 24 -> L_max 32; SOFFT grid 64^3 (L0 = 8, K = 4); 10 candidates;
-5 Newton steps; batch 64.
+5 Newton steps; plain path batch 64, tuned path batch 512.
+
+``roofline`` describes the kernel with the largest share of the step (the
+shift search's plane kernel), timed live with HIP events on its stream;
+``roofline_gemm16`` the tuned projection GEMM.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -38,13 +44,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 # 512 MiB reads 256 MiB); keyed by (workload, batch).  Its writes are the K-split
 # partials, splits x rows x Ntot x 4 B (WRITE_SIZE reads exactly that), added
 # per run for the split the autotuner chose.
-GEMM16_FETCH_BYTES = {("box96_L32", 64): 584368 * 1024 * 2}
+GEMM16_FETCH_BYTES = {("box96_L32", 64): 584368 * 1024 * 2, ("box96_L32", 512): 3048163 * 1024 * 2}
+# HBM bytes (FETCH_SIZE x 2.0 + WRITE_SIZE) per launch of the shift search's
+# plane kernel, keyed by (workload, batch); measured like the GEMM's.
+SHIFT_YZ_TRAFFIC_BYTES = {("box96_L32", 512): 10580205197}
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 
 WORKLOADS = {
-    "box96_L32": dict(N=96, n_blobs=120, sigma=(1.0, 5.0), bandlimit=32, radius=45, L_max=32,
+    "box96_L32": dict(N=96, n_blobs=120, sigma=(1.0, 5.0), bandlimit=32, radius=45, L_max=32, tuned_batch=512,
                       Li=[8, 16, 24], K=4, C=10, steps=5, batch=64, max_shift=6),
     "box64_L16": dict(N=64, n_blobs=40, sigma=(1.5, 4.0), bandlimit=16, radius=28, L_max=16,
                       Li=[8, 12], K=2, C=10, steps=5, batch=64, max_shift=4),
@@ -62,7 +71,7 @@ def parse():
     p.add_argument("--tuned-precision", default="bf16x3", choices=["bf16x3", "fp16x3"])
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP32 path")
     p.add_argument("--snr", type=float, default=0.0)
-    p.add_argument("--pool", type=int, default=256, help="distinct particles cycled through")
+    p.add_argument("--pool", type=int, default=512, help="distinct particles cycled through")
     p.add_argument("--alternations", type=int, default=2)
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -146,7 +155,7 @@ def run_path(name, torch, dev, w, args, pool_vols, pool_R, pool_t, precision, B,
     ev[0].record(stream)
     sigma, norms = ccm.get_sigma(vols0)
     ev[1].record(stream)
-    a0, b0, g0, s0 = matcher.search_orientations(sigma)
+    a0, b0, g0, s0 = matcher.search_orientations(sigma[:vols0.shape[0]])
     ev[2].record(stream)
     sm.search_shifts(subtomos=vols0, rotation_tracker=euler_to_shift_rotations(a0, b0, g0))
     ev[3].record(stream)
@@ -176,6 +185,30 @@ def time_gemm(torch, dev, ccm, vols, precision, reps=20):
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def time_shift_kernels(torch, dev, sm, vols, reps=10):
+    """Per-kernel ms of the shift search (HIP events around its launches on its
+    stream): forward x, plane (y/z FFTs + cross power), inverse x, final."""
+    import ctypes
+    from matcha_amd import _lib
+    from matcha_amd.core.ShiftMatcher import euler_to_shift_rotations
+    B = vols.shape[0]
+    g = torch.Generator(device="cpu").manual_seed(5)
+    rb = euler_to_shift_rotations(*[(torch.rand(B, generator=g) * s).to(dev) for s in (6.283, 3.1416, 6.283)])
+    lib = _lib.lib()
+    _lib.check(lib.mch_shift_plan_set_timing(sm._plan, 1), "mch_shift_plan_set_timing")
+    buf = (ctypes.c_float * 4)()
+    acc = np.zeros(4)
+    try:
+        sm.search_shifts(subtomos=vols, rotation_tracker=rb)
+        for _ in range(reps):
+            sm.search_shifts(subtomos=vols, rotation_tracker=rb)
+            _lib.check(lib.mch_shift_kernel_ms(sm._plan, buf), "mch_shift_kernel_ms")
+            acc += np.array(buf[:], dtype=np.float64)
+    finally:
+        lib.mch_shift_plan_set_timing(sm._plan, 0)
+    return acc / reps
 
 
 def _lib_depth(fle_):
@@ -337,7 +370,7 @@ def main():
 
     w = dict(WORKLOADS[args.workload])
     Bp = args.batch or w["batch"]
-    Bt = args.tuned_batch or w["batch"]
+    Bt = args.tuned_batch or w.get("tuned_batch", w["batch"])
     Bmax = max(Bp, Bt)
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -350,19 +383,26 @@ def main():
         del src, dst
     t_setup = time.perf_counter()
     wl = Workload(w["N"], w["n_blobs"], w["sigma"], args.seed, dev, mask_radius=w["radius"])
-    ccm = CrossCorrelationMatcher(w["N"], dev, batchsize=Bmax, bandlimit=w["bandlimit"], radius=w["radius"])
-    ccm.set_template(wl.template)
-    matcher = Matcha(Bmax, dev, w["L_max"], {"Li": w["Li"], "candidates": w["C"], "num_steps": w["steps"],
-                                             "oversampling_factor_K": w["K"]})
-    sm = ShiftMatcher({"N": w["N"], "num_subtomograms_per_batch": Bmax, "shift_upsample_factor": 1,
-                       "shift_search_radius": w["max_shift"], "voxel_size": 1.0}, dev)
-    sm.set_reference(wl.template.cpu().numpy())
+
+    def plans(B, precision):
+        """Projection, search and shift plans sized for one path's batch."""
+        c = CrossCorrelationMatcher(w["N"], dev, batchsize=B, bandlimit=w["bandlimit"], radius=w["radius"],
+                                    compute_precision=precision)
+        c.set_template(wl.template)
+        m = Matcha(B, dev, w["L_max"], {"Li": w["Li"], "candidates": w["C"], "num_steps": w["steps"],
+                                         "oversampling_factor_K": w["K"]})
+        sh = ShiftMatcher({"N": w["N"], "num_subtomograms_per_batch": B, "shift_upsample_factor": 1,
+                           "shift_search_radius": w["max_shift"], "voxel_size": 1.0}, dev)
+        sh.set_reference(wl.template.cpu().numpy())
+        return c, m, sh
+
+    ccm, matcher, sm = plans(Bt, args.tuned_precision)  # the 16-bit tables are built (and tuned) here
+    plain_plans = None if args.no_plain else plans(Bp, "fp32")
     P = max(args.pool, Bmax)
     chunks = [wl.particles(min(64, P - i), w["max_shift"], args.snr) for i in range(0, P, 64)]
     pool_vols = torch.cat([c[0] for c in chunks])
     pool_R = np.concatenate([c[1] for c in chunks])
     pool_t = np.concatenate([c[2] for c in chunks])
-    ccm.fle.set_compute_precision(args.tuned_precision)  # builds the 16-bit tables (setup)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
 
@@ -370,16 +410,16 @@ def main():
                      ccm, matcher, sm)
     plain = None
     if not args.no_plain:
-        plain = run_path("plain", torch, dev, w, args, pool_vols, pool_R, pool_t, "fp32", Bp, ccm, matcher, sm)
+        plain = run_path("plain", torch, dev, w, args, pool_vols, pool_R, pool_t, "fp32", Bp, *plain_plans)
 
-    # dominant kernel of the tuned path: the projection GEMM.  It streams the
-    # 16-bit W^T hi/lo once per launch, so HBM bounds it: algorithmic bytes =
-    # W^T hi + lo (4 B per element) + the FP32 X operand, each read once.
+    # the tuned path's projection GEMM: algorithmic bytes = W^T hi + lo (4 B per
+    # element) + the hi/lo X operand (4 B per element), each read once;
+    # algorithmic flops = the three split products
     fle_ = ccm.fle
     gemm_t = time_gemm(torch, dev, ccm, pool_vols[:Bt], args.tuned_precision)
-    gemm_plain = time_gemm(torch, dev, ccm, pool_vols[:Bp], "fp32") if plain else None
+    gemm_plain = time_gemm(torch, dev, plain_plans[0], pool_vols[:Bp], "fp32") if plain else None
     w_bytes = 4.0 * fle_.Ntot * fle_.Kpad          # W^T hi + lo, 2 + 2 B per element
-    x_bytes = 4.0 * 8 * (2 * Bt) * fle_.Kpad       # FP32 X operand of the 8 classes
+    x_bytes = 4.0 * 8 * (2 * Bt) * fle_.Kpad       # X hi + lo of the 8 classes
     alg_bytes = w_bytes + x_bytes
     achieved = alg_bytes / gemm_t / 1e9
     flops16 = 3 * 2.0 * (2 * Bt) * (2 * fle_.V0) * sum(fle_.class_cols)
@@ -388,6 +428,21 @@ def main():
     fetch = GEMM16_FETCH_BYTES.get((args.workload, Bt))
     rows = -(-2 * Bt // 128) * 128
     traffic = fetch + (tiles // 100) * rows * fle_.Ntot * 4.0 if fetch is not None else None
+    gemm_hbm = achieved / 8000.0
+    gemm_mfma = flops16 / gemm_t / 1e12 / 2500.0
+
+    # the dominant kernel: the shift search's plane kernel (one (particle, kx)
+    # plane: forward y/z FFTs, rotated-reference sampling, cross power, inverse
+    # z/y FFTs).  Algorithmic bytes: the plane spectrum read and written
+    # (B H N^2 x 8 B each way) + the mask spectrum XM and the reference pair
+    # table once; algorithmic flops: 4 passes of N length-N FFTs (5 N log2 N
+    # each) + 38 per sampled frequency (trilinear complex sample, norms, product)
+    shift_ms = time_shift_kernels(torch, dev, sm, pool_vols[:Bt])
+    N, H = w["N"], w["N"] // 2 + 1
+    yz_t = shift_ms[1] * 1e-3
+    yz_bytes = 2.0 * Bt * H * N * N * 8 + H * N * N * 8 + N * N * (N + 1) * 16
+    yz_flops = Bt * H * (4 * N * 5.0 * N * math.log2(N) + N * N * 38.0)
+    yz_gbs = yz_bytes / yz_t / 1e9
 
     dtype = {"bf16x3": "bf16x3+f32acc", "fp16x3": "fp16x3+f32acc"}[args.tuned_precision]
     out = {
@@ -418,11 +473,21 @@ def main():
                            "lds_stages": tiles % 10},
             "setup_s": t_setup,
         },
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": traffic,
-                     "kernel": f"fle_gemm16_kernel<{args.tuned_precision}>", "avg_ms": gemm_t * 1e3,
-                     "bytes_per_launch": alg_bytes,
-                     "mfma_tflops": flops16 / gemm_t / 1e12, "mfma_peak_tflops": 2500.0},
+        "roofline": {"bound": "hbm", "achieved": yz_gbs, "peak": 8000.0, "unit": "GB/s",
+                     "frac": yz_gbs / 8000.0, "traffic": SHIFT_YZ_TRAFFIC_BYTES.get((args.workload, Bt)),
+                     "kernel": f"shift_yz_kernel<{N}>", "avg_ms": yz_t * 1e3, "bytes_per_launch": yz_bytes,
+                     "flops_per_launch": yz_flops, "fp32_vector_tflops": yz_flops / yz_t / 1e12,
+                     "fp32_vector_frac": yz_flops / yz_t / 157.3e12,
+                     "share_of_step": 2 * yz_t * 1e3 / tuned["ms_per_step"],
+                     "shift_kernels_ms": dict(zip(("fwd_x", "yz", "inv_x", "final"), map(float, shift_ms)))},
+        "roofline_gemm16": {"bound": "mfma" if gemm_mfma > gemm_hbm else "hbm",
+                            "achieved": flops16 / gemm_t / 1e12 if gemm_mfma > gemm_hbm else achieved,
+                            "peak": 2500.0 if gemm_mfma > gemm_hbm else 8000.0,
+                            "unit": "TFLOP/s" if gemm_mfma > gemm_hbm else "GB/s",
+                            "frac": max(gemm_mfma, gemm_hbm), "hbm_frac": gemm_hbm, "mfma_frac": gemm_mfma,
+                            "traffic": traffic, "kernel": f"fle_gemm16_kernel<{args.tuned_precision}>",
+                            "avg_ms": gemm_t * 1e3, "bytes_per_launch": alg_bytes,
+                            "flops_per_launch": flops16, "share_of_step": 2 * gemm_t * 1e3 / tuned["ms_per_step"]},
     }
     if args.diff:
         out["differentiation"] = diff_workload(torch, dev, args)

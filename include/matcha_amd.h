@@ -137,6 +137,12 @@ int mch_corr_plan_destroy(void* plan);
 int mch_compute_sigma(void* plan, const void* aT, const void* aV, long long ldv, int ne, int B,
                       void* sigma, long long lds, float* norms, hipStream_t stream);
 
+/* mch_compute_sigma for the shells l <= L only (entries above L are not
+ * written): the search reads sigma up to its L_max, below the expansion's
+ * lmax.  Same reference counterpart. */
+int mch_compute_sigma_degree(void* plan, const void* aT, const void* aV, long long ldv, int ne, int B, int L,
+                             void* sigma, long long lds, float* norms, hipStream_t stream);
+
 /* ------------------------------------------------------------ shift --- */
 
 /* Plan of the FFT shift match: the centred Fourier reference (complex64
@@ -169,6 +175,13 @@ int mch_shift_plan_set_upsample(void* plan, int up, int radius);
 int mch_shift_search(void* plan, const float* subtomos, long long ld_vol, int B,
                      const float* shift_zyx, const float* rot, const float* ctf, float* out_shift,
                      float* out_score, hipStream_t stream);
+
+/* Per-kernel timing of mch_shift_search (a measurement aid, no reference
+ * counterpart): when enabled, every search records events around its four
+ * launches on its stream; mch_shift_kernel_ms waits for the last search and
+ * returns the four durations in ms (forward x, plane, inverse x / kx, final). */
+int mch_shift_plan_set_timing(void* plan, int enable);
+int mch_shift_kernel_ms(void* plan, float* ms4);
 
 /* Shifted patches patch[a] = vol[a + flip(shift_zyx)] (trilinear for fractional
  * shifts, zeros outside, border slice zeroed), volumes (B, N^3) float32; shift_zyx

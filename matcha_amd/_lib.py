@@ -49,11 +49,15 @@ _SIGNATURES = [
     ("mch_corr_plan_destroy", c_int, [c_void_p]),
     ("mch_compute_sigma", c_int, [c_void_p, c_void_p, c_void_p, c_ll, c_int, c_int, c_void_p, c_ll,
                                   c_void_p, c_void_p]),
+    ("mch_compute_sigma_degree", c_int, [c_void_p, c_void_p, c_void_p, c_ll, c_int, c_int, c_int, c_void_p,
+                                         c_ll, c_void_p, c_void_p]),
     ("mch_shift_plan_create", c_int, [c_int, c_void_p, c_float, c_int, ctypes.POINTER(c_void_p)]),
     ("mch_shift_plan_destroy", c_int, [c_void_p]),
     ("mch_shift_plan_set_upsample", c_int, [c_void_p, c_int, c_int]),
     ("mch_shift_search", c_int, [c_void_p, c_void_p, c_ll, c_int, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),
+    ("mch_shift_plan_set_timing", c_int, [c_void_p, c_int]),
+    ("mch_shift_kernel_ms", c_int, [c_void_p, ctypes.POINTER(ctypes.c_float)]),
     ("mch_extract_patches", c_int, [c_void_p, c_ll, c_int, c_int, c_void_p, c_void_p, c_ll, c_void_p]),
     ("mch_euler_to_shift_rot", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     ("mch_extract_rotated", c_int, [c_void_p, c_ll, c_int, c_int, c_int, c_void_p, c_void_p, c_double,

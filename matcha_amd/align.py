@@ -137,7 +137,7 @@ class Alternation:
         for it in range(self.num_alternations):
             if it > 0:
                 vols = extract_shifted_patches(subtomos, shift, out=self._patches)
-            sigma, norms = self.ccm.get_sigma(vols)
+            sigma, norms = self.ccm.get_sigma(vols, max_degree=self.rot.Lmax)
             a, b, g, s = self.rot.search_orientations(sigma[:B])
             nscore = s / (norms[:B] * self.ccm.bh_norm_template).clamp_min(1e-12)
             last = it == self.num_alternations - 1
@@ -163,7 +163,7 @@ class Alternation:
         for it in range(self.num_alternations):
             vols = extract_rotated_batch(subtomos, shift, self.tracker.rot_particle, radius,
                                          out=self._patches, work=self._work)
-            sigma, norms = self.ccm.get_sigma(vols)
+            sigma, norms = self.ccm.get_sigma(vols, max_degree=self.rot.Lmax)
             a, b, g, s = self.rot.search_orientations(sigma[:B])
             nscore = s / (norms[:B] * self.ccm.bh_norm_template).clamp_min(1e-12)
             self.tracker.update(a, b, g)

@@ -113,24 +113,28 @@ class CrossCorrelationMatcher:
         self.template_coefficients_alphas_torch = torch.nn.functional.pad(a, (0, 1))
         self.bh_norm_template = torch.linalg.norm(a)
 
-    def compute_sigma(self, bh_volumes: torch.Tensor, i_from: int, i_to: int):
-        """sigma rows [i_from, i_to) from particle coefficients (B, ne)."""
+    def compute_sigma(self, bh_volumes: torch.Tensor, i_from: int, i_to: int, max_degree: int = None):
+        """sigma rows [i_from, i_to) from particle coefficients (B, ne); with
+        ``max_degree`` only the shells l <= max_degree are written."""
         if self.template_coefficients_alphas_torch is None:
             raise RuntimeError("set_template must be called before compute_sigma")
         _lib.require_gpu_tensor(bh_volumes, "bh_volumes", dtype=self.dtype, device=self.device)
         bv = bh_volumes.contiguous()
         rows = self.sigma[i_from:i_to]
         norms = self.bh_norms[i_from:i_to]
-        rc = _lib.lib().mch_compute_sigma(self._corr, self._aT.data_ptr(), bv.data_ptr(), bv.stride(0),
-                                          self.fle.ne, bv.shape[0], rows.data_ptr(), rows.stride(0),
-                                          norms.data_ptr(), _lib.stream_handle(self.device))
+        L = self.lmax if max_degree is None else int(max_degree)
+        rc = _lib.lib().mch_compute_sigma_degree(self._corr, self._aT.data_ptr(), bv.data_ptr(), bv.stride(0),
+                                                 self.fle.ne, bv.shape[0], L, rows.data_ptr(), rows.stride(0),
+                                                 norms.data_ptr(), _lib.stream_handle(self.device))
         _lib.check(rc, "mch_compute_sigma")
 
-    def get_sigma(self, volumes: torch.Tensor):
+    def get_sigma(self, volumes: torch.Tensor, max_degree: int = None):
         """(sigma (batchsize, Dsize(lmax)) complex64, bh_norms (batchsize,)).
 
         Returns the persistent buffers (overwritten by the next call), as the
-        reference does; rows beyond the input batch are zero."""
+        reference does; rows beyond the input batch are zero.  ``max_degree``
+        (not in the reference) writes only the shells l <= max_degree, all a
+        search up to that L_max reads."""
         num = volumes.shape[0]
         if num > self.batchsize:
             raise ValueError(
@@ -139,5 +143,5 @@ class CrossCorrelationMatcher:
             self.sigma[num:].zero_()
             self.bh_norms[num:].zero_()
         a = self.fle.evaluate_t(volumes, out=self.bh_volumes_buffer[:num])
-        self.compute_sigma(a, 0, num)
+        self.compute_sigma(a, 0, num, max_degree)
         return self.sigma, self.bh_norms

@@ -365,8 +365,10 @@ __global__ __launch_bounds__(64 * NW) void fle_gemm16_kernel(
   extern __shared__ u32x4 lds[];   // NST x kStageU4
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lb = mch::xcd_remap(blockIdx.x, gridDim.x);
-  const int tn = lb % ntn, rest = lb / ntn;
-  const int mt = rest % nmt, ks = rest / nmt;
+  // row tile fastest: the row tiles that share a W^T tile run side by side on
+  // one XCD, so W^T is fetched from HBM once for all of them
+  const int mt = lb % nmt, rest = lb / nmt;
+  const int tn = rest % ntn, ks = rest / ntn;
   const int col0 = tn * BN16, row0 = mt * BM16;
   int cls = 0;
   while (cls < kClasses - 1 && col0 >= cols.col_off[cls + 1]) ++cls;
@@ -395,9 +397,17 @@ __global__ __launch_bounds__(64 * NW) void fle_gemm16_kernel(
     const int k0 = kbeg + stage * BK16;
 #pragma unroll
     for (int t = 0; t < AP; ++t) lds_dma16(asrc[t] + k0, base + adst[t]);
+    // one row tile: W^T is read once, stream it past L2 (non-temporal);
+    // several: keep it cached for the neighbouring row tiles
+    if (nmt == 1) {
 #pragma unroll
-    for (int q = 0; q < 4 * G; ++q)
-      lds_dma16<2>(wsrc + (long long)stage * 1024 + q * 64, base + 1024 + wave * G * 256 + q * 64);
+      for (int q = 0; q < 4 * G; ++q)
+        lds_dma16<2>(wsrc + (long long)stage * 1024 + q * 64, base + 1024 + wave * G * 256 + q * 64);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4 * G; ++q)
+        lds_dma16<0>(wsrc + (long long)stage * 1024 + q * 64, base + 1024 + wave * G * 256 + q * 64);
+    }
   };
   constexpr int PER_STAGE = AP + 4 * G;  // DMAs per wave per stage
   floatx4 acc[8][CB];
@@ -577,6 +587,11 @@ __global__ void fle_finalize_kernel(const float* __restrict__ part, int splits, 
 // ---------------------------------------------------------------------------
 // One workgroup per (l, particle): sigma[Dindex(l,u,v)] =
 // (-1)^{u+v} sum_k aT[l,k,-v] conj(aV[l,k,-u]).
+// One workgroup per (shell l, particle b): the shell's template and particle
+// coefficients are gathered into LDS once (rows m = -l..l, kdim columns, zero
+// past each row's radial count, so the k sum runs to the shorter row as the
+// reference pairs them), then each thread forms 4 x 4 blocks of (u, v):
+// sigma[Dindex(l,u,v)] = (-1)^{u+v} sum_k aT[l,k,-v] conj(aV[l,k,-u]).
 __global__ __launch_bounds__(256) void sigma_kernel(const float2* __restrict__ aT,
                                                     const float2* __restrict__ aV, long long ldv,
                                                     const int* __restrict__ idx, int lmax,
@@ -585,24 +600,59 @@ __global__ __launch_bounds__(256) void sigma_kernel(const float2* __restrict__ a
   const int l = blockIdx.x;
   const int b = blockIdx.y;
   const int w = 2 * l + 1;
+  extern __shared__ float2 shell[];
+  float2* T = shell;              // (w, kdim): row r holds m = r - l
+  float2* V = shell + w * kdim;
   const float2* av = aV + (long long)b * ldv;
   const int* il = idx + (long long)l * (2 * lmax + 1) * kdim;
+  for (int q = threadIdx.x; q < w * kdim; q += blockDim.x) {
+    const int j = il[q];
+    T[q] = j >= 0 ? aT[j] : make_float2(0.f, 0.f);
+    V[q] = j >= 0 ? av[j] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
   float2* out = sigma + (long long)b * lds + mch::dsize(l - 1);
-  for (int q = threadIdx.x; q < w * w; q += blockDim.x) {
-    const int u = q / w - l, v = q % w - l;
-    const int* iv = il + (long long)(-u + l) * kdim;   // particle, m = -u
-    const int* it = il + (long long)(-v + l) * kdim;   // template, m = -v
-    float re = 0.f, im = 0.f;
-    for (int k = 0; k < kdim; ++k) {
-      const int a = it[k], c = iv[k];
-      if (a < 0 || c < 0) break;
-      const float2 x = aT[a], y = av[c];
-      // x * conj(y)
-      re += x.x * y.x + x.y * y.y;
-      im += x.y * y.x - x.x * y.y;
+  const int nb = (w + 3) / 4;
+  for (int blk = threadIdx.x; blk < nb * nb; blk += blockDim.x) {
+    const int u0 = (blk / nb) * 4, v0 = (blk - (blk / nb) * nb) * 4;
+    float2 acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[r][c] = make_float2(0.f, 0.f);
+    // row of m = -u for u = u0 + r - l is 2l - (u0 + r); rows past the shell read row 0 (unused)
+    int ru[4], rv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      ru[r] = (u0 + r < w) ? (2 * l - u0 - r) * kdim : 0;
+      rv[r] = (v0 + r < w) ? (2 * l - v0 - r) * kdim : 0;
     }
-    const float sg = ((u + v) & 1) ? -1.f : 1.f;
-    out[q] = make_float2(sg * re, sg * im);
+    for (int k = 0; k < kdim; ++k) {
+      float2 p[4], t[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        p[r] = V[ru[r] + k];
+        t[r] = T[rv[r] + k];
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          // t * conj(p)
+          acc[r][c].x += t[c].x * p[r].x + t[c].y * p[r].y;
+          acc[r][c].y += t[c].y * p[r].x - t[c].x * p[r].y;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (u0 + r >= w) continue;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (v0 + c >= w) continue;
+        const float sg = ((u0 + r + v0 + c) & 1) ? -1.f : 1.f;
+        out[(u0 + r) * w + v0 + c] = make_float2(sg * acc[r][c].x, sg * acc[r][c].y);
+      }
+    }
   }
 }
 
@@ -1003,21 +1053,37 @@ int mch_corr_plan_destroy(void* plan) {
   return MCH_OK;
 }
 
-// sigma (B, dsize(lmax)) with row stride lds, and ||aV|| (B,) when norms != NULL.
-int mch_compute_sigma(void* plan, const void* aT, const void* aV, long long ldv, int ne, int B,
-                      void* sigma, long long lds, float* norms, hipStream_t stream) {
+// sigma rows (B, dsize(L)) with row stride lds (entries above L untouched),
+// and ||aV|| (B,) when norms != NULL.
+int mch_compute_sigma_degree(void* plan, const void* aT, const void* aV, long long ldv, int ne, int B, int L,
+                             void* sigma, long long lds, float* norms, hipStream_t stream) {
   CorrPlan* p = (CorrPlan*)plan;
   if (!p || !aT || !aV || !sigma) { mch::set_error("mch_compute_sigma: NULL argument"); return MCH_ERR_ARG; }
-  if (B < 1 || lds < mch::dsize(p->lmax) || ldv < ne) {
-    mch::set_error("mch_compute_sigma: bad sizes (B=%d lds=%lld)", B, lds);
+  if (B < 1 || L < 0 || L > p->lmax || lds < mch::dsize(L) || ldv < ne) {
+    mch::set_error("mch_compute_sigma: bad sizes (B=%d L=%d lds=%lld)", B, L, lds);
     return MCH_ERR_SIZE;
   }
-  hipLaunchKernelGGL(sigma_kernel, dim3(p->lmax + 1, B), dim3(256), 0, stream, (const float2*)aT,
+  const size_t shm = (size_t)2 * (2 * L + 1) * p->kdim * sizeof(float2);
+  if (shm > 64 * 1024) {
+    int rc0 = mch::hip_check(hipFuncSetAttribute((const void*)sigma_kernel,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm),
+                             "hipFuncSetAttribute(sigma_kernel)");
+    if (rc0) return rc0;
+  }
+  hipLaunchKernelGGL(sigma_kernel, dim3(L + 1, B), dim3(256), shm, stream, (const float2*)aT,
                      (const float2*)aV, ldv, p->idx, p->lmax, p->kdim, (float2*)sigma, lds);
   int rc = mch::hip_check(hipGetLastError(), "sigma_kernel launch");
   if (rc || !norms) return rc;
   hipLaunchKernelGGL(norms_kernel, dim3(B), dim3(256), 0, stream, (const float2*)aV, ldv, ne, norms);
   return mch::hip_check(hipGetLastError(), "norms_kernel launch");
+}
+
+// sigma (B, dsize(lmax)) with row stride lds, and ||aV|| (B,) when norms != NULL.
+int mch_compute_sigma(void* plan, const void* aT, const void* aV, long long ldv, int ne, int B,
+                      void* sigma, long long lds, float* norms, hipStream_t stream) {
+  CorrPlan* p = (CorrPlan*)plan;
+  if (!p) { mch::set_error("mch_compute_sigma: NULL argument"); return MCH_ERR_ARG; }
+  return mch_compute_sigma_degree(plan, aT, aV, ldv, ne, B, p->lmax, sigma, lds, norms, stream);
 }
 
 }  // extern "C"

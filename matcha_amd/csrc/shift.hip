@@ -62,6 +62,9 @@ struct ShiftPlan {
   float2* Qg;         // (B, H, R, R)
   float* up_val;      // (B, R)
   int* up_idx;        // (B, R)
+  // optional per-kernel timing: events around the four launches of a search
+  int timing;
+  hipEvent_t ev[5];
 };
 
 // Raw buffer resource over nbytes at p: loads at byte offsets >= nbytes return 0
@@ -786,6 +789,8 @@ int configure_kernels() {
 
 void free_plan(ShiftPlan* p) {
   if (!p) return;
+  if (p->timing)
+    for (int i = 0; i < 5; ++i) (void)hipEventDestroy(p->ev[i]);
   (void)hipFree(p->refpair); (void)hipFree(p->twid); (void)hipFree(p->lut); (void)hipFree(p->XM);
   (void)hipFree(p->T); (void)hipFree(p->rnorm_part); (void)hipFree(p->snorm_part);
   (void)hipFree(p->bg_part); (void)hipFree(p->amax_val); (void)hipFree(p->amax_idx);
@@ -832,33 +837,46 @@ template <int N>
 int launch_search(ShiftPlan* p, const SearchArgs& a, hipStream_t stream) {
   constexpr int H = N / 2 + 1;
   int rc;
+  auto mark = [&](int i) {
+    if (p->timing) (void)hipEventRecord(p->ev[i], stream);
+  };
+  mark(0);
   hipLaunchKernelGGL(shift_fwd_x_kernel<N>, dim3(N, a.B), dim3(row_threads<N>()),
                      row_lds<N>() + (size_t)p->lut_n * sizeof(float), stream, a.subtomos, a.ld_vol,
                      a.shift_zyx, p->lut, p->lut_lo, p->lut_n, p->twid, p->T, p->bg_part);
   if ((rc = mch::hip_check(hipGetLastError(), "shift_fwd_x_kernel"))) return rc;
+  mark(1);
   if (p->up > 1) {
     const int R = p->R;
     hipLaunchKernelGGL(shift_yz_up_kernel<N>, dim3(H * a.B), dim3(plane_threads<N>()), plane_up_lds<N>(R),
                        stream, p->T, p->XM, p->bg_part, p->wsum, p->refpair, a.rot, a.ctf, p->twid, p->t01, R,
                        p->Qg, p->rnorm_part, p->snorm_part);
     if ((rc = mch::hip_check(hipGetLastError(), "shift_yz_up_kernel"))) return rc;
+    mark(2);
     hipLaunchKernelGGL(shift_up_kx_kernel, dim3(R, a.B), dim3(256), (size_t)2 * H * R * sizeof(float2), stream,
                        p->Qg, H, R, p->t2, p->up_val, p->up_idx);
     if ((rc = mch::hip_check(hipGetLastError(), "shift_up_kx_kernel"))) return rc;
+    mark(3);
     hipLaunchKernelGGL(shift_up_final_kernel, dim3(a.B), dim3(64), 0, stream, p->up_val, p->up_idx, R,
                        (float)p->up, p->rnorm_part, p->snorm_part, H, a.out_shift, a.out_score);
-    return mch::hip_check(hipGetLastError(), "shift_up_final_kernel");
+    rc = mch::hip_check(hipGetLastError(), "shift_up_final_kernel");
+    mark(4);
+    return rc;
   }
   hipLaunchKernelGGL(shift_yz_kernel<N>, dim3(H * a.B), dim3(plane_threads<N>()), plane_lds<N>(), stream,
                      p->T, p->XM, p->bg_part, p->wsum, p->refpair, a.rot, a.ctf, p->twid, p->rnorm_part,
                      p->snorm_part);
   if ((rc = mch::hip_check(hipGetLastError(), "shift_yz_kernel"))) return rc;
+  mark(2);
   hipLaunchKernelGGL(shift_inv_x_kernel<N>, dim3(N, a.B), dim3(row_threads<N>()), row_lds<N>(), stream,
                      p->T, p->twid, p->amax_val, p->amax_idx);
   if ((rc = mch::hip_check(hipGetLastError(), "shift_inv_x_kernel"))) return rc;
+  mark(3);
   hipLaunchKernelGGL(shift_final_kernel, dim3(a.B), dim3(256), 0, stream, p->amax_val, p->amax_idx,
                      p->ntile_x, p->rnorm_part, p->snorm_part, H, N, a.out_shift, a.out_score);
-  return mch::hip_check(hipGetLastError(), "shift_final_kernel");
+  rc = mch::hip_check(hipGetLastError(), "shift_final_kernel");
+  mark(4);
+  return rc;
 }
 
 }  // namespace
@@ -867,6 +885,37 @@ int launch_search(ShiftPlan* p, const SearchArgs& a, hipStream_t stream) {
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
+
+int mch_shift_plan_set_timing(void* plan, int enable) {
+  ShiftPlan* p = (ShiftPlan*)plan;
+  if (!p) { mch::set_error("mch_shift_plan_set_timing: NULL plan"); return MCH_ERR_ARG; }
+  if (enable && !p->timing) {
+    for (int i = 0; i < 5; ++i) {
+      const int rc = mch::hip_check(hipEventCreate(&p->ev[i]), "hipEventCreate");
+      if (rc) {
+        for (int k = 0; k < i; ++k) (void)hipEventDestroy(p->ev[k]);
+        return rc;
+      }
+    }
+    p->timing = 1;
+  } else if (!enable && p->timing) {
+    for (int i = 0; i < 5; ++i) (void)hipEventDestroy(p->ev[i]);
+    p->timing = 0;
+  }
+  return MCH_OK;
+}
+
+int mch_shift_kernel_ms(void* plan, float* ms4) {
+  ShiftPlan* p = (ShiftPlan*)plan;
+  if (!p || !ms4 || !p->timing) {
+    mch::set_error("mch_shift_kernel_ms: timing is not enabled on this plan");
+    return MCH_ERR_ARG;
+  }
+  int rc = mch::hip_check(hipEventSynchronize(p->ev[4]), "hipEventSynchronize");
+  for (int i = 0; i < 4 && !rc; ++i)
+    rc = mch::hip_check(hipEventElapsedTime(&ms4[i], p->ev[i], p->ev[i + 1]), "hipEventElapsedTime");
+  return rc;
+}
 
 int mch_extract_patches(const float* vols, long long ld_vol, int B, int N, const float* shift_zyx,
                         float* out, long long ld_out, hipStream_t stream) {

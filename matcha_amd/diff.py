@@ -84,9 +84,9 @@ class ScoreEngine:
         coeffs = self.ccm.fle.evaluate_t(tang)
         dsz = self.ccm.sigma.shape[1]
         sigma = torch.empty((4 * B, dsz), dtype=torch.complex64, device=self.device)
-        _lib.check(lib.mch_compute_sigma(self.ccm._corr, a_t.data_ptr(), coeffs.data_ptr(), coeffs.stride(0),
-                                         self.ne, 4 * B, sigma.data_ptr(), sigma.stride(0), None, stream),
-                   "mch_compute_sigma")
+        _lib.check(lib.mch_compute_sigma_degree(self.ccm._corr, a_t.data_ptr(), coeffs.data_ptr(),
+                                                coeffs.stride(0), self.ne, 4 * B, self.L, sigma.data_ptr(),
+                                                sigma.stride(0), None, stream), "mch_compute_sigma")
         partial = torch.empty((B, self.L + 1, 8), dtype=torch.float32, device=self.device)
         out = torch.empty((B, 7), dtype=torch.float32, device=self.device)
         G = torch.empty((B, self.ne), dtype=torch.complex64, device=self.device) if want_template else None
