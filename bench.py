@@ -83,6 +83,10 @@ def parse():
     p.add_argument("--batching-particles", type=int, default=4096)
     p.add_argument("--batching-batch", type=int, default=64)
     p.add_argument("--diff-steps", type=int, default=3)
+    p.add_argument("--precision-study", action="store_true",
+                   help="also run the precision study (box 128, L = 48: FP32 / BF16x3 / FP16x3 on the same particles)")
+    p.add_argument("--precision-particles", type=int, default=512)
+    p.add_argument("--precision-batch", type=int, default=128)
     p.add_argument("--copy-calibration-mib", type=int, default=0,
                    help="untimed device copy of this size before the run (HBM counter calibration)")
     p.add_argument("--cpu-sample-coeffs", type=int, default=100)
@@ -279,6 +283,83 @@ def diff_workload(torch, dev, args):
         "step": "score + d/d(axis-angle, shift, template coefficients), forward and backward",
         "fd_check": {"particles": nchk, "steps": "1e-3 rad / 1e-2 px", "max_rel_err": rel},
         "setup_s": setup,
+    }
+
+
+def precision_workload(torch, dev, args):
+    """Precision study: particles at box 128 (200-blob template, sigma U(1,5) px),
+    FLE bandlimit 48 radius 60, march 8..40 -> L = 48, -6 dB, integer shifts in
+    [-6,6]^3.  The same particles go through two alternations with the
+    projection GEMM in FP32, BF16x3 and FP16x3 (16-bit hi/lo split operands,
+    three MFMA products, FP32 accumulation; the Wigner-d recurrences, quadrature
+    nodes and the 3x3 Newton solve stay FP64 in every mode).  Per precision:
+    volumes/s, error against the planted poses, and agreement with the FP32
+    result (geodesic angle, integer shifts, normalised scores, coefficients)."""
+    from matcha_amd.align import Alternation
+    from matcha_amd.core.CrossCorrelationMatcher import CrossCorrelationMatcher
+    from matcha_amd.core.Matcha import Matcha
+    from matcha_amd.core.ShiftMatcher import ShiftMatcher
+    from matcha_amd.synthetic import Workload, euler_zyz_to_matrix, geodesic_deg
+    N, bl, radius, L, Li, shift_r = 128, 48, 60, 48, [8, 16, 24, 32, 40], 6
+    P, B = args.precision_particles, args.precision_batch
+    t0 = time.perf_counter()
+    wl = Workload(N, 200, (1.0, 5.0), args.seed + 31, dev, mask_radius=radius)
+    ccm = CrossCorrelationMatcher(N, dev, batchsize=B, bandlimit=bl, radius=radius, compute_precision="fp32")
+    ccm.set_template(wl.template)
+    m = Matcha(B, dev, L, {"Li": Li, "candidates": 10, "num_steps": 5, "oversampling_factor_K": 4})
+    sh = ShiftMatcher({"N": N, "num_subtomograms_per_batch": B, "shift_upsample_factor": 1,
+                       "shift_search_radius": shift_r, "voxel_size": 1.0}, dev)
+    sh.set_reference(wl.template.cpu().numpy())
+    chunks = [wl.particles(min(64, P - i), shift_r, -6.0) for i in range(0, P, 64)]
+    vols = torch.cat([c[0] for c in chunks])
+    R = np.concatenate([c[1] for c in chunks])
+    tt = np.concatenate([c[2] for c in chunks])
+    pipe = Alternation(ccm, m, sh, num_alternations=2, mode="global")
+    batches = [vols[i:i + B] for i in range(0, P, B)]
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    res = {}
+    for mode in ("fp32", "bf16x3", "fp16x3"):
+        ccm.fle.set_compute_precision(mode)
+        pipe.run(batches[0])   # the 16-bit tables are built and tuned on first use
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        outs = []
+        for vb in batches:
+            a, b, g, ns, s, _ = pipe.run(vb)
+            outs.append(torch.cat([torch.stack([a, b, g, ns], 1).float(), s.float()], 1))
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t1
+        o = torch.cat(outs).double().cpu().numpy()
+        coeffs = ccm.fle.evaluate_t(batches[0]).clone()
+        res[mode] = {"out": o, "s": el, "coeffs": coeffs}
+    ccm.fle.set_compute_precision("fp32")
+    ref = res["fp32"]
+    Rref = euler_zyz_to_matrix(*ref["out"][:, :3].T)
+    report = {}
+    for mode, r in res.items():
+        o = r["out"]
+        Rm = euler_zyz_to_matrix(*o[:, :3].T)
+        err = geodesic_deg(R, Rm)
+        ok = np.all(o[:, 4:7].astype(int) == tt[:, ::-1], axis=1)
+        d = {"volumes_per_s": P / r["s"], "median_err_deg": float(np.median(err)),
+             "p90_err_deg": float(np.percentile(err, 90)), "shift_exact_frac": float(ok.mean())}
+        if mode != "fp32":
+            dis = geodesic_deg(Rref, Rm)
+            c32 = ref["coeffs"]
+            d["vs_fp32"] = {
+                "median_deg": float(np.median(dis)), "p99_deg": float(np.percentile(dis, 99)),
+                "max_deg": float(dis.max()), "within_0.1deg_frac": float((dis <= 0.1).mean()),
+                "shifts_equal_frac": float(np.all(o[:, 4:7] == ref["out"][:, 4:7], axis=1).mean()),
+                "score_max_rel": float(np.max(np.abs(o[:, 3] - ref["out"][:, 3]) / np.abs(ref["out"][:, 3]))),
+                "coeff_max_err_rel": float(((r["coeffs"] - c32).abs().max() / c32.abs().max()).item()),
+            }
+        report[mode] = d
+    return {
+        "workload": (f"{P} particles, box {N}, 200 blobs, FLE bandlimit {bl} radius {radius}, march "
+                     f"{Li}->{L}, grid 64^3, -6 dB, shifts in [-{shift_r},{shift_r}]^3, 2 alternations, "
+                     f"batch {B}"),
+        "ne": int(ccm.fle.ne), "lmax": int(ccm.lmax), "precisions": report, "setup_s": setup,
     }
 
 
@@ -493,6 +574,8 @@ def main():
         out["differentiation"] = diff_workload(torch, dev, args)
     if args.batching:
         out["batching"] = batching_workload(torch, dev, args)
+    if args.precision_study:
+        out["precision_study"] = precision_workload(torch, dev, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(w, args)
     print(json.dumps(out), flush=True)
