@@ -23,7 +23,7 @@ Li = 8, 16, 24 -> L_max 32; SOFFT grid 64^3 (L0 = 8, K = 4); 10 candidates;
 5 Newton steps; plain path batch 64, tuned path batch 512.
 
 ``roofline`` describes the kernel with the largest share of the step (the
-shift search's plane kernel), timed live with HIP events on its stream;
+shift search's column pass), timed live with HIP events on its stream;
 ``roofline_gemm16`` the tuned projection GEMM.
 """
 from __future__ import annotations
@@ -46,8 +46,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 # per run for the split the autotuner chose.
 GEMM16_FETCH_BYTES = {("box96_L32", 64): 584368 * 1024 * 2, ("box96_L32", 512): 3048163 * 1024 * 2}
 # HBM bytes (FETCH_SIZE x 2.0 + WRITE_SIZE) per launch of the shift search's
-# plane kernel, keyed by (workload, batch); measured like the GEMM's.
-SHIFT_YZ_TRAFFIC_BYTES = {("box96_L32", 512): 10580205197}
+# column pass, keyed by (workload, batch); measured like the GEMM's.
+SHIFT_Z_TRAFFIC_BYTES = {}
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
@@ -193,7 +193,8 @@ def time_gemm(torch, dev, ccm, vols, precision, reps=20):
 
 def time_shift_kernels(torch, dev, sm, vols, reps=10):
     """Per-kernel ms of the shift search (HIP events around its launches on its
-    stream): forward x, plane (y/z FFTs + cross power), inverse x, final."""
+    stream): forward (x, y), column pass (z FFTs + cross power), inverse (y, x),
+    final."""
     import ctypes
     from matcha_amd import _lib
     from matcha_amd.core.ShiftMatcher import euler_to_shift_rotations
@@ -512,18 +513,19 @@ def main():
     gemm_hbm = achieved / 8000.0
     gemm_mfma = flops16 / gemm_t / 1e12 / 2500.0
 
-    # the dominant kernel: the shift search's plane kernel (one (particle, kx)
-    # plane: forward y/z FFTs, rotated-reference sampling, cross power, inverse
-    # z/y FFTs).  Algorithmic bytes: the plane spectrum read and written
-    # (B H N^2 x 8 B each way) + the mask spectrum XM and the reference pair
-    # table once; algorithmic flops: 4 passes of N length-N FFTs (5 N log2 N
-    # each) + 38 per sampled frequency (trilinear complex sample, norms, product)
+    # the dominant kernel: the shift search's column pass (one block of 4 kx x
+    # 8 ky columns: forward z FFT, rotated-reference sampling, cross power,
+    # inverse z FFT).  Algorithmic bytes: the half spectrum read and written
+    # (B H N^2 x 8 B each way) + the background spectrum XMy and the reference
+    # pair table once; algorithmic flops: 2 passes of H N length-N FFTs
+    # (5 N log2 N each) + 38 per sampled frequency (trilinear complex sample,
+    # norms, product)
     shift_ms = time_shift_kernels(torch, dev, sm, pool_vols[:Bt])
     N, H = w["N"], w["N"] // 2 + 1
-    yz_t = shift_ms[1] * 1e-3
-    yz_bytes = 2.0 * Bt * H * N * N * 8 + H * N * N * 8 + N * N * (N + 1) * 16
-    yz_flops = Bt * H * (4 * N * 5.0 * N * math.log2(N) + N * N * 38.0)
-    yz_gbs = yz_bytes / yz_t / 1e9
+    z_t = shift_ms[1] * 1e-3
+    z_bytes = 2.0 * Bt * H * N * N * 8 + H * N * N * 8 + N * N * (N + 1) * 16
+    z_flops = Bt * H * (2 * N * 5.0 * N * math.log2(N) + N * N * 38.0)
+    z_gbs = z_bytes / z_t / 1e9
 
     dtype = {"bf16x3": "bf16x3+f32acc", "fp16x3": "fp16x3+f32acc"}[args.tuned_precision]
     out = {
@@ -554,13 +556,13 @@ def main():
                            "lds_stages": tiles % 10},
             "setup_s": t_setup,
         },
-        "roofline": {"bound": "hbm", "achieved": yz_gbs, "peak": 8000.0, "unit": "GB/s",
-                     "frac": yz_gbs / 8000.0, "traffic": SHIFT_YZ_TRAFFIC_BYTES.get((args.workload, Bt)),
-                     "kernel": f"shift_yz_kernel<{N}>", "avg_ms": yz_t * 1e3, "bytes_per_launch": yz_bytes,
-                     "flops_per_launch": yz_flops, "fp32_vector_tflops": yz_flops / yz_t / 1e12,
-                     "fp32_vector_frac": yz_flops / yz_t / 157.3e12,
-                     "share_of_step": 2 * yz_t * 1e3 / tuned["ms_per_step"],
-                     "shift_kernels_ms": dict(zip(("fwd_x", "yz", "inv_x", "final"), map(float, shift_ms)))},
+        "roofline": {"bound": "hbm", "achieved": z_gbs, "peak": 8000.0, "unit": "GB/s",
+                     "frac": z_gbs / 8000.0, "traffic": SHIFT_Z_TRAFFIC_BYTES.get((args.workload, Bt)),
+                     "kernel": f"shift_z_kernel<{N}>", "avg_ms": z_t * 1e3, "bytes_per_launch": z_bytes,
+                     "flops_per_launch": z_flops, "fp32_vector_tflops": z_flops / z_t / 1e12,
+                     "fp32_vector_frac": z_flops / z_t / 157.3e12,
+                     "share_of_step": 2 * z_t * 1e3 / tuned["ms_per_step"],
+                     "shift_kernels_ms": dict(zip(("fwd_xy", "z", "inv_xy", "final"), map(float, shift_ms)))},
         "roofline_gemm16": {"bound": "mfma" if gemm_mfma > gemm_hbm else "hbm",
                             "achieved": flops16 / gemm_t / 1e12 if gemm_mfma > gemm_hbm else achieved,
                             "peak": 2500.0 if gemm_mfma > gemm_hbm else 8000.0,

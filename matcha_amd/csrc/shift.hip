@@ -6,19 +6,23 @@
 //   compute_shift (complex space, no phase normalisation, score normalised,
 //   upsample 1) :94-282, estimate_shift_like_relion :619-674.
 //
-// Per batch (4 launches):
-//   fwd_x  : one (b, z) slice: shifted, rolled patch split by linearity of the
+// Per batch, integer lags (4 launches):
+//   fwd_xy : one (b, z) slice: shifted, rolled patch split by linearity of the
 //            cosine mask into A = (1 - w) v and bg * w (w = background weight);
-//            FFT along x of A, two real rows per complex line, kx = 0..N/2
-//            written as T[b][kx][z][y]; the slice's share of sum(w v)
-//   yz     : one workgroup per (b, kx) plane held in LDS: T + bg * XM (XM = the
-//            x-spectrum of w, precomputed), FFT along y and z (1/N^3), ||S||^2,
-//            the rotated reference sampled on the fly (trilinear, ifftshift,
-//            mask_full, ctf), ||R||^2, P = conj(R) S, inverse FFT along z and y
-//   inv_x  : inverse FFT along x of the half axis zero-padded to N, real part,
-//            first argmax per half slice
+//            FFT of A along x (two real rows per complex line) and along y,
+//            written as T[b][kx][z][ky]; the slice's share of sum(w v)
+//   z      : one workgroup per (b, kx, block of ky columns): T + bg * XMy (XMy =
+//            the (x, y) spectrum of w, precomputed), FFT along z (1/N^3),
+//            ||S||^2, the rotated reference sampled on the fly (trilinear,
+//            ifftshift, mask_full, ctf), ||R||^2, P = conj(R) S, inverse FFT
+//            along z
+//   inv_xy : one (b, z) slice: inverse FFT along y, then along x of the half
+//            axis zero-padded to N, real part, first argmax
 //   final  : first argmax per particle, wrap to (-N/2, N/2], flip to the
 //            reference's (z,y,x) labels, score / (||S|| ||R||)
+// Sub-voxel lags (upsample > 1): fwd_x (x only, T[b][kx][z][y]), a plane pass
+// (T + bg * XM, FFT along y and z, the same cross power, then the local matrix
+// DFT over y and z), a kx pass and the final argmax.
 // All 1-D transforms are compile-time four-step FFTs in LDS (fft_lds.hpp).
 #include "common.hpp"
 #include "fft_lds.hpp"
@@ -48,9 +52,10 @@ struct ShiftPlan {
   float2* twid;     // (N) e^{-2 pi i j / N}
   float* lut;       // (lut_n)
   float2* XM;       // (H, N, N): x-spectrum of the rolled background weight, [kx][z][y]
-  float2* T;        // (B, H, N, N) work spectrum, [b][kx][z][y]
-  float* rnorm_part;  // (B, H)
-  float* snorm_part;  // (B, H)
+  float2* XMy;      // (H, N, N): its (x, y) spectrum, [kx][z][ky]
+  float2* T;        // (B, H, N, N) work spectrum: [b][kx][z][y] (sub-voxel path), [b][kx][z][ky]
+  float* rnorm_part;  // (B, zparts): per column block on the integer path; (B, H) on the sub-voxel path
+  float* snorm_part;  // same
   float* bg_part;     // (B, N)
   float* amax_val;    // (B, N)
   int* amax_idx;      // (B, N)
@@ -161,6 +166,7 @@ __device__ __forceinline__ int roll_half(int i) { return i + N / 2 < N ? i + N /
 // The rolled, shifted patch is split by linearity of the mask: masked =
 // A + bg * w with A = (1 - w) v.  This pass transforms A and accumulates the
 // background numerator sum(w v) of the slice; the plane pass adds bg * XM.
+// (Sub-voxel path; the integer path uses shift_fwd_xy_kernel.)
 // Complex line j carries rows j (real part) and j + N/2 (imaginary part).
 // Each thread keeps one column x (loads) and one row y (spectrum stores).
 template <int N>
@@ -239,6 +245,46 @@ __global__ __launch_bounds__(row_threads<N>()) void shift_fwd_x_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// rotated reference sample
+// ---------------------------------------------------------------------------
+// Trilinear sample of the centred reference spectrum at the rotated position
+// (rx, ry, rz) (x the last axis), through the reference's grid_sample
+// normalisation with size_e = N + 1 (align_corners): i = ((r sc - 1) + 1) / 2 (N - 1).
+// The pair column x0 + 1 holds (ref[x0], ref[x0 + 1]) for -1 <= x0 < N; corners
+// outside the volume and samples outside the ball (inball false) read as zero.
+template <int N>
+__device__ __forceinline__ float2 ref_trilinear(__amdgpu_buffer_rsrc_t rref, float rx, float ry, float rz,
+                                                bool inball) {
+  const float sc = 2.0f / (float)(N - 1);  // 2 / (size_e - 2) with size_e = N + 1
+  const float um = (float)(N - 1);
+  const float ix = ((rx * sc - 1.0f) + 1.0f) * 0.5f * um;
+  const float iy = ((ry * sc - 1.0f) + 1.0f) * 0.5f * um;
+  const float iz = ((rz * sc - 1.0f) + 1.0f) * 0.5f * um;
+  const float fz0 = floorf(iz), fy0 = floorf(iy), fx0 = floorf(ix);
+  const int z0 = (int)fz0, y0 = (int)fy0, x0 = (int)fx0;
+  const float fz = iz - fz0, fy = iy - fy0, fx = ix - fx0;
+  const bool xok = x0 >= -1 && x0 < N;
+  const unsigned o00 = (unsigned)((z0 * N + y0) * (N + 1) + x0 + 1) * 16u;
+  const bool z0ok = z0 >= 0 && z0 < N, z1ok = z0 >= -1 && z0 < N - 1;
+  const bool y0ok = y0 >= 0 && y0 < N, y1ok = y0 >= -1 && y0 < N - 1;
+  // lanes outside the ball issue no loads (the address unit's time goes with the active lanes)
+  float4 q00 = make_float4(0.f, 0.f, 0.f, 0.f), q01 = q00, q10 = q00, q11 = q00;
+  if (inball) {
+    q00 = load_f4(rref, xok && z0ok && y0ok ? o00 : kOutOfRange);
+    q01 = load_f4(rref, xok && z0ok && y1ok ? o00 + (N + 1) * 16u : kOutOfRange);
+    q10 = load_f4(rref, xok && z1ok && y0ok ? o00 + N * (N + 1) * 16u : kOutOfRange);
+    q11 = load_f4(rref, xok && z1ok && y1ok ? o00 + (N + 1) * (N + 1) * 16u : kOutOfRange);
+  }
+  const float s00 = (1.f - fz) * (1.f - fy), s01 = (1.f - fz) * fy, s10 = fz * (1.f - fy), s11 = fz * fy;
+  float2 rv;
+  rv.x = s00 * fmaf(fx, q00.z - q00.x, q00.x) + s01 * fmaf(fx, q01.z - q01.x, q01.x) +
+         s10 * fmaf(fx, q10.z - q10.x, q10.x) + s11 * fmaf(fx, q11.z - q11.x, q11.x);
+  rv.y = s00 * fmaf(fx, q00.w - q00.y, q00.y) + s01 * fmaf(fx, q01.w - q01.y, q01.y) +
+         s10 * fmaf(fx, q10.w - q10.y, q10.y) + s11 * fmaf(fx, q11.w - q11.y, q11.y);
+  return rv;
+}
+
+// ---------------------------------------------------------------------------
 // one (b, kx) plane: FFT y, z; cross power with the rotated reference
 // ---------------------------------------------------------------------------
 // Leaves P = conj(R) S of plane kx of particle b in LDS, slot (sz, sy) holding
@@ -288,8 +334,6 @@ __device__ __forceinline__ void plane_cross_power(
   const float m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4], m5 = m[5], m6 = m[6], m7 = m[7],
               m8 = m[8];
   const float c = (float)N * 0.5f;
-  const float sc = 2.0f / (float)(N - 1);  // 2 / (size_e - 2) with size_e = N + 1
-  const float um = (float)(N - 1);
   const float inv = 1.0f / ((float)N * (float)N * (float)N);
   const int jp = kx < N / 2 ? kx : kx - H;
   const float bx = (float)(kx + N / 2) - c;
@@ -308,31 +352,9 @@ __device__ __forceinline__ void plane_cross_power(
     const int kp = kz < N / 2 ? kz : kz - N;
     const bool inball = kp * kp + r2xy <= (N / 2) * (N / 2);
     const float bz = (float)roll_half<N>(kz) - c;
-    const float rx = fmaf(m2, bz, px), ry = fmaf(m5, bz, py), rz = fmaf(m8, bz, pz);
-    const float ix = ((rx * sc - 1.0f) + 1.0f) * 0.5f * um;
-    const float iy = ((ry * sc - 1.0f) + 1.0f) * 0.5f * um;
-    const float iz = ((rz * sc - 1.0f) + 1.0f) * 0.5f * um;
-    const float fz0 = floorf(iz), fy0 = floorf(iy), fx0 = floorf(ix);
-    const int z0 = (int)fz0, y0 = (int)fy0, x0 = (int)fx0;
-    const float fz = iz - fz0, fy = iy - fy0, fx = ix - fx0;
-    // pair column x0 + 1 holds (ref[x0], ref[x0 + 1]) for -1 <= x0 < N; corners
-    // outside the volume and samples outside the ball read as zero
-    const bool xok = inball && x0 >= -1 && x0 < N;
-    const unsigned o00 = (unsigned)((z0 * N + y0) * (N + 1) + x0 + 1) * 16u;
-    const bool z0ok = z0 >= 0 && z0 < N, z1ok = z0 >= -1 && z0 < N - 1;
-    const bool y0ok = y0 >= 0 && y0 < N, y1ok = y0 >= -1 && y0 < N - 1;
-    const float4 q00 = load_f4(rref, xok && z0ok && y0ok ? o00 : kOutOfRange);
-    const float4 q01 = load_f4(rref, xok && z0ok && y1ok ? o00 + (N + 1) * 16u : kOutOfRange);
-    const float4 q10 = load_f4(rref, xok && z1ok && y0ok ? o00 + N * (N + 1) * 16u : kOutOfRange);
-    const float4 q11 = load_f4(rref, xok && z1ok && y1ok ? o00 + (N + 1) * (N + 1) * 16u : kOutOfRange);
+    float2 rv = ref_trilinear<N>(rref, fmaf(m2, bz, px), fmaf(m5, bz, py), fmaf(m8, bz, pz), inball);
     float cv = 1.f;
     if (ctfb && inball) cv = ctfb[(kz * N + ky) * H + kx];
-    const float s00 = (1.f - fz) * (1.f - fy), s01 = (1.f - fz) * fy, s10 = fz * (1.f - fy), s11 = fz * fy;
-    float2 rv;
-    rv.x = s00 * fmaf(fx, q00.z - q00.x, q00.x) + s01 * fmaf(fx, q01.z - q01.x, q01.x) +
-           s10 * fmaf(fx, q10.z - q10.x, q10.x) + s11 * fmaf(fx, q11.z - q11.x, q11.x);
-    rv.y = s00 * fmaf(fx, q00.w - q00.y, q00.y) + s01 * fmaf(fx, q01.w - q01.y, q01.y) +
-           s10 * fmaf(fx, q10.w - q10.y, q10.y) + s11 * fmaf(fx, q11.w - q11.y, q11.y);
     rv.x *= cv;
     rv.y *= cv;
     float2* slot = plane + sz * S + sy;
@@ -360,31 +382,323 @@ __device__ __forceinline__ void plane_cross_power(
 }
 
 // ---------------------------------------------------------------------------
-// integer lags: inverse FFT of the plane along z and y, back to HBM
+// integer lags, separable passes: (x, y) per slice -> z per column block ->
+// (y, x) per slice
 // ---------------------------------------------------------------------------
+// The forward pass transforms each (b, z) slice along x and y and stores
+// T[b][kx][z][ky] (ky in natural order); the column pass adds bg * XMy (the
+// (x, y) spectrum of the background weight), transforms along z, forms the
+// cross power with the rotated reference, transforms back along z and stores
+// in place; the inverse pass transforms each slice back along y and x and
+// takes the argmax.  A column block of KB ky columns needs N (KB + 1) complex
+// values of LDS, so several blocks share a CU, where a whole (y, z) plane needs
+// N (N + 1).
+// threads of the (x, y) slice passes: 4 per slice column, so that a CU holds
+// enough waves besides the slice's LDS
+template <int N> constexpr int slice_threads() { return 4 * N; }
+constexpr int kZThreads = 256, kZKx = 4, kZKy = 8;  // column block: 4 kx x 8 ky
+template <int N> constexpr int zgroups() { return (N / 2 + 1 + kZKx - 1) / kZKx; }
+// norm partials per particle on the integer path: one per column block
+template <int N> constexpr int zparts() { return zgroups<N>() * (N / kZKy); }
+
+// Forward transform along x and y of one (b, z) slice.  Loads and the mask
+// split as shift_fwd_x_kernel; the x spectra are unpacked in registers,
+// re-laid out as lines kx, transformed along y and stored with ky natural.
 template <int N>
-__global__ __launch_bounds__(plane_threads<N>()) void shift_yz_kernel(
-    float2* __restrict__ T, const float2* __restrict__ XM, const float* __restrict__ bg_part, float wsum,
+__global__ __launch_bounds__(slice_threads<N>()) void shift_fwd_xy_kernel(
+    const float* __restrict__ vols, long long ldv, const float* __restrict__ shift_zyx,
+    const float* __restrict__ lutg, int lut_lo, int lut_n, const float2* __restrict__ twg,
+    float2* __restrict__ T, float* __restrict__ bg_part) {
+  constexpr int NT = slice_threads<N>(), NL = N / 2, S = N + 1, H = N / 2 + 1, YSTEP = NT / N;
+  constexpr int KXP = (H + YSTEP - 1) / YSTEP;
+  static_assert(NT % N == 0 && N % YSTEP == 0, "column split");
+  extern __shared__ float4 smem4[];
+  float2* buf = reinterpret_cast<float2*>(smem4);  // [line][x], then [kx][y]; stride S
+  float2* tw = buf + H * S;
+  float* lut = reinterpret_cast<float*>(tw + N);
+  __shared__ float red[NT / 64];
+  const int z = blockIdx.x, b = blockIdx.y;
+  for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
+  for (int k = threadIdx.x; k < lut_n; k += NT) lut[k] = lutg[k];
+  __syncthreads();
+  ShiftSpec sp;
+  make_shift_spec(shift_zyx, b, N, sp);
+  const float* v = vols + (long long)b * ldv;
+  // roll by -N/2: rolled[i] = patch[(i + N/2) mod N]
+  const int x = threadIdx.x % N, yb = threadIdx.x / N;
+  const int a0 = roll_half<N>(z), a2 = roll_half<N>(x);
+  const int r2zx = (a0 - N / 2) * (a0 - N / 2) + (a2 - N / 2) * (a2 - N / 2);
+  float bgs = 0.f;
+  auto deposit = [&](int y, float pv) {
+    const int a1 = roll_half<N>(y);
+    const float w = bg_weight(r2zx + (a1 - N / 2) * (a1 - N / 2), lut, lut_lo, lut_n);
+    bgs += w * pv;
+    float* dst = reinterpret_cast<float*>(buf + (y < NL ? y : y - NL) * S + x);
+    dst[y >= NL ? 1 : 0] = (1.f - w) * pv;
+  };
+  if (sp.integral) {
+    const __amdgpu_buffer_rsrc_t rv = make_rsrc(v, (unsigned)(N * N * N) * 4u);
+    const int zs = a0 + sp.si[0], xs = a2 + sp.si[2];
+    const bool zxin = zs >= 0 && zs < N && xs >= 0 && xs < N;
+    const int zxoff = zs * N * N + xs;
+    constexpr int PER = N / YSTEP;
+    constexpr int U = load_batch<PER>();
+    for (int base = 0; base < PER; base += U) {
+      float pv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int y = yb + (base + u) * YSTEP;
+        const int ys = roll_half<N>(y) + sp.si[1];
+        const bool ok = zxin && ys >= 0 && ys < N;
+        pv[u] = load_f1(rv, ok ? (unsigned)(zxoff + ys * N) * 4u : kOutOfRange);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) deposit(yb + (base + u) * YSTEP, pv[u]);
+    }
+  } else {
+    for (int y = yb; y < N; y += YSTEP) deposit(y, patch_value(v, N, sp, a0, roll_half<N>(y), a2));
+  }
+  bgs = block_sum<NT>(bgs, red);  // its barrier also publishes buf
+  if (threadIdx.x == 0) bg_part[b * N + z] = bgs;
+  mch::fft::forward<N, NL, NT, -1>(buf, S, 1, tw);
+  // unpack A = (Z[k] + conj Z[N-k]) / 2 (row j), B = (Z[k] - conj Z[N-k]) / (2i) (row j + N/2)
+  const int y = threadIdx.x % N, kx0 = threadIdx.x / N;
+  float2 o[KXP];
+  {
+    const float2* line = buf + (y < NL ? y : y - NL) * S;
+#pragma unroll
+    for (int i = 0; i < KXP; ++i) {
+      const int kx = kx0 + i * YSTEP;
+      if (kx < H) {
+        const float2 Zk = line[mch::fft::slot_of_freq<N>(kx)];
+        const float2 Zn = line[mch::fft::slot_of_freq<N>(kx == 0 ? 0 : N - kx)];
+        if (y < NL) {
+          o[i] = make_float2(0.5f * (Zk.x + Zn.x), 0.5f * (Zk.y - Zn.y));
+        } else {
+          const float dx = Zk.x - Zn.x, dy = Zk.y + Zn.y;
+          o[i] = make_float2(0.5f * dy, -0.5f * dx);
+        }
+      }
+    }
+  }
+  __syncthreads();  // every read of the x spectra is done
+#pragma unroll
+  for (int i = 0; i < KXP; ++i) {
+    const int kx = kx0 + i * YSTEP;
+    if (kx < H) buf[kx * S + y] = o[i];
+  }
+  __syncthreads();
+  mch::fft::forward<N, H, NT, -1>(buf, S, 1, tw);  // along y: lines are kx
+  float2* out = T + (long long)b * H * N * N + (long long)z * N;
+  const int ky = threadIdx.x % N, sy = mch::fft::slot_of_freq<N>(ky);
+  for (int kx = kx0; kx < H; kx += YSTEP) out[(long long)kx * N * N + ky] = buf[kx * S + sy];
+}
+
+// One workgroup per (b, group of 4 kx, block of 8 ky): T + bg * XMy, FFT along z
+// (1/N^3), ||S||^2, the rotated reference sampled on the fly (trilinear,
+// ifftshift, mask_full, ctf), ||R||^2, P = conj(R) S, inverse FFT along z,
+// back to T in place.  The cross power runs over wave tiles of 4 kx x 4 ky x
+// 4 kz, whose samples fall in a compact region of the reference (lanes share
+// its cache lines).  Norm partials per workgroup.
+template <int N>
+__global__ __launch_bounds__(kZThreads) void shift_z_kernel(
+    float2* __restrict__ T, const float2* __restrict__ XMy, const float* __restrict__ bg_part, float wsum,
     const float4* __restrict__ refpair, const float* __restrict__ rot, const float* __restrict__ ctf,
     const float2* __restrict__ twg, float* __restrict__ rnorm_part, float* __restrict__ snorm_part) {
-  constexpr int NT = plane_threads<N>(), S = N + 1, H = N / 2 + 1, ZSTEP = NT / N;
+  constexpr int NT = kZThreads, NC = kZKx * kZKy, SC = NC + 1, H = N / 2 + 1, NG = zgroups<N>(),
+                NKB = N / kZKy, ZSTEP = NT / NC;
+  static_assert(NT == 256 && kZKx == 4 && kZKy == 8, "wave tiles of 4 x 4 x 4 in a 4 x 8 column block");
+  static_assert(N % kZKy == 0 && N % ZSTEP == 0 && N % 4 == 0, "column block split");
+  extern __shared__ float4 smem4[];
+  float2* tile = reinterpret_cast<float2*>(smem4);  // [z][column], stride SC; column = 8 (kx - kx0) + (ky - ky0)
+  float2* tw = tile + N * SC;
+  __shared__ float red[2 * NT / 64];
+  // XCD-aware order: neighbouring column blocks of one particle, which sample
+  // neighbouring regions of the reference, run on one XCD
+  const int lb = mch::xcd_remap(blockIdx.x, gridDim.x);
+  const int kb = lb % NKB, g = (lb / NKB) % NG, b = lb / (NKB * NG);
+  const int kx0 = g * kZKx, ky0 = kb * kZKy;
+  for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
+  float bgs = 0.f;
+  for (int zz = threadIdx.x & 63; zz < N; zz += 64) bgs += bg_part[b * N + zz];
+  bgs = mch::wave_sum(bgs);
+  const float bg = wsum > 0.f ? bgs / wsum : 0.f;
+  const int lc = threadIdx.x % NC, zb = threadIdx.x / NC;
+  const int lkx = kx0 + lc / kZKy, lky = ky0 + lc % kZKy;
+  const bool lok = lkx < H;  // the last group of kx can be partial
+  float2* col = T + ((long long)b * H + (lok ? lkx : 0)) * N * N + lky;
+  const float2* mcol = XMy + (long long)(lok ? lkx : 0) * N * N + lky;
+  {
+    constexpr int PER = N / ZSTEP;
+    constexpr int U = load_batch<PER>();
+    for (int base = 0; base < PER; base += U) {
+      float2 a[U], mm[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int zz = zb + (base + u) * ZSTEP;
+        a[u] = lok ? col[zz * N] : make_float2(0.f, 0.f);
+        mm[u] = lok ? mcol[zz * N] : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        tile[(zb + (base + u) * ZSTEP) * SC + lc] = make_float2(a[u].x + bg * mm[u].x, a[u].y + bg * mm[u].y);
+    }
+  }
+  __syncthreads();
+  mch::fft::forward<N, NC, NT, -1>(tile, 1, SC, tw);  // along z: lines are columns
+  // slot sz of a column holds frequency kz = freq(sz)
+  const float* m = rot + b * 9;
+  const float m0 = m[0], m1 = m[1], m2 = m[2], m3 = m[3], m4 = m[4], m5 = m[5], m6 = m[6], m7 = m[7],
+              m8 = m[8];
+  const float c = (float)N * 0.5f;
+  const float inv = 1.0f / ((float)N * (float)N * (float)N);
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int tx = l & 3, ty = (l >> 2) & 3, tz = l >> 4;
+  const int kx = kx0 + tx;
+  const bool kxok = kx < H;
+  const int jp = kx < N / 2 ? kx : kx - H;
+  const float bx = (float)(kx + N / 2) - c;
+  const __amdgpu_buffer_rsrc_t rref = make_rsrc(refpair, (unsigned)(N * N * (N + 1)) * 16u);
+  const float* ctfb = ctf ? ctf + (long long)b * N * N * H : nullptr;
+  float ns = 0.f, nr = 0.f;
+  constexpr int TILES = 2 * (N / 4);  // (half of the ky block, group of 4 kz)
+#pragma unroll 2
+  for (int t = w; t < TILES; t += NT / 64) {
+    const int ly = 4 * (t & 1) + ty;
+    const int ky = ky0 + ly, kz = 4 * (t >> 1) + tz;
+    const int ip = ky < N / 2 ? ky : ky - N;
+    const int kp = kz < N / 2 ? kz : kz - N;
+    const bool inball = kxok && kp * kp + ip * ip + jp * jp <= (N / 2) * (N / 2);
+    const float by = (float)roll_half<N>(ky) - c, bz = (float)roll_half<N>(kz) - c;
+    const float px = m0 * bx + m1 * by + c, py = m3 * bx + m4 * by + c, pz = m6 * bx + m7 * by + c;
+    float2 rv = ref_trilinear<N>(rref, fmaf(m2, bz, px), fmaf(m5, bz, py), fmaf(m8, bz, pz), inball);
+    float cv = 1.f;
+    if (ctfb && inball) cv = ctfb[(kz * N + ky) * H + kx];
+    rv.x *= cv;
+    rv.y *= cv;
+    if (kxok) {
+      float2* slot = tile + mch::fft::slot_of_freq<N>(kz) * SC + tx * kZKy + ly;
+      float2 s = *slot;
+      s.x *= inv;
+      s.y *= inv;
+      ns += s.x * s.x + s.y * s.y;
+      nr += rv.x * rv.x + rv.y * rv.y;
+      // conj(R) * S
+      *slot = make_float2(rv.x * s.x + rv.y * s.y, rv.x * s.y - rv.y * s.x);
+    }
+  }
+  ns = mch::wave_sum(ns);
+  nr = mch::wave_sum(nr);
+  if (l == 0) {
+    red[w] = ns;
+    red[NT / 64 + w] = nr;
+  }
+  __syncthreads();  // also publishes the cross power to every thread
+  if (threadIdx.x == 0) {
+    float a = 0.f, bsum = 0.f;
+    for (int q = 0; q < NT / 64; ++q) { a += red[q]; bsum += red[NT / 64 + q]; }
+    snorm_part[((long long)b * NG + g) * NKB + kb] = a;
+    rnorm_part[((long long)b * NG + g) * NKB + kb] = bsum;
+  }
+  mch::fft::inverse<N, NC, NT, 1>(tile, 1, SC, tw);  // along z, natural order out
+  if (lok)
+    for (int zz = zb; zz < N; zz += ZSTEP) col[zz * N] = tile[zz * SC + lc];
+}
+
+// Inverse transform of one (b, z) slice along y, then along x of the
+// zero-padded half axis (real part), and the slice's first argmax.
+// For a row X (kx = 0..N/2, zero above), Re(sum_{k<=N/2} X_k e^{2 pi i k n/N}) =
+// y[n] / 2 with y the inverse DFT of the Hermitian row Y (Y_0 = 2 Re X_0,
+// Y_{N/2} = 2 Re X_{N/2}, Y_k = X_k, Y_{N-k} = conj X_k).  Two such real rows
+// go through one complex FFT as Y1 + i Y2: rows j and j + N/2 of the slice.
+template <int N>
+__global__ __launch_bounds__(slice_threads<N>()) void shift_inv_xy_kernel(
+    const float2* __restrict__ T, const float2* __restrict__ twg, float* __restrict__ amax_val,
+    int* __restrict__ amax_idx) {
+  constexpr int NT = slice_threads<N>(), NL = N / 2, S = N + 1, H = N / 2 + 1, YSTEP = NT / N;
+  constexpr int KXP = (H + YSTEP - 1) / YSTEP, ITEMS = NL * H, IP = (ITEMS + NT - 1) / NT;
   static_assert(NT % 64 == 0 && NT % N == 0, "whole waves, column split");
   extern __shared__ float4 smem4[];
-  float2* plane = reinterpret_cast<float2*>(smem4);  // [z][y], stride S
-  float2* tw = plane + N * S;
-  __shared__ float red[2 * NT / 64];
-  // XCD-aware order: the 8 XCDs take consecutive ranges of (b, kx) planes, so
-  // neighbouring planes of one particle (which sample neighbouring planes of
-  // the reference) share one L2
-  const int lb = mch::xcd_remap(blockIdx.x, gridDim.x);
-  const int kx = lb % H, b = lb / H;
-  plane_cross_power<N>(plane, tw, red, kx, b, T, XM, bg_part, wsum, refpair, rot, ctf, twg, rnorm_part,
-                       snorm_part);
-  mch::fft::inverse<N, N, NT, 1>(plane, 1, S, tw);  // along z
-  mch::fft::inverse<N, N, NT, 1>(plane, S, 1, tw);  // along y
-  float2* P = T + ((long long)b * H + kx) * N * N;
-  const int cy = threadIdx.x % N, zb = threadIdx.x / N;
-  for (int zz = zb; zz < N; zz += ZSTEP) P[zz * N + cy] = plane[zz * S + cy];
+  float2* buf = reinterpret_cast<float2*>(smem4);  // [kx][y], then [line][x]; stride S
+  float2* tw = buf + H * S;
+  __shared__ float rv[NT / 64];
+  __shared__ int ri[NT / 64];
+  const int z = blockIdx.x, b = blockIdx.y;
+  for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
+  const float2* in = T + (long long)b * H * N * N + (long long)z * N;
+  {
+    // rows kx of the slice, ky natural -> slot order for the inverse along y
+    const int ky = threadIdx.x % N, sy = mch::fft::slot_of_freq<N>(ky), kx0 = threadIdx.x / N;
+    float2 v[KXP];
+#pragma unroll
+    for (int i = 0; i < KXP; ++i) {
+      const int kx = kx0 + i * YSTEP;
+      if (kx < H) v[i] = in[(long long)kx * N * N + ky];
+    }
+#pragma unroll
+    for (int i = 0; i < KXP; ++i) {
+      const int kx = kx0 + i * YSTEP;
+      if (kx < H) buf[kx * S + sy] = v[i];
+    }
+  }
+  __syncthreads();
+  mch::fft::inverse<N, H, NT, 1>(buf, S, 1, tw);  // along y: natural y out
+  // lines j: Y1 + i Y2 of rows j and j + N/2, built in registers, then stored
+  float2 ek[IP], en[IP];
+#pragma unroll
+  for (int i = 0; i < IP; ++i) {
+    const int t = i * NT + threadIdx.x;
+    if (t < ITEMS) {
+      const int j = t % NL, k = t / NL;
+      const float2 x1 = buf[k * S + j], x2 = buf[k * S + j + NL];
+      if (k == 0 || k == N / 2) {
+        ek[i] = make_float2(2.f * x1.x, 2.f * x2.x);
+      } else {
+        ek[i] = make_float2(x1.x - x2.y, x1.y + x2.x);
+        en[i] = make_float2(x1.x + x2.y, x2.x - x1.y);
+      }
+    }
+  }
+  __syncthreads();  // every read of the y transforms is done
+#pragma unroll
+  for (int i = 0; i < IP; ++i) {
+    const int t = i * NT + threadIdx.x;
+    if (t < ITEMS) {
+      const int j = t % NL, k = t / NL;
+      float2* line = buf + j * S;
+      line[k] = ek[i];
+      if (k != 0 && k != N / 2) line[N - k] = en[i];
+    }
+  }
+  __syncthreads();
+  mch::fft::forward<N, NL, NT, 1>(buf, S, 1, tw);
+  float bv = -INFINITY;
+  int bi = 0x7fffffff;
+  for (int t = threadIdx.x; t < NL * N; t += NT) {
+    const int j = t / N, s = t - (t / N) * N;
+    const float2 zv = buf[j * S + s];
+    const int xx = mch::fft::freq_of_slot<N>(s);
+    const int f1 = (z * N + j) * N + xx, f2 = f1 + NL * N;
+    const float v1 = 0.5f * zv.x, v2 = 0.5f * zv.y;
+    if (v1 > bv || (v1 == bv && f1 < bi)) { bv = v1; bi = f1; }
+    if (v2 > bv || (v2 == bv && f2 < bi)) { bv = v2; bi = f2; }
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const float ov = __shfl_xor(bv, off, 64);
+    const int oi = __shfl_xor(bi, off, 64);
+    if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+  }
+  if ((threadIdx.x & 63) == 0) { rv[threadIdx.x >> 6] = bv; ri[threadIdx.x >> 6] = bi; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float v = rv[0];
+    int i = ri[0];
+    for (int w = 1; w < NT / 64; ++w)
+      if (rv[w] > v || (rv[w] == v && ri[w] < i)) { v = rv[w]; i = ri[w]; }
+    amax_val[(long long)b * N + z] = v;
+    amax_idx[(long long)b * N + z] = i;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -569,90 +883,10 @@ __global__ __launch_bounds__(64) void shift_up_final_kernel(const float* __restr
   }
 }
 
-// ---------------------------------------------------------------------------
-// inverse FFT along x of the zero-padded half axis, real part, argmax
-// ---------------------------------------------------------------------------
-// For a row X (kx = 0..N/2, zero above), Re(sum_{k<=N/2} X_k e^{2 pi i k n/N}) =
-// y[n] / 2 with y the inverse DFT of the Hermitian row Y (Y_0 = 2 Re X_0,
-// Y_{N/2} = 2 Re X_{N/2}, Y_k = X_k, Y_{N-k} = conj X_k).  Two such real rows
-// go through one complex FFT as Y1 + i Y2: rows j and j + N/2 of plane z.
-template <int N>
-__global__ __launch_bounds__(row_threads<N>()) void shift_inv_x_kernel(
-    const float2* __restrict__ T, const float2* __restrict__ twg, float* __restrict__ amax_val,
-    int* __restrict__ amax_idx) {
-  constexpr int NT = row_threads<N>(), NL = N / 2, S = N + 1, H = N / 2 + 1;
-  static_assert(NT % 64 == 0, "whole waves");
-  extern __shared__ float4 smem4[];
-  float2* buf = reinterpret_cast<float2*>(smem4);  // [line][x], stride S
-  float2* tw = buf + NL * S;
-  __shared__ float rv[NT / 64];
-  __shared__ int ri[NT / 64];
-  const int z = blockIdx.x, b = blockIdx.y;
-  for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
-  const float2* in = T + (long long)b * H * N * N + (long long)z * N;
-  {
-    constexpr int ITEMS = NL * H;
-    constexpr int U = 4;
-    for (int base = 0; base < ITEMS; base += U * NT) {
-      float2 x1[U], x2[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int t = base + u * NT + threadIdx.x;
-        const int j = t % NL, k = t / NL;
-        const long long o = t < ITEMS ? (long long)k * N * N + j : 0;
-        x1[u] = in[o];
-        x2[u] = in[o + NL];
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int t = base + u * NT + threadIdx.x;
-        if (t >= ITEMS) break;
-        const int j = t % NL, k = t / NL;
-        float2* line = buf + j * S;
-        if (k == 0 || k == N / 2) {
-          line[k] = make_float2(2.f * x1[u].x, 2.f * x2[u].x);
-        } else {
-          line[k] = make_float2(x1[u].x - x2[u].y, x1[u].y + x2[u].x);
-          line[N - k] = make_float2(x1[u].x + x2[u].y, x2[u].x - x1[u].y);
-        }
-      }
-    }
-  }
-  __syncthreads();
-  mch::fft::forward<N, NL, NT, 1>(buf, S, 1, tw);
-  float bv = -INFINITY;
-  int bi = 0x7fffffff;
-  for (int t = threadIdx.x; t < NL * N; t += NT) {
-    const int j = t / N, s = t - (t / N) * N;
-    const float2 zv = buf[j * S + s];
-    const int xx = mch::fft::freq_of_slot<N>(s);
-    const int f1 = (z * N + j) * N + xx, f2 = f1 + NL * N;
-    const float v1 = 0.5f * zv.x, v2 = 0.5f * zv.y;
-    if (v1 > bv || (v1 == bv && f1 < bi)) { bv = v1; bi = f1; }
-    if (v2 > bv || (v2 == bv && f2 < bi)) { bv = v2; bi = f2; }
-  }
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) {
-    const float ov = __shfl_xor(bv, off, 64);
-    const int oi = __shfl_xor(bi, off, 64);
-    if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-  }
-  if ((threadIdx.x & 63) == 0) { rv[threadIdx.x >> 6] = bv; ri[threadIdx.x >> 6] = bi; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float v = rv[0];
-    int i = ri[0];
-    for (int w = 1; w < NT / 64; ++w)
-      if (rv[w] > v || (rv[w] == v && ri[w] < i)) { v = rv[w]; i = ri[w]; }
-    amax_val[(long long)b * N + z] = v;
-    amax_idx[(long long)b * N + z] = i;
-  }
-}
-
 __global__ __launch_bounds__(256) void shift_final_kernel(const float* __restrict__ amax_val,
                                                           const int* __restrict__ amax_idx,
                                                           int ntile_x, const float* __restrict__ rn,
-                                                          const float* __restrict__ sn, int H,
+                                                          const float* __restrict__ sn, int nparts,
                                                           int N, float* __restrict__ out_shift,
                                                           float* __restrict__ out_score) {
   const int b = blockIdx.x;
@@ -666,9 +900,9 @@ __global__ __launch_bounds__(256) void shift_final_kernel(const float* __restric
     const int i = amax_idx[(long long)b * ntile_x + t];
     if (v > bv || (v == bv && i < bi)) { bv = v; bi = i; }
   }
-  for (int t = threadIdx.x; t < H; t += blockDim.x) {
-    r2 += rn[(long long)b * H + t];
-    s2 += sn[(long long)b * H + t];
+  for (int t = threadIdx.x; t < nparts; t += blockDim.x) {
+    r2 += rn[(long long)b * nparts + t];
+    s2 += sn[(long long)b * nparts + t];
   }
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
@@ -749,6 +983,9 @@ namespace {
 
 template <int N> constexpr size_t row_lds() { return ((size_t)(N / 2) * (N + 1) + N) * sizeof(float2); }
 template <int N> constexpr size_t plane_lds() { return ((size_t)N * (N + 1) + N) * sizeof(float2); }
+// (x, y) slice passes: H lines of N + 1 and the twiddles; z column block: N rows of KB + 1
+template <int N> constexpr size_t slice_lds() { return ((size_t)(N / 2 + 1) * (N + 1) + N) * sizeof(float2); }
+template <int N> constexpr size_t zcol_lds() { return ((size_t)N * (kZKx * kZKy + 1) + N) * sizeof(float2); }
 
 // Checks the LDS the kernels for box N need against the device and raises the
 // dynamic-LDS limit of those that exceed the default 64 KiB.
@@ -761,29 +998,24 @@ int configure_kernels() {
                       "hipDeviceGetAttribute");
   if (rc) return rc;
   const size_t fwd_lds = row_lds<N>() + kMaxLut * sizeof(float);
-  if ((size_t)maxlds < plane_lds<N>() + 1024 || (size_t)maxlds < fwd_lds + 1024) {
-    mch::set_error("mch_shift_plan_create: a %dx%d plane needs %zu B of LDS, device allows %d", N, N,
-                   plane_lds<N>(), maxlds);
+  const size_t fwd_xy_lds = slice_lds<N>() + kMaxLut * sizeof(float);
+  if ((size_t)maxlds < fwd_lds + 1024 || (size_t)maxlds < fwd_xy_lds + 1024 || (size_t)maxlds < zcol_lds<N>()) {
+    mch::set_error("mch_shift_plan_create: box %d needs %zu B of LDS per slice, device allows %d", N, fwd_xy_lds,
+                   maxlds);
     return MCH_ERR_UNSUPPORTED;
   }
-  if (plane_lds<N>() > 64 * 1024) {
-    rc = mch::hip_check(hipFuncSetAttribute((const void*)shift_yz_kernel<N>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_lds<N>()),
-                        "hipFuncSetAttribute(shift_yz_kernel)");
-    if (rc) return rc;
-  }
-  if (fwd_lds > 64 * 1024) {
-    rc = mch::hip_check(hipFuncSetAttribute((const void*)shift_fwd_x_kernel<N>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)fwd_lds),
-                        "hipFuncSetAttribute(shift_fwd_x_kernel)");
-    if (rc) return rc;
-  }
-  if (row_lds<N>() > 64 * 1024) {
-    rc = mch::hip_check(hipFuncSetAttribute((const void*)shift_inv_x_kernel<N>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)row_lds<N>()),
-                        "hipFuncSetAttribute(shift_inv_x_kernel)");
-    if (rc) return rc;
-  }
+  auto raise = [&](const void* fn, size_t lds, const char* what) {
+    if (lds <= 64 * 1024) return (int)MCH_OK;
+    return mch::hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds), what);
+  };
+  if ((rc = raise((const void*)shift_fwd_x_kernel<N>, fwd_lds, "hipFuncSetAttribute(shift_fwd_x_kernel)")))
+    return rc;
+  if ((rc = raise((const void*)shift_fwd_xy_kernel<N>, fwd_xy_lds, "hipFuncSetAttribute(shift_fwd_xy_kernel)")))
+    return rc;
+  if ((rc = raise((const void*)shift_z_kernel<N>, zcol_lds<N>(), "hipFuncSetAttribute(shift_z_kernel)")))
+    return rc;
+  if ((rc = raise((const void*)shift_inv_xy_kernel<N>, slice_lds<N>(), "hipFuncSetAttribute(shift_inv_xy_kernel)")))
+    return rc;
   return MCH_OK;
 }
 
@@ -792,6 +1024,7 @@ void free_plan(ShiftPlan* p) {
   if (p->timing)
     for (int i = 0; i < 5; ++i) (void)hipEventDestroy(p->ev[i]);
   (void)hipFree(p->refpair); (void)hipFree(p->twid); (void)hipFree(p->lut); (void)hipFree(p->XM);
+  (void)hipFree(p->XMy);
   (void)hipFree(p->T); (void)hipFree(p->rnorm_part); (void)hipFree(p->snorm_part);
   (void)hipFree(p->bg_part); (void)hipFree(p->amax_val); (void)hipFree(p->amax_idx);
   (void)hipFree(p->t01); (void)hipFree(p->t2); (void)hipFree(p->Qg); (void)hipFree(p->up_val);
@@ -841,13 +1074,13 @@ int launch_search(ShiftPlan* p, const SearchArgs& a, hipStream_t stream) {
     if (p->timing) (void)hipEventRecord(p->ev[i], stream);
   };
   mark(0);
-  hipLaunchKernelGGL(shift_fwd_x_kernel<N>, dim3(N, a.B), dim3(row_threads<N>()),
-                     row_lds<N>() + (size_t)p->lut_n * sizeof(float), stream, a.subtomos, a.ld_vol,
-                     a.shift_zyx, p->lut, p->lut_lo, p->lut_n, p->twid, p->T, p->bg_part);
-  if ((rc = mch::hip_check(hipGetLastError(), "shift_fwd_x_kernel"))) return rc;
-  mark(1);
   if (p->up > 1) {
     const int R = p->R;
+    hipLaunchKernelGGL(shift_fwd_x_kernel<N>, dim3(N, a.B), dim3(row_threads<N>()),
+                       row_lds<N>() + (size_t)p->lut_n * sizeof(float), stream, a.subtomos, a.ld_vol,
+                       a.shift_zyx, p->lut, p->lut_lo, p->lut_n, p->twid, p->T, p->bg_part);
+    if ((rc = mch::hip_check(hipGetLastError(), "shift_fwd_x_kernel"))) return rc;
+    mark(1);
     hipLaunchKernelGGL(shift_yz_up_kernel<N>, dim3(H * a.B), dim3(plane_threads<N>()), plane_up_lds<N>(R),
                        stream, p->T, p->XM, p->bg_part, p->wsum, p->refpair, a.rot, a.ctf, p->twid, p->t01, R,
                        p->Qg, p->rnorm_part, p->snorm_part);
@@ -863,17 +1096,22 @@ int launch_search(ShiftPlan* p, const SearchArgs& a, hipStream_t stream) {
     mark(4);
     return rc;
   }
-  hipLaunchKernelGGL(shift_yz_kernel<N>, dim3(H * a.B), dim3(plane_threads<N>()), plane_lds<N>(), stream,
-                     p->T, p->XM, p->bg_part, p->wsum, p->refpair, a.rot, a.ctf, p->twid, p->rnorm_part,
+  hipLaunchKernelGGL(shift_fwd_xy_kernel<N>, dim3(N, a.B), dim3(slice_threads<N>()),
+                     slice_lds<N>() + (size_t)p->lut_n * sizeof(float), stream, a.subtomos, a.ld_vol,
+                     a.shift_zyx, p->lut, p->lut_lo, p->lut_n, p->twid, p->T, p->bg_part);
+  if ((rc = mch::hip_check(hipGetLastError(), "shift_fwd_xy_kernel"))) return rc;
+  mark(1);
+  hipLaunchKernelGGL(shift_z_kernel<N>, dim3(zparts<N>() * a.B), dim3(kZThreads), zcol_lds<N>(), stream, p->T,
+                     p->XMy, p->bg_part, p->wsum, p->refpair, a.rot, a.ctf, p->twid, p->rnorm_part,
                      p->snorm_part);
-  if ((rc = mch::hip_check(hipGetLastError(), "shift_yz_kernel"))) return rc;
+  if ((rc = mch::hip_check(hipGetLastError(), "shift_z_kernel"))) return rc;
   mark(2);
-  hipLaunchKernelGGL(shift_inv_x_kernel<N>, dim3(N, a.B), dim3(row_threads<N>()), row_lds<N>(), stream,
+  hipLaunchKernelGGL(shift_inv_xy_kernel<N>, dim3(N, a.B), dim3(slice_threads<N>()), slice_lds<N>(), stream,
                      p->T, p->twid, p->amax_val, p->amax_idx);
-  if ((rc = mch::hip_check(hipGetLastError(), "shift_inv_x_kernel"))) return rc;
+  if ((rc = mch::hip_check(hipGetLastError(), "shift_inv_xy_kernel"))) return rc;
   mark(3);
   hipLaunchKernelGGL(shift_final_kernel, dim3(a.B), dim3(256), 0, stream, p->amax_val, p->amax_idx,
-                     p->ntile_x, p->rnorm_part, p->snorm_part, H, N, a.out_shift, a.out_score);
+                     p->ntile_x, p->rnorm_part, p->snorm_part, zparts<N>(), N, a.out_shift, a.out_score);
   rc = mch::hip_check(hipGetLastError(), "shift_final_kernel");
   mark(4);
   return rc;
@@ -992,8 +1230,9 @@ int mch_shift_plan_create(int N, const void* ref_host /* complex64 (N,N,N) */, f
   for (int i = 0; i < p->lut_n; ++i) lut[i] = wr2[lo + i];
   // weights of the rolled patch (offset (i + N/2) mod N - N/2), their sum and
   // x-spectrum XM[kx][z][y] = sum_x w(z, y, x) e^{-2 pi i kx x / N}
-  std::vector<float2> xm((size_t)H * N * N);
+  std::vector<float2> xm((size_t)H * N * N), xmy((size_t)H * N * N);
   {
+    std::vector<double> xre((size_t)H * N * N), xim((size_t)H * N * N);
     std::vector<double> c(N), s(N);
     for (int k = 0; k < N; ++k) {
       c[k] = cos(-2.0 * M_PI * k / N);
@@ -1017,10 +1256,27 @@ int mch_shift_plan_create(int N, const void* ref_host /* complex64 (N,N,N) */, f
             re += row[x] * c[j];
             im += row[x] * s[j];
           }
-          xm[((size_t)kx * N + z) * N + y] = make_float2((float)re, (float)im);
+          const size_t o = ((size_t)kx * N + z) * N + y;
+          xm[o] = make_float2((float)re, (float)im);
+          xre[o] = re;
+          xim[o] = im;
         }
       }
     p->wsum = (float)wsum;
+    // XMy[kx][z][ky] = sum_y XM[kx][z][y] e^{-2 pi i ky y / N}, from the FP64 values
+    for (size_t kz = 0; kz < (size_t)H * N; ++kz) {
+      const double* r = &xre[kz * N];
+      const double* q = &xim[kz * N];
+      for (int ky = 0; ky < N; ++ky) {
+        double re = 0.0, im = 0.0;
+        for (int y = 0; y < N; ++y) {
+          const int j = (ky * y) % N;
+          re += r[y] * c[j] - q[y] * s[j];
+          im += r[y] * s[j] + q[y] * c[j];
+        }
+        xmy[kz * N + ky] = make_float2((float)re, (float)im);
+      }
+    }
   }
   std::vector<float2> tw(N);
   for (int k = 0; k < N; ++k) {
@@ -1050,9 +1306,16 @@ int mch_shift_plan_create(int N, const void* ref_host /* complex64 (N,N,N) */, f
   MCH_TRY(hipMemcpy(p->lut, lut.data(), lut.size() * sizeof(float), hipMemcpyHostToDevice), "copy lut");
   MCH_TRY(hipMalloc(&p->XM, half * sizeof(float2)), "hipMalloc(XM)");
   MCH_TRY(hipMemcpy(p->XM, xm.data(), half * sizeof(float2), hipMemcpyHostToDevice), "copy XM");
+  MCH_TRY(hipMalloc(&p->XMy, half * sizeof(float2)), "hipMalloc(XMy)");
+  MCH_TRY(hipMemcpy(p->XMy, xmy.data(), half * sizeof(float2), hipMemcpyHostToDevice), "copy XMy");
   MCH_TRY(hipMalloc(&p->T, (size_t)max_batch * half * sizeof(float2)), "hipMalloc(T)");
-  MCH_TRY(hipMalloc(&p->rnorm_part, (size_t)max_batch * H * sizeof(float)), "hipMalloc");
-  MCH_TRY(hipMalloc(&p->snorm_part, (size_t)max_batch * H * sizeof(float)), "hipMalloc");
+  // norm partials: per column block on the integer path, per kx on the sub-voxel path
+  {
+    const int zp = ((H + 3) / 4) * (N / 8);
+    const size_t nparts = (size_t)max_batch * (zp > H ? zp : H);
+    MCH_TRY(hipMalloc(&p->rnorm_part, nparts * sizeof(float)), "hipMalloc");
+    MCH_TRY(hipMalloc(&p->snorm_part, nparts * sizeof(float)), "hipMalloc");
+  }
   MCH_TRY(hipMalloc(&p->bg_part, (size_t)max_batch * N * sizeof(float)), "hipMalloc");
   MCH_TRY(hipMalloc(&p->amax_val, (size_t)max_batch * p->ntile_x * sizeof(float)), "hipMalloc");
   MCH_TRY(hipMalloc(&p->amax_idx, (size_t)max_batch * p->ntile_x * sizeof(int)), "hipMalloc");
