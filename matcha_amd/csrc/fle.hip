@@ -318,17 +318,22 @@ __global__ __launch_bounds__(kGemmThreads) void fle_gemm_kernel(
 // ---------------------------------------------------------------------------
 // 1-D grid over (column tile, row tile, K split), XCD-aware so that the column
 // tiles of one class and K range (which read the same X rows) share one L2.
-// Tile 128 rows x 128 columns x 32 k per stage; wave w owns columns 32w..32w+31
-// (2 column blocks) and all 8 row blocks.  Both operands arrive by LDS-DMA
+// Tile 128 rows x 128 columns x 32 k per stage; a wave multiplies 64 columns
+// (4 column blocks) by 64 rows (4 waves, 2 x 2) or by all 128 rows (2 waves),
+// which bounds the LDS fragment reads per MFMA.  Both operands arrive by LDS-DMA
 // (global_load_lds_dwordx4: lane i's 16 bytes land at slot + 16 i), laid out
 // in MFMA fragment order, so every lane reads its own 16-byte fragment back:
 //   X hi/lo: 8 row blocks x 64 lanes (row rb*16 + lane%16, k 8*(lane/16)),
 //            shared by the 4 waves (each wave fetches 2 row blocks);
-//   W^T hi/lo: per wave 2 column blocks x 64 lanes, private to the wave.
+//   W^T hi/lo: 4 groups of 32 columns x 4 parts x 64 lanes, wave w fetching
+//            4 / NW groups.
 // NST stages rotate through LDS: while stage st is multiplied, stages
 // st + 1 .. st + NST - 1 are in flight (HBM latency x bandwidth), and no
 // operand passes through registers on its way in.
-constexpr int kStageU4 = 2048;   // u32x4 per stage: A hi 512, A lo 512, W 4 waves x 4 x 64
+// 128-row tiles per workgroup (8 waves: a 256-row tile) and u32x4 per stage
+// (A hi and lo 512 each per 128 rows, W^T 4 groups x 4 parts x 64)
+constexpr int gemm16_row_tiles(int nw) { return nw == 8 ? 2 : 1; }
+constexpr int gemm16_stage_u4(int nw) { return 1024 * gemm16_row_tiles(nw) + 1024; }
 
 // s_waitcnt immediates (gfx9 encoding: vmcnt bits [3:0] and [15:14], expcnt
 // [6:4], lgkmcnt [11:8]); the counters not named are left unconstrained
@@ -358,18 +363,25 @@ __global__ __launch_bounds__(64 * NW) void fle_gemm16_kernel(
     const uint16_t* __restrict__ Xhi, const uint16_t* __restrict__ Xlo, const u32x4* __restrict__ Wt16,
     float* __restrict__ part, int Mpad, int Kpad, int Ntot, int kchunk, int ntn, int nmt, FlePlan cols) {
   static_assert(NST >= 2 && NST <= 3, "pipeline depth");
-  static_assert(NW == 2 || NW == 4, "waves per workgroup");
-  constexpr int G = 4 / NW;        // 32-column groups per wave
-  constexpr int CB = 2 * G;        // 16-column blocks per wave
-  constexpr int AP = 16 / NW;      // A (row block, hi/lo) parts fetched per wave
-  extern __shared__ u32x4 lds[];   // NST x kStageU4
+  static_assert(NW == 2 || NW == 4 || NW == 8, "waves per workgroup");
+  constexpr int RT = gemm16_row_tiles(NW);  // 128-row tiles per workgroup (8 waves: 2)
+  constexpr int AREG = 512 * RT;            // u32x4 of the A hi (or lo) part of a stage
+  constexpr int W0 = 2 * AREG;              // start of the W^T part of a stage
+  constexpr int AP = 16 * RT / NW;          // A (row block, hi/lo) parts fetched per wave
+  constexpr int WP = 16 / NW;               // W^T parts (of 16) fetched per wave
+  // compute layout: every wave multiplies 64 columns (4 column blocks) by 64
+  // rows (4 waves as 2 x 2, 8 waves as 4 x 2 over 256 rows) or by all 128 rows
+  // (2 waves)
+  constexpr int RB = NW == 2 ? 8 : 4;  // 16-row blocks per wave
+  constexpr int CB = 4;                // 16-column blocks per wave
+  extern __shared__ u32x4 lds[];   // NST x gemm16_stage_u4(NW)
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int lb = mch::xcd_remap(blockIdx.x, gridDim.x);
   // row tile fastest: the row tiles that share a W^T tile run side by side on
   // one XCD, so W^T is fetched from HBM once for all of them
   const int mt = lb % nmt, rest = lb / nmt;
   const int tn = rest % ntn, ks = rest / ntn;
-  const int col0 = tn * BN16, row0 = mt * BM16;
+  const int col0 = tn * BN16, row0 = mt * BM16 * RT;
   int cls = 0;
   while (cls < kClasses - 1 && col0 >= cols.col_off[cls + 1]) ++cls;
   const int kbeg = ks * kchunk;
@@ -377,23 +389,23 @@ __global__ __launch_bounds__(64 * NW) void fle_gemm16_kernel(
   const int nstage = kend > kbeg ? (kend - kbeg) / BK16 : 0;
 
   // this lane's DMA sources: A parts (row block rb, hi/lo) of the row-major X,
-  // and this wave's G x 4 KB of the DMA-ordered W^T per stage
+  // and this wave's WP of the 16 1-KB parts of the DMA-ordered W^T per stage
   const long long xbase = ((long long)cls * Mpad + row0) * Kpad + (lane >> 4) * 8;
   const uint16_t* asrc[AP];
   int adst[AP];
 #pragma unroll
   for (int t = 0; t < AP; ++t) {
-    const int part = wave * AP + t;            // 0..15: row block part / 2, hi/lo part % 2
+    const int part = wave * AP + t;            // row block part / 2, hi/lo part % 2
     const int rb = part >> 1, hl = part & 1;
     asrc[t] = (hl ? Xlo : Xhi) + xbase + (long long)(rb * 16 + (lane & 15)) * Kpad;
-    adst[t] = hl * 512 + rb * 64;
+    adst[t] = hl * AREG + rb * 64;
   }
   const int nK = Kpad / BK16;
-  const u32x4* wsrc = Wt16 + ((long long)tn * nK + kbeg / BK16) * 1024 + wave * G * 256 + lane;
-  // stage layout (u32x4): A hi row block rb at rb*64, A lo at 512 + rb*64,
-  // W of 32-column group g, part q = {j * 2 + hi/lo} at 1024 + g*256 + q*64
+  const u32x4* wsrc = Wt16 + ((long long)tn * nK + kbeg / BK16) * 1024 + wave * WP * 64 + lane;
+  // stage layout (u32x4): A hi row block rb at rb*64, A lo at AREG + rb*64,
+  // W of 32-column group g, part q = {j * 2 + hi/lo} at W0 + g*256 + q*64
   auto issue = [&](int stage) {
-    u32x4* base = lds + (stage % NST) * kStageU4;
+    u32x4* base = lds + (stage % NST) * gemm16_stage_u4(NW);
     const int k0 = kbeg + stage * BK16;
 #pragma unroll
     for (int t = 0; t < AP; ++t) lds_dma16(asrc[t] + k0, base + adst[t]);
@@ -401,18 +413,20 @@ __global__ __launch_bounds__(64 * NW) void fle_gemm16_kernel(
     // several: keep it cached for the neighbouring row tiles
     if (nmt == 1) {
 #pragma unroll
-      for (int q = 0; q < 4 * G; ++q)
-        lds_dma16<2>(wsrc + (long long)stage * 1024 + q * 64, base + 1024 + wave * G * 256 + q * 64);
+      for (int q = 0; q < WP; ++q)
+        lds_dma16<2>(wsrc + (long long)stage * 1024 + q * 64, base + W0 + (wave * WP + q) * 64);
     } else {
 #pragma unroll
-      for (int q = 0; q < 4 * G; ++q)
-        lds_dma16<0>(wsrc + (long long)stage * 1024 + q * 64, base + 1024 + wave * G * 256 + q * 64);
+      for (int q = 0; q < WP; ++q)
+        lds_dma16<0>(wsrc + (long long)stage * 1024 + q * 64, base + W0 + (wave * WP + q) * 64);
     }
   };
-  constexpr int PER_STAGE = AP + 4 * G;  // DMAs per wave per stage
-  floatx4 acc[8][CB];
+  constexpr int PER_STAGE = AP + WP;  // DMAs per wave per stage
+  const int rb0 = NW == 8 ? RB * (wave & 3) : (NW == 4 ? RB * (wave & 1) : 0);  // first row block
+  const int chalf = NW == 8 ? (wave >> 2) : (NW == 4 ? (wave >> 1) : wave);   // 64-column half
+  floatx4 acc[RB][CB];
 #pragma unroll
-  for (int i = 0; i < 8; ++i)
+  for (int i = 0; i < RB; ++i)
 #pragma unroll
     for (int c = 0; c < CB; ++c) acc[i][c] = floatx4{0.f, 0.f, 0.f, 0.f};
 
@@ -431,33 +445,40 @@ __global__ __launch_bounds__(64 * NW) void fle_gemm16_kernel(
     // stage st - 1, whose slot is refilled next
     wg_barrier_keep_loads();
     if (st + NST - 1 < nstage) issue(st + NST - 1);
-    const u32x4* base = lds + (st % NST) * kStageU4;
-    u32x4 w[4 * G];
+    const u32x4* base = lds + (st % NST) * gemm16_stage_u4(NW);
+    // W^T parts of this wave's two 32-column groups (fetched by other waves too)
+    // every fragment of the stage is requested before the first MFMA, so that
+    // the LDS reads overlap the products instead of preceding each row block
+    u32x4 w[2 * CB], ah[RB], al[RB];
 #pragma unroll
-    for (int q = 0; q < 4 * G; ++q) w[q] = base[1024 + wave * G * 256 + q * 64 + lane];
+    for (int q = 0; q < 2 * CB; ++q) w[q] = base[W0 + chalf * 512 + q * 64 + lane];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const u32x4 ah = base[i * 64 + lane];
-      const u32x4 al = base[512 + i * 64 + lane];
+    for (int i = 0; i < RB; ++i) {
+      ah[i] = base[(rb0 + i) * 64 + lane];
+      al[i] = base[AREG + (rb0 + i) * 64 + lane];
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the products
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
 #pragma unroll
       for (int c = 0; c < CB; ++c) {
         // column block c = group c / 2, block c % 2: parts q = 4 (c / 2) + 2 (c % 2) + {0, 1}
-        acc[i][c] = Split16<FP16>::mfma(ah, w[2 * c + 0], acc[i][c]);
-        acc[i][c] = Split16<FP16>::mfma(ah, w[2 * c + 1], acc[i][c]);
-        acc[i][c] = Split16<FP16>::mfma(al, w[2 * c + 0], acc[i][c]);
+        acc[i][c] = Split16<FP16>::mfma(ah[i], w[2 * c + 0], acc[i][c]);
+        acc[i][c] = Split16<FP16>::mfma(ah[i], w[2 * c + 1], acc[i][c]);
+        acc[i][c] = Split16<FP16>::mfma(al[i], w[2 * c + 0], acc[i][c]);
       }
     }
   }
   // D layout of the 16x16 MFMAs: lane holds rows 4 (lane / 16) + r, column lane % 16
   float* P = part + (long long)ks * Mpad * Ntot;
 #pragma unroll
-  for (int i = 0; i < 8; ++i)
+  for (int i = 0; i < RB; ++i)
 #pragma unroll
     for (int c = 0; c < CB; ++c)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = row0 + i * 16 + 4 * (lane >> 4) + r;
-        const int col = col0 + wave * G * 32 + c * 16 + (lane & 15);
+        const int row = row0 + (rb0 + i) * 16 + 4 * (lane >> 4) + r;
+        const int col = col0 + chalf * 64 + c * 16 + (lane & 15);
         P[(long long)row * Ntot + col] = acc[i][c][r];
       }
 }
@@ -689,23 +710,33 @@ void free_fle_plan(FlePlan* p) {
   delete p;
 }
 
-// the projection GEMM of the plan's precision on its X operand
-// Autotune candidates of the 16-bit GEMM: pipeline depth x waves per
-// workgroup x rounds of resident workgroups the K split fills.
-constexpr int kGemm16Candidates = 12;
-inline int gemm16_nst(int cand) { return 2 + (cand & 1); }
-inline int gemm16_nw(int cand) { return (cand & 2) ? 2 : 4; }
-inline int gemm16_rounds(int cand) { return 1 + (cand >> 2); }
+// Autotune candidates of the 16-bit GEMM: (pipeline depth, waves per
+// workgroup, rounds of resident workgroups the K split fills).  8 waves hold a
+// 256-row tile (candidates only when the plan's rows come in 256s).
+struct Gemm16Cand { int nst, nw, rounds; };
+constexpr Gemm16Cand kGemm16Cands[] = {
+    {2, 4, 1}, {3, 4, 1}, {2, 2, 1}, {3, 2, 1}, {2, 4, 2}, {3, 4, 2}, {2, 2, 2}, {3, 2, 2},
+    {2, 4, 3}, {3, 4, 3}, {2, 2, 3}, {3, 2, 3}, {2, 8, 1}, {3, 8, 1}, {2, 8, 2}, {3, 8, 2}};
+constexpr int kGemm16Candidates = (int)(sizeof(kGemm16Cands) / sizeof(kGemm16Cands[0]));
+inline bool gemm16_usable(const FlePlan* p, int cand) {
+  return kGemm16Cands[cand].nw != 8 || p->Mpad % (2 * BM16) == 0;
+}
+
+inline size_t gemm16_lds(int nst, int nw) { return (size_t)nst * gemm16_stage_u4(nw) * sizeof(u32x4); }
+
+template <bool FP16>
+const void* gemm16_fn(int nst, int nw) {
+  if (nw == 8) return nst == 3 ? (const void*)fle_gemm16_kernel<FP16, 3, 8> : (const void*)fle_gemm16_kernel<FP16, 2, 8>;
+  if (nw == 4) return nst == 3 ? (const void*)fle_gemm16_kernel<FP16, 3, 4> : (const void*)fle_gemm16_kernel<FP16, 2, 4>;
+  return nst == 3 ? (const void*)fle_gemm16_kernel<FP16, 3, 2> : (const void*)fle_gemm16_kernel<FP16, 2, 2>;
+}
 
 // Resident workgroups per CU of one tile configuration.
 int gemm16_occupancy(int nst, int nw, int* occ) {
-  const size_t lds = (size_t)nst * kStageU4 * sizeof(u32x4);
-  const void* fn = nullptr;
-  if (nw == 4) fn = nst == 3 ? (const void*)fle_gemm16_kernel<false, 3, 4> : (const void*)fle_gemm16_kernel<false, 2, 4>;
-  else fn = nst == 3 ? (const void*)fle_gemm16_kernel<false, 3, 2> : (const void*)fle_gemm16_kernel<false, 2, 2>;
   int n = 0;
-  const int rc = mch::hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, 64 * nw, lds),
-                                "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  const int rc = mch::hip_check(
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm16_fn<false>(nst, nw), 64 * nw, gemm16_lds(nst, nw)),
+      "hipOccupancyMaxActiveBlocksPerMultiprocessor");
   *occ = n > 0 ? n : 1;
   return rc;
 }
@@ -713,8 +744,8 @@ int gemm16_occupancy(int nst, int nw, int* occ) {
 // K split that fills `rounds` rounds of `slots` resident workgroups with
 // the (column tile x row tile) grid, rounded so that every split but the
 // last has the same number of K blocks.
-int gemm16_splits(const FlePlan* p, int slots, int rounds) {
-  const int ntiles = p->Ntot / BN16 * (p->Mpad / BM16);
+int gemm16_splits(const FlePlan* p, int nw, int slots, int rounds) {
+  const int ntiles = p->Ntot / BN16 * (p->Mpad / (BM16 * gemm16_row_tiles(nw)));
   const int kblocks = p->Kpad / BK16;
   int s = rounds * slots / ntiles;
   s = std::min(std::max(s, 1), kblocks);
@@ -737,23 +768,25 @@ int launch_gemm(FlePlan* p, int B, hipStream_t stream, int* splits_out) {
   const int kchunk = ((kblocks + p->ksplit16 - 1) / p->ksplit16) * BK16;
   const int splits = (p->Kpad + kchunk - 1) / kchunk;
   const int ntn = p->Ntot / BN16;
-  const dim3 grid(ntn * mt * splits);
-  const size_t lds = (size_t)p->nst16 * kStageU4 * sizeof(u32x4);
+  const int rt = gemm16_row_tiles(p->nw16);
+  const int mt16 = (2 * B + BM16 * rt - 1) / (BM16 * rt);
+  const dim3 grid(ntn * mt16 * splits);
+  const size_t lds = gemm16_lds(p->nst16, p->nw16);
   const bool fp16 = p->precision == kFp16x3;
 #define MCH_GEMM16(F, S, W)                                                                        \
   hipLaunchKernelGGL((fle_gemm16_kernel<F, S, W>), grid, dim3(64 * W), lds, stream, p->Xhi, p->Xlo, \
-                     (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, kchunk, ntn, mt, *p)
-  if (p->nw16 == 2) {
-    if (fp16 && p->nst16 == 3) MCH_GEMM16(true, 3, 2);
-    else if (fp16) MCH_GEMM16(true, 2, 2);
-    else if (p->nst16 == 3) MCH_GEMM16(false, 3, 2);
-    else MCH_GEMM16(false, 2, 2);
-  } else {
-    if (fp16 && p->nst16 == 3) MCH_GEMM16(true, 3, 4);
-    else if (fp16) MCH_GEMM16(true, 2, 4);
-    else if (p->nst16 == 3) MCH_GEMM16(false, 3, 4);
-    else MCH_GEMM16(false, 2, 4);
-  }
+                     (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, kchunk, ntn, mt16, *p)
+#define MCH_GEMM16_NW(W)                               \
+  do {                                                 \
+    if (fp16 && p->nst16 == 3) MCH_GEMM16(true, 3, W); \
+    else if (fp16) MCH_GEMM16(true, 2, W);             \
+    else if (p->nst16 == 3) MCH_GEMM16(false, 3, W);   \
+    else MCH_GEMM16(false, 2, W);                      \
+  } while (0)
+  if (p->nw16 == 8) MCH_GEMM16_NW(8);
+  else if (p->nw16 == 2) MCH_GEMM16_NW(2);
+  else MCH_GEMM16_NW(4);
+#undef MCH_GEMM16_NW
 #undef MCH_GEMM16
   *splits_out = splits;
   return mch::hip_check(hipGetLastError(), "fle_gemm16_kernel launch");
@@ -873,15 +906,16 @@ int mch_fle_set_precision(void* plan, int precision) {
     MCH_TRY(hipMalloc(&p->Wt16, 2 * nw * sizeof(uint16_t)), "hipMalloc(Wt16)");
     MCH_TRY(hipMalloc(&p->colscale, (size_t)p->Ntot * sizeof(float)), "hipMalloc(colscale)");
     MCH_TRY(hipMalloc(&p->rowmax, (size_t)p->Mpad * sizeof(float)), "hipMalloc(rowmax)");
-    const int lds3 = 3 * kStageU4 * (int)sizeof(u32x4);
-    MCH_TRY(hipFuncSetAttribute((const void*)fle_gemm16_kernel<false, 3, 4>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, lds3), "hipFuncSetAttribute");
-    MCH_TRY(hipFuncSetAttribute((const void*)fle_gemm16_kernel<true, 3, 4>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, lds3), "hipFuncSetAttribute");
-    MCH_TRY(hipFuncSetAttribute((const void*)fle_gemm16_kernel<false, 3, 2>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, lds3), "hipFuncSetAttribute");
-    MCH_TRY(hipFuncSetAttribute((const void*)fle_gemm16_kernel<true, 3, 2>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, lds3), "hipFuncSetAttribute");
+    // dynamic LDS above 64 KB for every tile configuration that needs it
+    for (int cand = 0; cand < kGemm16Candidates; ++cand) {
+      const int nst = kGemm16Cands[cand].nst, nw = kGemm16Cands[cand].nw;
+      const int lds = (int)gemm16_lds(nst, nw);
+      if (lds <= 64 * 1024) continue;
+      MCH_TRY(hipFuncSetAttribute(gemm16_fn<false>(nst, nw), hipFuncAttributeMaxDynamicSharedMemorySize, lds),
+              "hipFuncSetAttribute");
+      MCH_TRY(hipFuncSetAttribute(gemm16_fn<true>(nst, nw), hipFuncAttributeMaxDynamicSharedMemorySize, lds),
+              "hipFuncSetAttribute");
+    }
     // partials for the largest K split any autotune candidate uses
     int cus = 0, dev = 0;
     MCH_TRY(hipGetDevice(&dev), "hipGetDevice");
@@ -889,9 +923,11 @@ int mch_fle_set_precision(void* plan, int precision) {
     p->cus = cus > 0 ? cus : 256;
     int smax = 1;
     for (int cand = 0; cand < kGemm16Candidates; ++cand) {
+      if (!gemm16_usable(p, cand)) continue;
+      const Gemm16Cand& c = kGemm16Cands[cand];
       int occ = 0;
-      if ((rc = gemm16_occupancy(gemm16_nst(cand), gemm16_nw(cand), &occ))) return rc;
-      smax = std::max(smax, gemm16_splits(p, occ * p->cus, gemm16_rounds(cand)));
+      if ((rc = gemm16_occupancy(c.nst, c.nw, &occ))) return rc;
+      smax = std::max(smax, gemm16_splits(p, c.nw, occ * p->cus, c.rounds));
     }
     p->splits16 = smax;
     p->ksplit16 = smax;
@@ -925,22 +961,24 @@ int mch_fle_set_precision(void* plan, int precision) {
     p->table_precision = precision;
     p->precision = precision;
     // autotune on the full batch (timing does not depend on the operand
-    // values): pipeline depth (2 stages fit two workgroups per CU, 3 keep more
-    // HBM traffic in flight per workgroup), waves per workgroup, and the K
-    // split as 1, 2 or 3 full rounds of resident workgroups (a partial last
-    // round leaves most CUs idle while it runs)
+    // values): pipeline depth (more stages keep more traffic in flight per
+    // workgroup), waves per workgroup (8: a 256-row tile, which halves the
+    // W^T bytes per MFMA but holds one workgroup per CU), and the K split as
+    // 1, 2 or 3 full rounds of resident workgroups (a partial last round
+    // leaves most CUs idle while it runs)
     hipEvent_t e0, e1;
     MCH_TRY(hipEventCreate(&e0), "hipEventCreate");
     MCH_TRY(hipEventCreate(&e1), "hipEventCreate");
     float best = 1e30f;
     int best_nst = 2, best_nw = 4, best_split = p->splits16;
     for (int cand = 0; cand < kGemm16Candidates; ++cand) {
-      const int nst = gemm16_nst(cand), nw = gemm16_nw(cand);
+      if (!gemm16_usable(p, cand)) continue;
+      const int nst = kGemm16Cands[cand].nst, nw = kGemm16Cands[cand].nw;
       int occ = 0;
       if ((rc = gemm16_occupancy(nst, nw, &occ))) break;
       p->nst16 = nst;
       p->nw16 = nw;
-      p->ksplit16 = gemm16_splits(p, occ * p->cus, gemm16_rounds(cand));
+      p->ksplit16 = gemm16_splits(p, nw, occ * p->cus, kGemm16Cands[cand].rounds);
       int splits = 0;
       for (int rep = 0; rep < 2; ++rep)
         if ((rc = launch_gemm(p, p->max_batch, 0, &splits))) break;

@@ -950,6 +950,33 @@ __global__ __launch_bounds__(256) void extract_patch_kernel(const float* __restr
   make_shift_spec(shift_zyx, b, N, sp);
   const float* v = vols + (long long)b * ldv;
   float* o = out + (long long)b * ldo + (long long)z * N * N;
+  if (sp.integral && (N & 3) == 0 && (reinterpret_cast<uintptr_t>(o) & 15) == 0) {
+    // integral shift: a shifted copy, four outputs per thread and store;
+    // (y, q) advanced incrementally (no division in the loop)
+    const int nq = N >> 2;
+    const int zs = z + sp.si[0];
+    const bool zok = zs >= 0 && zs < N;
+    const float* src = v + (long long)(zok ? zs : 0) * N * N;
+    const int dy = blockDim.x / nq, dq = blockDim.x - dy * nq;
+    int y = threadIdx.x / nq, q = threadIdx.x - (threadIdx.x / nq) * nq;
+    float4* o4 = reinterpret_cast<float4*>(o);
+#pragma unroll 4
+    for (int t = threadIdx.x; t < N * nq; t += blockDim.x) {
+      const int ys = y + sp.si[1], x0 = 4 * q + sp.si[2];
+      const bool rok = zok && ys >= 0 && ys < N;
+      const float* row = src + (long long)(rok ? ys : 0) * N;
+      float4 r;
+      r.x = rok && x0 >= 0 && x0 < N ? row[x0] : 0.f;
+      r.y = rok && x0 + 1 >= 0 && x0 + 1 < N ? row[x0 + 1] : 0.f;
+      r.z = rok && x0 + 2 >= 0 && x0 + 2 < N ? row[x0 + 2] : 0.f;
+      r.w = rok && x0 + 3 >= 0 && x0 + 3 < N ? row[x0 + 3] : 0.f;
+      o4[t] = r;
+      q += dq;
+      y += dy;
+      if (q >= nq) { q -= nq; ++y; }
+    }
+    return;
+  }
   for (int t = threadIdx.x; t < N * N; t += blockDim.x) {
     const int y = t / N, x = t - y * N;
     o[t] = patch_value(v, N, sp, z, y, x);

@@ -124,6 +124,26 @@ def test_extract_patches_matches_oracle(gpu_device):
         assert np.max(np.abs(got[b] - ref)) < 1e-5
 
 
+@pytest.mark.parametrize("N", [48, 96])
+def test_extract_patches_integer_shifts_exact(gpu_device, N):
+    # integral shifts are a shifted copy with zeros outside: exact, including
+    # shifts that leave part or all of the volume (per axis and both signs)
+    from matcha_amd.core.ShiftMatcher import extract_shifted_patches
+    rng = np.random.default_rng(N)
+    B = 8
+    vols = rng.standard_normal((B, N, N, N)).astype(np.float32)
+    shifts = rng.integers(-N // 3, N // 3 + 1, (B, 3)).astype(np.float32)
+    shifts[0] = (0, 0, 0)
+    shifts[1] = (N - 1, -(N - 1), 1)
+    shifts[2] = (-N, 0, 0)
+    shifts[3] = (3, 0, -N - 2)
+    got = extract_shifted_patches(torch.tensor(vols, device=gpu_device),
+                                  torch.tensor(shifts, device=gpu_device)).cpu().numpy()
+    for b in range(B):
+        ref = S.extract_shifted_patch(vols[b].astype(np.float64), shifts[b][::-1].astype(np.float64))
+        assert np.array_equal(got[b], ref.astype(np.float32)), b
+
+
 @pytest.mark.parametrize("mode", ["reference", "global"])
 def test_alternation_recovers_rotation_and_shift(gpu_device, tmp_path, mode):
     from scipy.spatial.transform import Rotation
