@@ -106,6 +106,12 @@ int mch_fle_plan_destroy(void* plan);
  * FLEBasis3D.evaluate_t (src/matcha/core/FLEBasis3D.py:945-976,1026-1267). */
 int mch_fle_project(void* plan, const float* vols, long long ld_vol, int B, void* coeffs,
                     long long lda, hipStream_t stream);
+/* The same on the patches vol[a + flip(shift_zyx[b])] (shift_zyx (B, 3)
+ * float32 on the device, NULL for none), read straight from the volumes:
+ * evaluate_t of extract_shifted_patch_batch's output
+ * (src/matcha/utils/io_utils.py:113-178) without the patch copy. */
+int mch_fle_project_shifted(void* plan, const float* vols, long long ld_vol, int B, const float* shift_zyx,
+                            void* coeffs, long long lda, hipStream_t stream);
 
 /* The projection GEMM alone on the operand of the last mch_fle_project
  * (timing aid; no reference counterpart). */

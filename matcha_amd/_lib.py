@@ -42,6 +42,7 @@ _SIGNATURES = [
                                     c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(c_void_p)]),
     ("mch_fle_plan_destroy", c_int, [c_void_p]),
     ("mch_fle_project", c_int, [c_void_p, c_void_p, c_ll, c_int, c_void_p, c_ll, c_void_p]),
+    ("mch_fle_project_shifted", c_int, [c_void_p, c_void_p, c_ll, c_int, c_void_p, c_void_p, c_ll, c_void_p]),
     ("mch_fle_gemm_only", c_int, [c_void_p, c_int, c_void_p]),
     ("mch_fle_set_precision", c_int, [c_void_p, c_int]),
     ("mch_fle_tuned_depth", c_int, [c_void_p]),

@@ -21,7 +21,7 @@ from __future__ import annotations
 import torch
 
 from . import _lib
-from .core.ShiftMatcher import euler_to_shift_rotations, extract_shifted_patches
+from .core.ShiftMatcher import euler_to_shift_rotations
 
 
 def extract_rotated_batch(subtomos: torch.Tensor, shift: torch.Tensor | None, rot: torch.Tensor,
@@ -100,6 +100,7 @@ class Alternation:
         self.mask_radius = mask_radius
         self._patches = None
         self._work = None
+        self._shift0 = None
         self.history = []
         self.tracker = None
 
@@ -121,31 +122,34 @@ class Alternation:
             raise ValueError("a prior rotation needs mode='reference'")
         return self._run_global(subtomos, ctfs)
 
-    def _buffers(self, subtomos):
-        if self._patches is None or self._patches.shape != subtomos.shape:
+    def _buffers(self, subtomos, patches: bool):
+        """Running-shift buffer, and (reference mode) the rotated-patch buffers."""
+        if self._shift0 is None or self._shift0.shape[0] != subtomos.shape[0]:
+            self._shift0 = torch.empty((subtomos.shape[0], 3), dtype=torch.float32, device=subtomos.device)
+        if patches and (self._patches is None or self._patches.shape != subtomos.shape):
             self._patches = torch.empty_like(subtomos, dtype=torch.float32)
             self._work = torch.empty_like(subtomos, dtype=torch.float32)
-            self._shift0 = torch.empty((subtomos.shape[0], 3), dtype=torch.float32, device=subtomos.device)
 
     def _run_global(self, subtomos, ctfs=None):
         B = subtomos.shape[0]
-        self._buffers(subtomos)
+        self._buffers(subtomos, patches=False)
         shift = self._shift0.zero_()
-        vols = subtomos
         out = None
         self.history = []
         for it in range(self.num_alternations):
-            if it > 0:
-                vols = extract_shifted_patches(subtomos, shift, out=self._patches)
-            sigma, norms = self.ccm.get_sigma(vols, max_degree=self.rot.Lmax)
+            # alternations after the first see the patches at the running shift;
+            # the projection and the shift search read them straight from the
+            # particles (the values extract_shifted_patches would copy out)
+            sh = shift if it > 0 else None
+            sigma, norms = self.ccm.get_sigma(subtomos, max_degree=self.rot.Lmax, shift_zyx=sh)
             a, b, g, s = self.rot.search_orientations(sigma[:B])
             nscore = s / (norms[:B] * self.ccm.bh_norm_template).clamp_min(1e-12)
             last = it == self.num_alternations - 1
             sscore = None
             if not last or self.final_shift:
                 rb = euler_to_shift_rotations(a, b, g)
-                ds, sscore = self.sm.search_shifts(subtomos=vols, ctfs=ctfs,
-                                                   shift_zyx=None, rotation_tracker=rb)
+                ds, sscore = self.sm.search_shifts(subtomos=subtomos, ctfs=ctfs,
+                                                   shift_zyx=sh, rotation_tracker=rb)
                 shift = shift + ds
             out = (a, b, g, nscore, shift, sscore)
             self.history.append(out)
@@ -155,7 +159,7 @@ class Alternation:
         B, N = subtomos.shape[0], subtomos.shape[1]
         dev = subtomos.device
         radius = self.mask_radius if self.mask_radius is not None else getattr(self.ccm, "radius", N // 2)
-        self._buffers(subtomos)
+        self._buffers(subtomos, patches=True)
         shift = self._shift0.zero_()
         self.tracker = RotationTracker(B, dev, prior_tracker)
         out = None

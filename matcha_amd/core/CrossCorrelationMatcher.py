@@ -128,13 +128,14 @@ class CrossCorrelationMatcher:
                                                  norms.data_ptr(), _lib.stream_handle(self.device))
         _lib.check(rc, "mch_compute_sigma")
 
-    def get_sigma(self, volumes: torch.Tensor, max_degree: int = None):
+    def get_sigma(self, volumes: torch.Tensor, max_degree: int = None, shift_zyx: torch.Tensor = None):
         """(sigma (batchsize, Dsize(lmax)) complex64, bh_norms (batchsize,)).
 
         Returns the persistent buffers (overwritten by the next call), as the
         reference does; rows beyond the input batch are zero.  ``max_degree``
         (not in the reference) writes only the shells l <= max_degree, all a
-        search up to that L_max reads."""
+        search up to that L_max reads; ``shift_zyx`` (not in the reference)
+        projects the patches extract_shifted_patches would cut at those shifts."""
         num = volumes.shape[0]
         if num > self.batchsize:
             raise ValueError(
@@ -142,6 +143,6 @@ class CrossCorrelationMatcher:
         if num < self.batchsize:
             self.sigma[num:].zero_()
             self.bh_norms[num:].zero_()
-        a = self.fle.evaluate_t(volumes, out=self.bh_volumes_buffer[:num])
+        a = self.fle.evaluate_t(volumes, out=self.bh_volumes_buffer[:num], shift_zyx=shift_zyx)
         self.compute_sigma(a, 0, num, max_degree)
         return self.sigma, self.bh_norms

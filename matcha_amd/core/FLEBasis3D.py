@@ -322,11 +322,15 @@ class FLEBasis3D:
                 pass
 
     # -- transform ---------------------------------------------------------------
-    def evaluate_t(self, f: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+    def evaluate_t(self, f: torch.Tensor, out: torch.Tensor | None = None,
+                   shift_zyx: torch.Tensor | None = None) -> torch.Tensor:
         """a = B^H f for a batch (Bt, N, N, N) (or one (N, N, N) volume) of real volumes.
 
         Returns (Bt, ne) complex64.  Unlike the reference, the caller's tensor is
-        not modified (the reference zeroes its out-of-ball voxels in place)."""
+        not modified (the reference zeroes its out-of-ball voxels in place).
+        ``shift_zyx`` (Bt, 3) (not in the reference): project the patches
+        ``extract_shifted_patches(f, shift_zyx)`` would cut, read straight from
+        ``f`` (no patch copy)."""
         if f.ndim == 3:
             f = f.unsqueeze(0)
         _lib.require_gpu_tensor(f, "volumes", device=self.device)
@@ -340,8 +344,16 @@ class FLEBasis3D:
         f = f.contiguous()
         if out is None:
             out = torch.empty((Bt, self.ne), dtype=torch.complex64, device=self.device)
-        rc = _lib.lib().mch_fle_project(self._plan, f.data_ptr(), self.N1 ** 3, Bt, out.data_ptr(),
-                                        out.stride(0), _lib.stream_handle(self.device))
+        if shift_zyx is None:
+            rc = _lib.lib().mch_fle_project(self._plan, f.data_ptr(), self.N1 ** 3, Bt, out.data_ptr(),
+                                            out.stride(0), _lib.stream_handle(self.device))
+        else:
+            _lib.require_gpu_tensor(shift_zyx, "shift_zyx", device=self.device)
+            sh = shift_zyx.to(torch.float32).reshape(-1, 3).contiguous()
+            if sh.shape[0] != Bt:
+                raise ValueError(f"shift_zyx must be ({Bt}, 3), got {tuple(shift_zyx.shape)}")
+            rc = _lib.lib().mch_fle_project_shifted(self._plan, f.data_ptr(), self.N1 ** 3, Bt, sh.data_ptr(),
+                                                    out.data_ptr(), out.stride(0), _lib.stream_handle(self.device))
         _lib.check(rc, "mch_fle_project")
         return out
 

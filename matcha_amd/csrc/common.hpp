@@ -70,6 +70,61 @@ __device__ __forceinline__ float tri_sample_real(const float* __restrict__ v, in
   return acc;
 }
 
+// Raw buffer resource over nbytes at p: loads at byte offsets >= nbytes return 0
+// (hardware range check), so out-of-volume samples need no branch.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)nbytes, 0x00020000);
+}
+constexpr unsigned kOutOfRange = 0x80000000u;
+__device__ __forceinline__ float4 load_f4(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  typedef unsigned v4u __attribute__((ext_vector_type(4)));
+  const v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
+}
+__device__ __forceinline__ float load_f1(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
+// Shifted patch value at array index (a0, a1, a2): vol[a + s] with trilinear
+// sampling for fractional s and the border slice zeroed as the reference does.
+struct ShiftSpec {
+  float s[3];
+  int integral;       // all components integral
+  int zero_idx[3];    // index of the slice to zero along each axis (-1: none)
+  int si[3];
+};
+
+__device__ __forceinline__ float patch_value(const float* __restrict__ v, int N, const ShiftSpec& sp,
+                                             int a0, int a1, int a2) {
+  if ((sp.zero_idx[0] >= 0 && a0 == sp.zero_idx[0]) || (sp.zero_idx[1] >= 0 && a1 == sp.zero_idx[1]) ||
+      (sp.zero_idx[2] >= 0 && a2 == sp.zero_idx[2]))
+    return 0.f;
+  if (sp.integral) {
+    const int z = a0 + sp.si[0], y = a1 + sp.si[1], x = a2 + sp.si[2];
+    if (z < 0 || z >= N || y < 0 || y >= N || x < 0 || x >= N) return 0.f;
+    return v[((long long)z * N + y) * N + x];
+  }
+  return tri_sample_real(v, N, a0 + sp.s[0], a1 + sp.s[1], a2 + sp.s[2]);
+}
+
+__device__ __forceinline__ void make_shift_spec(const float* __restrict__ shift_zyx, int b, int N, ShiftSpec& sp) {
+  // the patch shift in array-axis order is the flipped (z, y, x) shift
+  sp.integral = 1;
+  for (int d = 0; d < 3; ++d) {
+    const float s = shift_zyx ? shift_zyx[b * 3 + (2 - d)] : 0.f;
+    sp.s[d] = s;
+    const float a = fabsf(s);
+    const float fl = floorf(a);
+    sp.si[d] = (int)rintf(s);
+    sp.zero_idx[d] = -1;
+    if (a - fl > 0.f) {
+      sp.integral = 0;
+      // idx = floor(|s|) if s < 0 else -ceil(|s|) (python negative index)
+      sp.zero_idx[d] = (s < 0.f) ? (int)fl : N - (int)ceilf(a);
+    }
+  }
+}
+
 // XCD-aware block remap: the hardware hands consecutive workgroups to the 8
 // XCDs round-robin; map them so that a contiguous range of logical blocks
 // (e.g. all candidates of one particle) lands on one XCD and shares its L2.

@@ -27,6 +27,13 @@
 
 namespace {
 
+using mch::kOutOfRange;
+using mch::load_f1;
+using mch::make_rsrc;
+using mch::make_shift_spec;
+using mch::patch_value;
+using mch::ShiftSpec;
+
 constexpr int kClasses = 8;
 constexpr int BM = 128, BN = 64, BK = 16;
 constexpr int kGemmThreads = 256;
@@ -41,6 +48,7 @@ struct FlePlan {
   int N, V0, Kpad, Ntot, ne, max_batch;
   int col_off[kClasses + 1];  // padded column offsets per class
   int* orbit;                 // (V0, 16)
+  int* fcoord;                // (V0) fundamental voxel (x, y, z) >= 0, packed x | y << 10 | z << 20
   float* W;                   // (Kpad, Ntot)
   int* chan_plus;             // (Ntot)
   int* chan_minus;            // (Ntot)
@@ -119,14 +127,15 @@ __host__ __device__ __forceinline__ float pow2_scale(float m) {
 // ---------------------------------------------------------------------------
 // For class (eps, rho): G_t = sum_q (-1)^{q eps} sum_p (-i)^{rho p} f[p,q,t],
 // X1 = A0+A1, X2 = B0-B1, X3 = B0+B1, X4 = A1-A0 with G_t = A_t + i B_t.
-// SPLIT = false: X in FP32; rowmax (Mpad floats, zeroed by the caller; NULL to
-// skip) receives max |X| of each GEMM row, for the power-of-2 row scale of the
-// FP16 split GEMM.  SPLIT = true: X written directly as BF16 hi/lo (Xhi, Xlo).
-template <bool SPLIT>
+// X in FP32; rowmax (Mpad floats, zeroed by the caller; NULL to skip)
+// receives max |X| of each GEMM row, for the power-of-2 row scale of the FP16
+// split GEMM.  shift_zyx (B, 3) or NULL: the volumes are read as the patches
+// extract_shifted_patches would cut from them (vol[a + flip(shift)], trilinear
+// for fractional shifts).
 __global__ void fle_symmetrize_kernel(const float* __restrict__ vols, long long ld_vol, int B,
-                                      const int* __restrict__ orbit, int V0, int Kpad, int Mpad,
-                                      float* __restrict__ X, float* __restrict__ rowmax,
-                                      uint16_t* __restrict__ Xhi, uint16_t* __restrict__ Xlo) {
+                                      const int* __restrict__ orbit, int V0, int N, int Kpad, int Mpad,
+                                      const float* __restrict__ shift_zyx, float* __restrict__ X,
+                                      float* __restrict__ rowmax) {
   const long long tot = (long long)B * V0;
   const long long stride = (long long)gridDim.x * blockDim.x;
   // block-uniform trip count so that the wave reductions see every lane
@@ -140,13 +149,22 @@ __global__ void fle_symmetrize_kernel(const float* __restrict__ vols, long long 
       const float* f = vols + (long long)b * ld_vol;
       const int4* o4 = reinterpret_cast<const int4*>(orbit + (long long)v * 16);
       float g[16];
+      if (shift_zyx) {
+        ShiftSpec sp;
+        make_shift_spec(shift_zyx, b, N, sp);
+        for (int i = 0; i < 16; ++i) {
+          const int id = orbit[(long long)v * 16 + i];
+          g[i] = patch_value(f, N, sp, id / (N * N), (id / N) % N, id % N);
+        }
+      } else {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int4 id = o4[i];
-        g[4 * i + 0] = f[id.x];
-        g[4 * i + 1] = f[id.y];
-        g[4 * i + 2] = f[id.z];
-        g[4 * i + 3] = f[id.w];
+        for (int i = 0; i < 4; ++i) {
+          const int4 id = o4[i];
+          g[4 * i + 0] = f[id.x];
+          g[4 * i + 1] = f[id.y];
+          g[4 * i + 2] = f[id.z];
+          g[4 * i + 3] = f[id.w];
+        }
       }
       // g[p + 4q + 8t]
 #pragma unroll
@@ -180,20 +198,8 @@ __global__ void fle_symmetrize_kernel(const float* __restrict__ vols, long long 
           const float X4 = A[1][rho] - A[0][rho];
           const long long orow = ((long long)cls * Mpad + b) * Kpad + 2 * v;
           const long long oim = ((long long)cls * Mpad + B + b) * Kpad + 2 * v;
-          if constexpr (SPLIT) {
-            uint16_t h1, l1, h2, l2, h3, l3, h4, l4;
-            Split16<false>::split(X1, h1, l1);
-            Split16<false>::split(X2, h2, l2);
-            Split16<false>::split(X3, h3, l3);
-            Split16<false>::split(X4, h4, l4);
-            *reinterpret_cast<unsigned*>(Xhi + orow) = (unsigned)h1 | ((unsigned)h2 << 16);
-            *reinterpret_cast<unsigned*>(Xlo + orow) = (unsigned)l1 | ((unsigned)l2 << 16);
-            *reinterpret_cast<unsigned*>(Xhi + oim) = (unsigned)h3 | ((unsigned)h4 << 16);
-            *reinterpret_cast<unsigned*>(Xlo + oim) = (unsigned)l3 | ((unsigned)l4 << 16);
-          } else {
-            *reinterpret_cast<float2*>(X + orow) = make_float2(X1, X2);
-            *reinterpret_cast<float2*>(X + oim) = make_float2(X3, X4);
-          }
+          *reinterpret_cast<float2*>(X + orow) = make_float2(X1, X2);
+          *reinterpret_cast<float2*>(X + oim) = make_float2(X3, X4);
           mre = fmaxf(mre, fmaxf(fabsf(X1), fabsf(X2)));
           mim = fmaxf(mim, fmaxf(fabsf(X3), fabsf(X4)));
         }
@@ -226,6 +232,133 @@ __global__ void fle_symmetrize_kernel(const float* __restrict__ vols, long long 
         }
       }
     }
+  }
+}
+
+// BF16x3 path: the same combinations for 4 consecutive fundamental voxels per
+// thread.  Their 64 orbit values are requested at once (indices from the
+// packed coordinates: image p + 4q + 8t of (x, y, z) is R^p My^t Mz^q, R the
+// quarter turn (x, y) -> (-y, x)), and each row's 4 x (X1, X2) or (X3, X4)
+// hi/lo pairs leave as one 16-byte store.  Columns of a last partial group
+// (v >= V0) receive zeros, inside the row's K padding.
+template <bool SHIFTED>
+__global__ __launch_bounds__(256) void fle_symmetrize_bf16_kernel(
+    const float* __restrict__ vols, long long ld_vol, int B, const int* __restrict__ fcoord, int V0, int N,
+    int Kpad, int Mpad, const float* __restrict__ shift_zyx, uint16_t* __restrict__ Xhi,
+    uint16_t* __restrict__ Xlo) {
+  const int nq = (V0 + 3) >> 2;
+  const long long tot = (long long)B * nq;
+  const int c = N >> 1;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < tot;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int b = (int)(t / nq), qd = (int)(t - (long long)b * nq);
+    const float* f = vols + (long long)b * ld_vol;
+    // the patch vol[a + s] at the particle's shift (SHIFTED; values as
+    // extract_shifted_patches copies them out)
+    ShiftSpec sp;
+    make_shift_spec(SHIFTED ? shift_zyx : nullptr, b, N, sp);
+    int pcs[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int v = 4 * qd + u;
+      pcs[u] = fcoord[v < V0 ? v : 0];
+    }
+    // an integral shift under which every image of the thread's voxels stays in
+    // the volume reads them directly at an offset; otherwise through the buffer
+    // range check (zero outside), or trilinear for a fractional shift
+    int shoff = 0;
+    bool direct = !SHIFTED;
+    if (SHIFTED && sp.integral) {
+      int m = 0, mz = 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        m = max(m, max(pcs[u] & 1023, (pcs[u] >> 10) & 1023));
+        mz = max(mz, pcs[u] >> 20);
+      }
+      direct = c - m + sp.si[0] >= 0 && c + m + sp.si[0] < N && c - m + sp.si[1] >= 0 && c + m + sp.si[1] < N &&
+               c - mz + sp.si[2] >= 0 && c + mz + sp.si[2] < N;
+      shoff = (sp.si[0] * N + sp.si[1]) * N + sp.si[2];
+    }
+    const __amdgpu_buffer_rsrc_t rv = make_rsrc(f, (unsigned)(N * N * N) * 4u);
+    float g[4][16];
+    auto fill = [&](auto&& value_at) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const bool ok = 4 * qd + u < V0;
+        const int x = pcs[u] & 1023, y = (pcs[u] >> 10) & 1023, z = pcs[u] >> 20;
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const int zi = q ? c - z : c + z;
+            const int y1 = tt ? -y : y;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+              const int a0 = (p == 0 ? x : (p == 1 ? -y1 : (p == 2 ? -x : y1))) + c;
+              const int a1 = (p == 0 ? y1 : (p == 1 ? x : (p == 2 ? -y1 : -x))) + c;
+              const float val = value_at(a0, a1, zi);
+              g[u][p + 4 * q + 8 * tt] = ok ? val : 0.f;
+            }
+          }
+      }
+    };
+    if (direct) {
+      fill([&](int a0, int a1, int a2) { return f[(a0 * N + a1) * N + a2 + shoff]; });
+    } else if (sp.integral) {
+      fill([&](int a0, int a1, int a2) {
+        const int i0 = a0 + sp.si[0], i1 = a1 + sp.si[1], i2 = a2 + sp.si[2];
+        const bool in = (unsigned)i0 < (unsigned)N && (unsigned)i1 < (unsigned)N && (unsigned)i2 < (unsigned)N;
+        return load_f1(rv, in ? (unsigned)((i0 * N + i1) * N + i2) * 4u : kOutOfRange);
+      });
+    } else {
+      fill([&](int a0, int a1, int a2) { return patch_value(f, N, sp, a0, a1, a2); });
+    }
+    // class (eps, rho): G_t = sum_q (-1)^{q eps} sum_p (-i)^{rho p} f[p, q, t]
+#pragma unroll
+    for (int eps = 0; eps < 2; ++eps)
+#pragma unroll
+      for (int rho = 0; rho < 4; ++rho) {
+        unsigned hr[4], lr[4], hm[4], lm[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          float A[2], Bv[2];
+#pragma unroll
+          for (int tt = 0; tt < 2; ++tt) {
+            float a = 0.f, bi = 0.f;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+              const float sg = (eps && q) ? -1.f : 1.f;
+              const float* h = &g[u][4 * q + 8 * tt];
+              if (rho == 0) {
+                a += sg * (h[0] + h[1] + h[2] + h[3]);
+              } else if (rho == 2) {
+                a += sg * (h[0] - h[1] + h[2] - h[3]);
+              } else {
+                a += sg * (h[0] - h[2]);
+                bi += sg * (h[3] - h[1]);
+              }
+            }
+            A[tt] = a;
+            Bv[tt] = rho == 3 ? -bi : bi;  // rho = 3: conjugate of rho = 1
+          }
+          uint16_t h1, l1, h2, l2, h3, l3, h4, l4;
+          Split16<false>::split(A[0] + A[1], h1, l1);
+          Split16<false>::split(Bv[0] - Bv[1], h2, l2);
+          Split16<false>::split(Bv[0] + Bv[1], h3, l3);
+          Split16<false>::split(A[1] - A[0], h4, l4);
+          hr[u] = (unsigned)h1 | ((unsigned)h2 << 16);
+          lr[u] = (unsigned)l1 | ((unsigned)l2 << 16);
+          hm[u] = (unsigned)h3 | ((unsigned)h4 << 16);
+          lm[u] = (unsigned)l3 | ((unsigned)l4 << 16);
+        }
+        const int cls = eps * 4 + rho;
+        const long long orow = ((long long)cls * Mpad + b) * Kpad + 8LL * qd;
+        const long long oim = ((long long)cls * Mpad + B + b) * Kpad + 8LL * qd;
+        *reinterpret_cast<uint4*>(Xhi + orow) = make_uint4(hr[0], hr[1], hr[2], hr[3]);
+        *reinterpret_cast<uint4*>(Xlo + orow) = make_uint4(lr[0], lr[1], lr[2], lr[3]);
+        *reinterpret_cast<uint4*>(Xhi + oim) = make_uint4(hm[0], hm[1], hm[2], hm[3]);
+        *reinterpret_cast<uint4*>(Xlo + oim) = make_uint4(lm[0], lm[1], lm[2], lm[3]);
+      }
   }
 }
 
@@ -703,7 +836,7 @@ namespace {
 
 void free_fle_plan(FlePlan* p) {
   if (!p) return;
-  (void)hipFree(p->orbit); (void)hipFree(p->W); (void)hipFree(p->chan_plus);
+  (void)hipFree(p->orbit); (void)hipFree(p->fcoord); (void)hipFree(p->W); (void)hipFree(p->chan_plus);
   (void)hipFree(p->chan_minus); (void)hipFree(p->chan_m); (void)hipFree(p->X); (void)hipFree(p->part);
   (void)hipFree(p->Wt16); (void)hipFree(p->colscale); (void)hipFree(p->rowmax);
   (void)hipFree(p->part16); (void)hipFree(p->Xhi); (void)hipFree(p->Xlo);
@@ -850,6 +983,23 @@ int mch_fle_plan_create(int N, int V0, int Kpad, int Ntot, const int* class_colp
 #define MCH_TRY(expr, what) do { if ((rc = mch::hip_check((expr), what)) != MCH_OK) goto fail; } while (0)
   MCH_TRY(hipMalloc(&p->orbit, (size_t)V0 * 16 * sizeof(int)), "hipMalloc(orbit)");
   MCH_TRY(hipMemcpy(p->orbit, orbit, (size_t)V0 * 16 * sizeof(int), hipMemcpyHostToDevice), "copy orbit");
+  {
+    // the identity image (first orbit entry) of each fundamental voxel, as centred coordinates
+    std::vector<int> fc(V0);
+    const int c = N / 2;
+    for (int v = 0; v < V0; ++v) {
+      const int id = orbit[(size_t)v * 16];
+      const int x = id / (N * N) - c, y = (id / N) % N - c, z = id % N - c;
+      if (x < 0 || y < 0 || z < 0 || x > 1023 || y > 1023 || z > 1023) {
+        mch::set_error("orbit table: first image of a fundamental voxel must have x, y, z >= 0");
+        rc = MCH_ERR_ARG;
+        goto fail;
+      }
+      fc[v] = x | (y << 10) | (z << 20);
+    }
+    MCH_TRY(hipMalloc(&p->fcoord, (size_t)V0 * sizeof(int)), "hipMalloc(fcoord)");
+    MCH_TRY(hipMemcpy(p->fcoord, fc.data(), (size_t)V0 * sizeof(int), hipMemcpyHostToDevice), "copy fcoord");
+  }
   MCH_TRY(hipMalloc(&p->W, (size_t)Kpad * Ntot * sizeof(float)), "hipMalloc(W)");
   MCH_TRY(hipMemcpy(p->W, basis, (size_t)Kpad * Ntot * sizeof(float), hipMemcpyHostToDevice), "copy W");
   MCH_TRY(hipMalloc(&p->chan_plus, (size_t)Ntot * sizeof(int)), "hipMalloc(chan)");
@@ -1013,8 +1163,18 @@ int mch_fle_tuned_depth(void* plan) {
 }
 
 // a (B, ne) complex64 rows of stride lda = B^H f for volumes (B, N^3) float32.
+int mch_fle_project_shifted(void* plan, const float* vols, long long ld_vol, int B, const float* shift_zyx,
+                            void* coeffs, long long lda, hipStream_t stream);
+
 int mch_fle_project(void* plan, const float* vols, long long ld_vol, int B, void* coeffs,
                     long long lda, hipStream_t stream) {
+  return mch_fle_project_shifted(plan, vols, ld_vol, B, nullptr, coeffs, lda, stream);
+}
+
+// The same for the patches vol[a + flip(shift_zyx[b])] (extract_shifted_patches
+// semantics), read straight from the volumes: no patch copy.
+int mch_fle_project_shifted(void* plan, const float* vols, long long ld_vol, int B, const float* shift_zyx,
+                            void* coeffs, long long lda, hipStream_t stream) {
   FlePlan* p = (FlePlan*)plan;
   if (!p || !vols || !coeffs) { mch::set_error("mch_fle_project: NULL argument"); return MCH_ERR_ARG; }
   if (B < 1 || B > p->max_batch || ld_vol < (long long)p->N * p->N * p->N || lda < p->ne) {
@@ -1033,12 +1193,17 @@ int mch_fle_project(void* plan, const float* vols, long long ld_vol, int B, void
     if ((rc = mch::hip_check(hipGetLastError(), "fle_fill_kernel launch"))) return rc;
   }
   if (p->precision == kBf16x3) {
-    hipLaunchKernelGGL(fle_symmetrize_kernel<true>, dim3(nb), dim3(256), 0, stream, vols, ld_vol, B,
-                       p->orbit, p->V0, p->Kpad, p->Mpad, p->X, nullptr, p->Xhi, p->Xlo);
+    const long long nq = (long long)B * ((p->V0 + 3) / 4);
+    const int nb4 = (int)std::min<long long>((nq + 255) / 256, 16384);
+    if (shift_zyx)
+      hipLaunchKernelGGL(fle_symmetrize_bf16_kernel<true>, dim3(nb4), dim3(256), 0, stream, vols, ld_vol, B,
+                         p->fcoord, p->V0, p->N, p->Kpad, p->Mpad, shift_zyx, p->Xhi, p->Xlo);
+    else
+      hipLaunchKernelGGL(fle_symmetrize_bf16_kernel<false>, dim3(nb4), dim3(256), 0, stream, vols, ld_vol, B,
+                         p->fcoord, p->V0, p->N, p->Kpad, p->Mpad, nullptr, p->Xhi, p->Xlo);
   } else {
-    hipLaunchKernelGGL(fle_symmetrize_kernel<false>, dim3(nb), dim3(256), 0, stream, vols, ld_vol, B,
-                       p->orbit, p->V0, p->Kpad, p->Mpad, p->X, fp16 ? p->rowmax : nullptr, nullptr,
-                       nullptr);
+    hipLaunchKernelGGL(fle_symmetrize_kernel, dim3(nb), dim3(256), 0, stream, vols, ld_vol, B, p->orbit, p->V0,
+                       p->N, p->Kpad, p->Mpad, shift_zyx, p->X, fp16 ? p->rowmax : nullptr);
   }
   rc = mch::hip_check(hipGetLastError(), "fle_symmetrize_kernel launch");
   if (rc) return rc;

@@ -72,60 +72,13 @@ struct ShiftPlan {
   hipEvent_t ev[5];
 };
 
-// Raw buffer resource over nbytes at p: loads at byte offsets >= nbytes return 0
-// (hardware range check), so out-of-volume samples need no branch.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned nbytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)nbytes, 0x00020000);
-}
-constexpr unsigned kOutOfRange = 0x80000000u;
-__device__ __forceinline__ float4 load_f4(__amdgpu_buffer_rsrc_t r, unsigned off) {
-  typedef unsigned v4u __attribute__((ext_vector_type(4)));
-  const v4u v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
-  return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
-}
-__device__ __forceinline__ float load_f1(__amdgpu_buffer_rsrc_t r, unsigned off) {
-  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-}
-
-// Shifted patch value at array index (a0, a1, a2): vol[a + s] with trilinear
-// sampling for fractional s and the border slice zeroed as the reference does.
-struct ShiftSpec {
-  float s[3];
-  int integral;       // all components integral
-  int zero_idx[3];    // index of the slice to zero along each axis (-1: none)
-  int si[3];
-};
-
-__device__ __forceinline__ float patch_value(const float* __restrict__ v, int N, const ShiftSpec& sp,
-                                             int a0, int a1, int a2) {
-  if ((sp.zero_idx[0] >= 0 && a0 == sp.zero_idx[0]) || (sp.zero_idx[1] >= 0 && a1 == sp.zero_idx[1]) ||
-      (sp.zero_idx[2] >= 0 && a2 == sp.zero_idx[2]))
-    return 0.f;
-  if (sp.integral) {
-    const int z = a0 + sp.si[0], y = a1 + sp.si[1], x = a2 + sp.si[2];
-    if (z < 0 || z >= N || y < 0 || y >= N || x < 0 || x >= N) return 0.f;
-    return v[((long long)z * N + y) * N + x];
-  }
-  return mch::tri_sample_real(v, N, a0 + sp.s[0], a1 + sp.s[1], a2 + sp.s[2]);
-}
-
-__device__ void make_shift_spec(const float* __restrict__ shift_zyx, int b, int N, ShiftSpec& sp) {
-  // the patch shift in array-axis order is the flipped (z, y, x) shift
-  sp.integral = 1;
-  for (int d = 0; d < 3; ++d) {
-    const float s = shift_zyx ? shift_zyx[b * 3 + (2 - d)] : 0.f;
-    sp.s[d] = s;
-    const float a = fabsf(s);
-    const float fl = floorf(a);
-    sp.si[d] = (int)rintf(s);
-    sp.zero_idx[d] = -1;
-    if (a - fl > 0.f) {
-      sp.integral = 0;
-      // idx = floor(|s|) if s < 0 else -ceil(|s|) (python negative index)
-      sp.zero_idx[d] = (s < 0.f) ? (int)fl : N - (int)ceilf(a);
-    }
-  }
-}
+using mch::kOutOfRange;
+using mch::load_f1;
+using mch::load_f4;
+using mch::make_rsrc;
+using mch::make_shift_spec;
+using mch::patch_value;
+using mch::ShiftSpec;
 
 // Background weight of the cosine mask at squared distance r2 from the centre:
 // 0 for r < radius (the patch is kept), 1 for r > radius_p (replaced by the
@@ -953,27 +906,37 @@ __global__ __launch_bounds__(256) void extract_patch_kernel(const float* __restr
   if (sp.integral && (N & 3) == 0 && (reinterpret_cast<uintptr_t>(o) & 15) == 0) {
     // integral shift: a shifted copy, four outputs per thread and store;
     // (y, q) advanced incrementally (no division in the loop)
+    // out-of-volume voxels load through an out-of-range buffer offset, which
+    // reads zero without a branch, so the loads of several iterations overlap
     const int nq = N >> 2;
     const int zs = z + sp.si[0];
     const bool zok = zs >= 0 && zs < N;
-    const float* src = v + (long long)(zok ? zs : 0) * N * N;
+    const __amdgpu_buffer_rsrc_t rv = make_rsrc(v, (unsigned)(N * N * N) * 4u);
     const int dy = blockDim.x / nq, dq = blockDim.x - dy * nq;
     int y = threadIdx.x / nq, q = threadIdx.x - (threadIdx.x / nq) * nq;
     float4* o4 = reinterpret_cast<float4*>(o);
-#pragma unroll 4
-    for (int t = threadIdx.x; t < N * nq; t += blockDim.x) {
-      const int ys = y + sp.si[1], x0 = 4 * q + sp.si[2];
-      const bool rok = zok && ys >= 0 && ys < N;
-      const float* row = src + (long long)(rok ? ys : 0) * N;
-      float4 r;
-      r.x = rok && x0 >= 0 && x0 < N ? row[x0] : 0.f;
-      r.y = rok && x0 + 1 >= 0 && x0 + 1 < N ? row[x0 + 1] : 0.f;
-      r.z = rok && x0 + 2 >= 0 && x0 + 2 < N ? row[x0 + 2] : 0.f;
-      r.w = rok && x0 + 3 >= 0 && x0 + 3 < N ? row[x0 + 3] : 0.f;
-      o4[t] = r;
-      q += dq;
-      y += dy;
-      if (q >= nq) { q -= nq; ++y; }
+    const int n4 = N * nq;
+    constexpr int U = 4;  // iterations whose loads are all issued before their stores
+    for (int t0 = threadIdx.x; t0 < n4; t0 += U * (int)blockDim.x) {
+      float4 r[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int ys = y + sp.si[1], x0 = 4 * q + sp.si[2];
+        const bool rok = zok && ys >= 0 && ys < N && t0 + u * (int)blockDim.x < n4;
+        const int rowoff = (zs * N + ys) * N;
+        r[u].x = load_f1(rv, rok && x0 >= 0 && x0 < N ? (unsigned)(rowoff + x0) * 4u : kOutOfRange);
+        r[u].y = load_f1(rv, rok && x0 + 1 >= 0 && x0 + 1 < N ? (unsigned)(rowoff + x0 + 1) * 4u : kOutOfRange);
+        r[u].z = load_f1(rv, rok && x0 + 2 >= 0 && x0 + 2 < N ? (unsigned)(rowoff + x0 + 2) * 4u : kOutOfRange);
+        r[u].w = load_f1(rv, rok && x0 + 3 >= 0 && x0 + 3 < N ? (unsigned)(rowoff + x0 + 3) * 4u : kOutOfRange);
+        q += dq;
+        y += dy;
+        if (q >= nq) { q -= nq; ++y; }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int t = t0 + u * (int)blockDim.x;
+        if (t < n4) o4[t] = r[u];
+      }
     }
     return;
   }

@@ -167,3 +167,27 @@ def test_projection_split_precision_full_size(gpu_device, mode):
     f.set_compute_precision("fp32")
     again = f.evaluate_t(torch.tensor(vols, device=gpu_device)).cpu().numpy()[0]
     assert np.array_equal(again, plain)
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16x3", "fp16x3"])
+def test_projection_at_shift_equals_projection_of_extracted_patch(gpu_device, mode):
+    # the shifted projection reads vol[a + flip(shift)] straight from the volumes
+    # (integral: buffer reads with zeros outside; fractional: trilinear with the
+    # border slice zeroed) -- the values extract_shifted_patches copies out, so
+    # the coefficients are bit-identical
+    from matcha_amd.core.FLEBasis3D import FLEBasis3D
+    from matcha_amd.core.ShiftMatcher import extract_shifted_patches
+    N, bl, radius, B = 32, 12, 15, 6
+    rng = np.random.default_rng(77)
+    vols = torch.tensor(rng.standard_normal((B, N, N, N)).astype(np.float32), device=gpu_device)
+    shifts = np.array([[0, 0, 0], [2, -1, 3], [-6, 5, 0], [N, 0, -1], [0.5, -1.25, 2.0], [1.75, 0, -0.5]],
+                      dtype=np.float32)
+    sh = torch.tensor(shifts, device=gpu_device)
+    f = FLEBasis3D(N, bl, 1e-4, batchsize=B, radius=radius, device=gpu_device, compute_precision=mode)
+    direct = f.evaluate_t(vols, shift_zyx=sh).cpu().numpy()
+    patches = extract_shifted_patches(vols, sh)
+    via_patch = f.evaluate_t(patches).cpu().numpy()
+    assert np.array_equal(direct, via_patch)
+    # and no shift at all is the plain projection
+    assert np.array_equal(f.evaluate_t(vols, shift_zyx=torch.zeros_like(sh)).cpu().numpy(),
+                          f.evaluate_t(vols).cpu().numpy())
