@@ -41,13 +41,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 
 # HBM bytes the tuned projection GEMM fetches per launch, measured with
 # rocprofv3 --pmc FETCH_SIZE and scaled by 2.0 (FETCH_SIZE of a device copy of
-# 512 MiB reads 256 MiB); keyed by (workload, batch).  Its writes are the K-split
+# 512 MiB reads 256 MiB); keyed by (workload, batch, waves per workgroup of the
+# tile the autotuner chose: 8 is the 256-row tile).  Its writes are the K-split
 # partials, splits x rows x Ntot x 4 B (WRITE_SIZE reads exactly that), added
 # per run for the split the autotuner chose.
-GEMM16_FETCH_BYTES = {("box96_L32", 64): 584368 * 1024 * 2, ("box96_L32", 512): 3048163 * 1024 * 2}
+GEMM16_FETCH_BYTES = {("box96_L32", 64, 4): 587089 * 1024 * 2, ("box96_L32", 512, 8): 1720199 * 1024 * 2}
 # HBM bytes (FETCH_SIZE x 2.0 + WRITE_SIZE) per launch of the shift search's
 # column pass, keyed by (workload, batch); measured like the GEMM's.
-SHIFT_Z_TRAFFIC_BYTES = {}
+SHIFT_Z_TRAFFIC_BYTES = {("box96_L32", 512): 3553865 * 1024 * 2 + 2102084 * 1024}
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
@@ -507,7 +508,7 @@ def main():
     flops16 = 3 * 2.0 * (2 * Bt) * (2 * fle_.V0) * sum(fle_.class_cols)
     fle_.set_compute_precision(args.tuned_precision)
     tiles = int(_lib_depth(fle_))
-    fetch = GEMM16_FETCH_BYTES.get((args.workload, Bt))
+    fetch = GEMM16_FETCH_BYTES.get((args.workload, Bt, tiles % 100 // 10))
     rows = -(-2 * Bt // 128) * 128
     traffic = fetch + (tiles // 100) * rows * fle_.Ntot * 4.0 if fetch is not None else None
     gemm_hbm = achieved / 8000.0
