@@ -539,21 +539,12 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
     const float fm2 = fm * fm, fmp2 = fmp * fmp, mmp = fm * fmp;
     const unsigned voff = 16u * (unsigned)pc;
     typedef unsigned v4u __attribute__((ext_vector_type(4)));
-    // sigma of shells l and l + 1 in flight while shell l's recurrence runs
-    // (the offsets are wave-uniform; reads past the particle's sigma return 0)
-    unsigned soff = 4u * (unsigned)mch::pk_offset(lstart);
-    v4u q0 = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
-    soff += 16u * (unsigned)((lstart + 1) * (lstart + 1));
-    v4u q1 = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
-    soff += 16u * (unsigned)((lstart + 2) * (lstart + 2));
-    for (int l = lstart; l <= Li; ++l) {
-      const v4u q = q0;
-      q0 = q1;
-      if (l + 2 <= Li) {
-        q1 = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
-        soff += 16u * (unsigned)((l + 3) * (l + 3));
-      }
-      if (l > mp) {
+    // shell l: the recurrence step to d^l and the accumulation of sigma^l; the
+    // guarded form for shells that some lanes of the wave have not reached
+    // (l <= mp), the plain form once every lane has (invalid tail lanes then
+    // compute values that are never used)
+    auto shell = [&](int l, const v4u q, bool guarded) {
+      if (!guarded || l > mp) {
         const float4 c = lrec[l];
         const float Pl = (c.x - fm2) * (c.x - fmp2);  // >= 1 for l > mp >= |m|
         const float r = __builtin_amdgcn_rsqf(Pl);
@@ -566,7 +557,7 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
         d = nd; dp = ndp; dpp = ndpp;
         Cprev = Pl * r * c.w;
       }
-      if (valid && l >= mp) {
+      if (!guarded || (valid && l >= mp)) {
         const float2 sv[2] = {make_float2(__uint_as_float(q.x), __uint_as_float(q.y)),
                               make_float2(__uint_as_float(q.z), __uint_as_float(q.w))};
 #pragma unroll
@@ -576,6 +567,42 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
           U2[i].x += sv[i].x * dpp; U2[i].y += sv[i].y * dpp;
         }
       }
+    };
+    // the largest start degree among the wave's valid lanes (wave-uniform)
+    int mpv = valid ? mp : lstart;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mpv = max(mpv, __shfl_xor(mpv, off, 64));
+    const int l1 = min(__builtin_amdgcn_readfirstlane(mpv), Li);
+    // sigma of shells l + 1 and l + 2 in flight while shell l runs (the offsets
+    // are wave-uniform; reads past the particle's sigma return 0)
+    unsigned soff = 4u * (unsigned)mch::pk_offset(lstart);
+    v4u q0 = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
+    soff += 16u * (unsigned)((lstart + 1) * (lstart + 1));
+    v4u q1 = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
+    soff += 16u * (unsigned)((lstart + 2) * (lstart + 2));
+    for (int l = lstart; l <= l1; ++l) {
+      const v4u q = q0;
+      q0 = q1;
+      if (l + 2 <= Li) {
+        q1 = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
+        soff += 16u * (unsigned)((l + 3) * (l + 3));
+      }
+      shell(l, q, true);
+    }
+    // every lane active: two shells per iteration, each shell's sigma loaded
+    // two shells ahead into one of two registers, unconditionally (loads past
+    // the particle's sigma return 0 and are not used), so that neither a
+    // register rotation nor a wait on the newest load is needed; soff is the
+    // offset of shell l1 + 3
+    v4u qa = q0, qb = q1;
+    for (int l = l1 + 1; l <= Li; l += 2) {
+      shell(l, qa, false);
+      qa = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
+      soff += 16u * (unsigned)((l + 3) * (l + 3));
+      if (l + 1 > Li) break;
+      shell(l + 1, qb, false);
+      qb = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
+      soff += 16u * (unsigned)((l + 4) * (l + 4));
     }
     if (valid) {
       const int us[2] = {mp, m};
