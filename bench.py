@@ -1,11 +1,13 @@
 """Benchmark: aligned volumes/s for Matcha's orientation-search hot path on one MI355X.
 
 Headline (the metric's own definition): the tuned path -- batched (512
-particles per batch) with the projection GEMM on 16-bit hi/lo split operands
-(three MFMA products, FP32 accumulation) -- against the plain FP32 path at
-batch 64, both measured in this run on the same synthetic particles, each
-required to reach <= 1 deg median geodesic error.  ``value`` is the tuned
-path's throughput; the plain path's throughput and the ratio are in ``config``.
+particles per batch, consecutive batches alternating over two streams so that
+independent batches overlap on the device) with the projection GEMM on 16-bit
+hi/lo split operands (three MFMA products, FP32 accumulation) -- against the
+plain FP32 path at batch 64 on one stream, both measured in this run on the
+same synthetic particles, each required to reach <= 1 deg median geodesic
+error.  ``value`` is the tuned path's throughput; the plain path's throughput
+and the ratio are in ``config``.
 
 A step = one batch of particles aligned on the GPU by two rotation/shift
 alternations (matcha_amd.align.Alternation), each:
@@ -70,6 +72,8 @@ def parse():
     p.add_argument("--batch", type=int, default=None, help="plain-path batch (default: the workload's)")
     p.add_argument("--tuned-batch", type=int, default=None, help="tuned-path batch (default: the workload's)")
     p.add_argument("--tuned-precision", default="bf16x3", choices=["bf16x3", "fp16x3"])
+    p.add_argument("--tuned-streams", type=int, default=2,
+                   help="streams the tuned path's batches alternate over (one plan set each)")
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP32 path")
     p.add_argument("--snr", type=float, default=0.0)
     p.add_argument("--pool", type=int, default=512, help="distinct particles cycled through")
@@ -123,22 +127,36 @@ def cpu_baseline(w, args):
     }
 
 
-def run_path(name, torch, dev, w, args, pool_vols, pool_R, pool_t, precision, B, ccm, matcher, sm):
-    """Throughput, accuracy and per-stage times of one path."""
+def run_path(name, torch, dev, w, args, pool_vols, pool_R, pool_t, precision, B, plan_sets):
+    """Throughput, accuracy and per-stage times of one path.
+
+    plan_sets: one (correlator, matcher, shift matcher) per stream; batch k of
+    the timed steps runs on stream k mod len(plan_sets), so that independent
+    batches overlap on the device (each stream owns its plans' buffers)."""
     from matcha_amd.align import Alternation
     from matcha_amd.core.ShiftMatcher import euler_to_shift_rotations
     from matcha_amd.synthetic import euler_zyz_to_matrix, geodesic_deg
-    ccm.fle.set_compute_precision(precision)
-    pipe = Alternation(ccm, matcher, sm, num_alternations=args.alternations, mode="global")
+    for c, _, _ in plan_sets:
+        c.fle.set_compute_precision(precision)
+    pipes = [Alternation(c, m, s, num_alternations=args.alternations, mode="global") for c, m, s in plan_sets]
+    ccm, matcher, sm = plan_sets[0]
+    pipe = pipes[0]
+    nstreams = len(pipes)
+    streams = [torch.cuda.current_stream(dev)] + [torch.cuda.Stream(device=dev) for _ in range(nstreams - 1)]
     P = pool_vols.shape[0]
     nb = max(1, P // B)
     batches = [pool_vols[i * B:(i + 1) * B] for i in range(nb)]
-    for i in range(args.warmup):
-        pipe.run(batches[i % nb])
+
+    def step(k):
+        with torch.cuda.stream(streams[k % nstreams]):
+            pipes[k % nstreams].run(batches[k % nb])
+
+    for i in range(max(args.warmup, nstreams)):
+        step(i)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        pipe.run(batches[i % nb])
+        step(i)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     vps = args.steps * B / elapsed
@@ -166,7 +184,8 @@ def run_path(name, torch, dev, w, args, pool_vols, pool_R, pool_t, precision, B,
     ev[3].record(stream)
     torch.cuda.synchronize()
     return {
-        "precision": precision, "batch": B, "volumes_per_s": vps, "ms_per_step": elapsed / args.steps * 1e3,
+        "precision": precision, "batch": B, "streams": nstreams, "volumes_per_s": vps,
+        "ms_per_step": elapsed / args.steps * 1e3,
         "median_err_deg": float(np.median(errs)), "p90_err_deg": float(np.percentile(errs, 90)),
         "shift_exact_frac": float(shift_ok.mean()), "particles_checked": int(errs.size),
         "stage_ms_per_batch": {"get_sigma": ev[0].elapsed_time(ev[1]),
@@ -479,7 +498,9 @@ def main():
         sh.set_reference(wl.template.cpu().numpy())
         return c, m, sh
 
-    ccm, matcher, sm = plans(Bt, args.tuned_precision)  # the 16-bit tables are built (and tuned) here
+    # the 16-bit tables are built (and the GEMM tuned) here, one plan set per stream
+    tuned_sets = [plans(Bt, args.tuned_precision) for _ in range(max(1, args.tuned_streams))]
+    ccm, matcher, sm = tuned_sets[0]
     plain_plans = None if args.no_plain else plans(Bp, "fp32")
     P = max(args.pool, Bmax)
     chunks = [wl.particles(min(64, P - i), w["max_shift"], args.snr) for i in range(0, P, 64)]
@@ -490,10 +511,10 @@ def main():
     t_setup = time.perf_counter() - t_setup
 
     tuned = run_path("tuned", torch, dev, w, args, pool_vols, pool_R, pool_t, args.tuned_precision, Bt,
-                     ccm, matcher, sm)
+                     tuned_sets)
     plain = None
     if not args.no_plain:
-        plain = run_path("plain", torch, dev, w, args, pool_vols, pool_R, pool_t, "fp32", Bp, *plain_plans)
+        plain = run_path("plain", torch, dev, w, args, pool_vols, pool_R, pool_t, "fp32", Bp, [plain_plans])
 
     # the tuned path's projection GEMM: algorithmic bytes = W^T hi + lo (4 B per
     # element) + the hi/lo X operand (4 B per element), each read once;
@@ -547,7 +568,8 @@ def main():
                          f"{w['bandlimit']} radius {w['radius']}, march {w['Li']}->{w['L_max']}, "
                          f"grid {2 * min(w['Li']) * w['K']}^3, {w['C']} candidates, {w['steps']} Newton "
                          f"steps, SNR {args.snr} dB, {P} particles cycled"),
-            "path": f"{args.alternations} x (get_sigma + search_orientations + search_shifts)",
+            "path": (f"{args.alternations} x (get_sigma + search_orientations + search_shifts); tuned batches "
+                     f"over {max(1, args.tuned_streams)} streams"),
             "tuned": tuned,
             "plain": plain,
             "tuned_over_plain": (tuned["volumes_per_s"] / plain["volumes_per_s"]) if plain else None,
