@@ -43,11 +43,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 
 # HBM bytes the tuned projection GEMM fetches per launch, measured with
 # rocprofv3 --pmc FETCH_SIZE and scaled by 2.0 (FETCH_SIZE of a device copy of
-# 512 MiB reads 256 MiB); keyed by (workload, batch, waves per workgroup of the
-# tile the autotuner chose: 8 is the 256-row tile).  Its writes are the K-split
+# 512 MiB reads 256 MiB); keyed by (workload, batch, waves per workgroup and
+# 128-column tiles per workgroup of the tile the autotuner chose: 8 waves is a
+# 256-row tile).  Its writes are the K-split
 # partials, splits x rows x Ntot x 4 B (WRITE_SIZE reads exactly that), added
 # per run for the split the autotuner chose.
-GEMM16_FETCH_BYTES = {("box96_L32", 64, 4): 587089 * 1024 * 2, ("box96_L32", 512, 8): 1720199 * 1024 * 2}
+GEMM16_FETCH_BYTES = {("box96_L32", 64, 4, 1): 587089 * 1024 * 2, ("box96_L32", 512, 8, 1): 1720199 * 1024 * 2,
+                      ("box96_L32", 512, 8, 2): 1434256 * 1024 * 2}
 # HBM bytes (FETCH_SIZE x 2.0 + WRITE_SIZE) per launch of the shift search's
 # column pass, keyed by (workload, batch); measured like the GEMM's.
 SHIFT_Z_TRAFFIC_BYTES = {("box96_L32", 512): 3553865 * 1024 * 2 + 2102084 * 1024}
@@ -528,10 +530,13 @@ def main():
     achieved = alg_bytes / gemm_t / 1e9
     flops16 = 3 * 2.0 * (2 * Bt) * (2 * fle_.V0) * sum(fle_.class_cols)
     fle_.set_compute_precision(args.tuned_precision)
-    tiles = int(_lib_depth(fle_))
-    fetch = GEMM16_FETCH_BYTES.get((args.workload, Bt, tiles % 100 // 10))
+    depth = int(_lib_depth(fle_))
+    tiles = {"k_splits": depth % 10000 // 100, "waves_per_workgroup": depth % 100 // 10,
+             "lds_stages": depth % 10, "column_tiles_per_workgroup": depth // 10000 + 1}
+    fetch = GEMM16_FETCH_BYTES.get((args.workload, Bt, tiles["waves_per_workgroup"],
+                                    tiles["column_tiles_per_workgroup"]))
     rows = -(-2 * Bt // 128) * 128
-    traffic = fetch + (tiles // 100) * rows * fle_.Ntot * 4.0 if fetch is not None else None
+    traffic = fetch + tiles["k_splits"] * rows * fle_.Ntot * 4.0 if fetch is not None else None
     gemm_hbm = achieved / 8000.0
     gemm_mfma = flops16 / gemm_t / 1e12 / 2500.0
 
@@ -575,8 +580,7 @@ def main():
             "tuned_over_plain": (tuned["volumes_per_s"] / plain["volumes_per_s"]) if plain else None,
             "meets_accuracy": tuned["median_err_deg"] <= 1.0 and (plain is None or plain["median_err_deg"] <= 1.0),
             "gemm_ms": {"tuned": gemm_t * 1e3, "plain_fp32": gemm_plain * 1e3 if gemm_plain else None},
-            "gemm_tiles": {"k_splits": tiles // 100, "waves_per_workgroup": tiles % 100 // 10,
-                           "lds_stages": tiles % 10},
+            "gemm_tiles": tiles,
             "setup_s": t_setup,
         },
         "roofline": {"bound": "hbm", "achieved": z_gbs, "peak": 8000.0, "unit": "GB/s",

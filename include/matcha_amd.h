@@ -126,10 +126,17 @@ int mch_fle_gemm_only(void* plan, int B, hipStream_t stream);
 int mch_fle_set_precision(void* plan, int precision);
 
 /* Tile configuration the 16-bit GEMM was autotuned to when the plan first
- * switched to a 16-bit mode: 100 * (K splits) + 10 * (waves per workgroup,
- * 2 or 4) + pipeline depth (2 or 3 LDS stages); 0 before that.  No reference
- * counterpart (reporting aid). */
+ * switched to a 16-bit mode: 10000 * (128-column tiles per workgroup - 1) +
+ * 100 * (K splits) + 10 * (waves per workgroup, 2, 4 or 8) + pipeline depth
+ * (2 or 3 LDS stages); 0 before that.  No reference counterpart (reporting
+ * aid). */
 int mch_fle_tuned_depth(void* plan);
+
+/* Pin the 16-bit GEMM to one of the autotuner's tile configurations
+ * (stages, waves per workgroup, 128-column tiles per workgroup, K splits);
+ * errors for a configuration the plan cannot run.  No reference counterpart
+ * (tuning and testing aid). */
+int mch_fle_set_gemm_tiles(void* plan, int stages, int waves, int column_tiles, int k_splits);
 
 /* Index table (lmax+1, 2*lmax+1, kdim) of coefficient positions for
  * (l, m, k) (-1 = absent).  Replaces CrossCorrelationMatcher._precompute_indices

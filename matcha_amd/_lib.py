@@ -46,6 +46,7 @@ _SIGNATURES = [
     ("mch_fle_gemm_only", c_int, [c_void_p, c_int, c_void_p]),
     ("mch_fle_set_precision", c_int, [c_void_p, c_int]),
     ("mch_fle_tuned_depth", c_int, [c_void_p]),
+    ("mch_fle_set_gemm_tiles", c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     ("mch_corr_plan_create", c_int, [c_int, c_int, c_void_p, c_int, ctypes.POINTER(c_void_p)]),
     ("mch_corr_plan_destroy", c_int, [c_void_p]),
     ("mch_compute_sigma", c_int, [c_void_p, c_void_p, c_void_p, c_ll, c_int, c_int, c_void_p, c_ll,

@@ -277,6 +277,18 @@ class FLEBasis3D:
         self.compute_precision = "fp32"
         self.set_compute_precision(compute_precision)
 
+    def set_gemm_tiles(self, stages: int, waves: int, column_tiles: int, k_splits: int):
+        """Pin the 16-bit GEMM to one autotune candidate (tuning/testing aid;
+        the plan must have switched to a 16-bit mode)."""
+        _lib.check(_lib.lib().mch_fle_set_gemm_tiles(self._plan, int(stages), int(waves), int(column_tiles),
+                                                     int(k_splits)), "mch_fle_set_gemm_tiles")
+
+    def gemm_tiles(self) -> dict:
+        """The 16-bit GEMM's tile configuration (autotuned or pinned)."""
+        d = int(_lib.lib().mch_fle_tuned_depth(self._plan))
+        return {"column_tiles": d // 10000 + 1, "k_splits": d % 10000 // 100, "waves": d % 100 // 10,
+                "stages": d % 10}
+
     def set_compute_precision(self, mode: str):
         """Switch the projection GEMM between "fp32", "bf16x3" and "fp16x3"."""
         if mode not in COMPUTE_PRECISIONS:

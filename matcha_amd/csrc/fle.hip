@@ -62,7 +62,8 @@ struct FlePlan {
   uint16_t* Xhi;              // (kClasses, Mpad, Kpad) 16-bit split X operand
   uint16_t* Xlo;              // (kClasses, Mpad, Kpad)
   int nst16;                  // pipeline depth of the 16-bit GEMM (autotuned: 2 or 3)
-  int nw16;                   // waves per workgroup of the 16-bit GEMM (autotuned: 2 or 4)
+  int nw16;                   // waves per workgroup of the 16-bit GEMM (autotuned: 2, 4 or 8)
+  int ct16;                   // 128-column tiles per workgroup (autotuned: 1, or 2 with 8 waves)
   uint16_t* Wt16;             // W^T hi/lo in DMA order, Ntot * Kpad * 2 (see fle_wsplit_kernel)
   float* colscale;            // (Ntot)
   float* rowmax;              // (Mpad) max |X| of each GEMM row (FP16 scaling)
@@ -616,6 +617,121 @@ __global__ __launch_bounds__(64 * NW) void fle_gemm16_kernel(
       }
 }
 
+// Wide variant: 256 rows x 256 columns (two 128-column tiles of one class) per
+// workgroup of 8 waves, two stages of 64 KB.  X is re-read from L2 once per
+// 256 columns instead of once per 128 (the operand traffic into LDS, not the
+// MFMAs, bounds the 256 x 128 tile at large batches).  Waves: 4 row quarters
+// x 2 column halves, 64 x 128 per wave.  The grid enumerates column units
+// (pairs of tiles within a class, and the single last tile of a class with an
+// odd count), row tiles (fastest) and K splits; a single-tile unit reads its
+// tile twice and stores the first half only.
+constexpr int kWideStageU4 = 4096;  // A hi 1024, A lo 1024, W 2 tiles x 1024
+
+template <bool FP16>
+__global__ __launch_bounds__(512) void fle_gemm16w_kernel(
+    const uint16_t* __restrict__ Xhi, const uint16_t* __restrict__ Xlo, const u32x4* __restrict__ Wt16,
+    float* __restrict__ part, int Mpad, int Kpad, int Ntot, int kchunk, int nunits, int nmt, FlePlan cols) {
+  constexpr int NST = 2, NW = 8, AREG = 1024, W0 = 2048, AP = 4, WP = 4, RB = 4, CB = 8;
+  extern __shared__ u32x4 lds[];   // NST x kWideStageU4
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lb = mch::xcd_remap(blockIdx.x, gridDim.x);
+  const int mt = lb % nmt, rest = lb / nmt;
+  const int un = rest % nunits, ks = rest / nunits;
+  // column unit un -> class, first column and tile count
+  int cls = 0, ubase = 0;
+  for (;;) {
+    const int nt = (cols.col_off[cls + 1] - cols.col_off[cls]) / BN16;
+    const int nu = (nt + 1) >> 1;
+    if (un < ubase + nu || cls == kClasses - 1) break;
+    ubase += nu;
+    ++cls;
+  }
+  const int col0 = cols.col_off[cls] + (un - ubase) * 2 * BN16;
+  const int ntile = min(2, (cols.col_off[cls + 1] - col0) / BN16);
+  const int tn0 = col0 / BN16, tn1 = ntile == 2 ? tn0 + 1 : tn0;
+  const int row0 = mt * 2 * BM16;
+  const int kbeg = ks * kchunk;
+  const int kend = min(kbeg + kchunk, Kpad);
+  const int nstage = kend > kbeg ? (kend - kbeg) / BK16 : 0;
+
+  const long long xbase = ((long long)cls * Mpad + row0) * Kpad + (lane >> 4) * 8;
+  const uint16_t* asrc[AP];
+  int adst[AP];
+#pragma unroll
+  for (int t = 0; t < AP; ++t) {
+    const int part = wave * AP + t;            // row block part / 2, hi/lo part % 2
+    const int rb = part >> 1, hl = part & 1;
+    asrc[t] = (hl ? Xlo : Xhi) + xbase + (long long)(rb * 16 + (lane & 15)) * Kpad;
+    adst[t] = hl * AREG + rb * 64;
+  }
+  const int nK = Kpad / BK16;
+  // W^T parts P = 4 wave .. 4 wave + 3 of the unit's 32: tile P / 16, part P % 16
+  const u32x4* wsrc[WP];
+#pragma unroll
+  for (int q = 0; q < WP; ++q) {
+    const int P = wave * WP + q;
+    const int tn = (P >> 4) ? tn1 : tn0;
+    wsrc[q] = Wt16 + ((long long)tn * nK + kbeg / BK16) * 1024 + (P & 15) * 64 + lane;
+  }
+  auto issue = [&](int stage) {
+    u32x4* base = lds + (stage % NST) * kWideStageU4;
+    const int k0 = kbeg + stage * BK16;
+#pragma unroll
+    for (int t = 0; t < AP; ++t) lds_dma16(asrc[t] + k0, base + adst[t]);
+#pragma unroll
+    for (int q = 0; q < WP; ++q) lds_dma16<0>(wsrc[q] + (long long)stage * 1024, base + W0 + (wave * WP + q) * 64);
+  };
+  const int rb0 = RB * (wave & 3);   // first row block of this wave
+  const int chalf = wave >> 2;       // 128-column half of the unit
+  floatx4 acc[RB][CB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int c = 0; c < CB; ++c) acc[i][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  if (nstage > 0) issue(0);
+  for (int st = 0; st < nstage; ++st) {
+    // this wave's DMAs of stage st have landed; after the barrier every
+    // wave's have, and every wave has finished stage st - 1, whose slot the
+    // next issue refills
+    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    wg_barrier_keep_loads();
+    if (st + 1 < nstage) issue(st + 1);
+    const u32x4* base = lds + (st % NST) * kWideStageU4;
+    u32x4 w[2 * CB], ah[RB], al[RB];
+#pragma unroll
+    for (int q = 0; q < 2 * CB; ++q) w[q] = base[W0 + chalf * 1024 + q * 64 + lane];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      ah[i] = base[(rb0 + i) * 64 + lane];
+      al[i] = base[AREG + (rb0 + i) * 64 + lane];
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the products
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+#pragma unroll
+      for (int c = 0; c < CB; ++c) {
+        // column block c = group c / 2, block c % 2: parts 4 (c / 2) + 2 (c % 2) + {0, 1}
+        acc[i][c] = Split16<FP16>::mfma(ah[i], w[2 * c + 0], acc[i][c]);
+        acc[i][c] = Split16<FP16>::mfma(ah[i], w[2 * c + 1], acc[i][c]);
+        acc[i][c] = Split16<FP16>::mfma(al[i], w[2 * c + 0], acc[i][c]);
+      }
+    }
+  }
+  if (chalf >= ntile) return;  // a single-tile unit: the second half is a copy
+  float* P = part + (long long)ks * Mpad * Ntot;
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int c = 0; c < CB; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = row0 + (rb0 + i) * 16 + 4 * (lane >> 4) + r;
+        const int col = col0 + chalf * BN16 + c * 16 + (lane & 15);
+        P[(long long)row * Ntot + col] = acc[i][c][r];
+      }
+}
+
 // FP16 mode: X (FP32) -> Xhi/Xlo of X * rowscale (rows of every class)
 __global__ void fle_xsplit_kernel(const float* __restrict__ X, const float* __restrict__ rowmax, int Mpad,
                                   int Kpad, int rows, uint16_t* __restrict__ Xhi, uint16_t* __restrict__ Xlo) {
@@ -846,29 +962,41 @@ void free_fle_plan(FlePlan* p) {
 // Autotune candidates of the 16-bit GEMM: (pipeline depth, waves per
 // workgroup, rounds of resident workgroups the K split fills).  8 waves hold a
 // 256-row tile (candidates only when the plan's rows come in 256s).
-struct Gemm16Cand { int nst, nw, rounds; };
+struct Gemm16Cand { int nst, nw, rounds, ct; };
 constexpr Gemm16Cand kGemm16Cands[] = {
-    {2, 4, 1}, {3, 4, 1}, {2, 2, 1}, {3, 2, 1}, {2, 4, 2}, {3, 4, 2}, {2, 2, 2}, {3, 2, 2},
-    {2, 4, 3}, {3, 4, 3}, {2, 2, 3}, {3, 2, 3}, {2, 8, 1}, {3, 8, 1}, {2, 8, 2}, {3, 8, 2}};
+    {2, 4, 1, 1}, {3, 4, 1, 1}, {2, 2, 1, 1}, {3, 2, 1, 1}, {2, 4, 2, 1}, {3, 4, 2, 1}, {2, 2, 2, 1},
+    {3, 2, 2, 1}, {2, 4, 3, 1}, {3, 4, 3, 1}, {2, 2, 3, 1}, {3, 2, 3, 1}, {2, 8, 1, 1}, {3, 8, 1, 1},
+    {2, 8, 2, 1}, {3, 8, 2, 1}, {2, 8, 1, 2}, {2, 8, 2, 2}, {2, 8, 3, 2}};
 constexpr int kGemm16Candidates = (int)(sizeof(kGemm16Cands) / sizeof(kGemm16Cands[0]));
 inline bool gemm16_usable(const FlePlan* p, int cand) {
   return kGemm16Cands[cand].nw != 8 || p->Mpad % (2 * BM16) == 0;
 }
 
-inline size_t gemm16_lds(int nst, int nw) { return (size_t)nst * gemm16_stage_u4(nw) * sizeof(u32x4); }
+inline size_t gemm16_lds(int nst, int nw, int ct = 1) {
+  return (size_t)nst * (ct == 2 ? kWideStageU4 : gemm16_stage_u4(nw)) * sizeof(u32x4);
+}
+
+// column units of the wide tile: pairs of 128-column tiles within a class
+int gemm16_units(const FlePlan* p) {
+  int u = 0;
+  for (int c = 0; c < kClasses; ++c) u += ((p->col_off[c + 1] - p->col_off[c]) / BN16 + 1) / 2;
+  return u;
+}
 
 template <bool FP16>
-const void* gemm16_fn(int nst, int nw) {
+const void* gemm16_fn(int nst, int nw, int ct = 1) {
+  if (ct == 2) return (const void*)fle_gemm16w_kernel<FP16>;
   if (nw == 8) return nst == 3 ? (const void*)fle_gemm16_kernel<FP16, 3, 8> : (const void*)fle_gemm16_kernel<FP16, 2, 8>;
   if (nw == 4) return nst == 3 ? (const void*)fle_gemm16_kernel<FP16, 3, 4> : (const void*)fle_gemm16_kernel<FP16, 2, 4>;
   return nst == 3 ? (const void*)fle_gemm16_kernel<FP16, 3, 2> : (const void*)fle_gemm16_kernel<FP16, 2, 2>;
 }
 
 // Resident workgroups per CU of one tile configuration.
-int gemm16_occupancy(int nst, int nw, int* occ) {
+int gemm16_occupancy(int nst, int nw, int ct, int* occ) {
   int n = 0;
   const int rc = mch::hip_check(
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm16_fn<false>(nst, nw), 64 * nw, gemm16_lds(nst, nw)),
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, gemm16_fn<false>(nst, nw, ct), 64 * nw,
+                                                   gemm16_lds(nst, nw, ct)),
       "hipOccupancyMaxActiveBlocksPerMultiprocessor");
   *occ = n > 0 ? n : 1;
   return rc;
@@ -877,8 +1005,9 @@ int gemm16_occupancy(int nst, int nw, int* occ) {
 // K split that fills `rounds` rounds of `slots` resident workgroups with
 // the (column tile x row tile) grid, rounded so that every split but the
 // last has the same number of K blocks.
-int gemm16_splits(const FlePlan* p, int nw, int slots, int rounds) {
-  const int ntiles = p->Ntot / BN16 * (p->Mpad / (BM16 * gemm16_row_tiles(nw)));
+int gemm16_splits(const FlePlan* p, int nw, int ct, int slots, int rounds) {
+  const int ncol = ct == 2 ? gemm16_units(p) : p->Ntot / BN16;
+  const int ntiles = ncol * (p->Mpad / (BM16 * gemm16_row_tiles(nw)));
   const int kblocks = p->Kpad / BK16;
   int s = rounds * slots / ntiles;
   s = std::min(std::max(s, 1), kblocks);
@@ -903,9 +1032,22 @@ int launch_gemm(FlePlan* p, int B, hipStream_t stream, int* splits_out) {
   const int ntn = p->Ntot / BN16;
   const int rt = gemm16_row_tiles(p->nw16);
   const int mt16 = (2 * B + BM16 * rt - 1) / (BM16 * rt);
+  const bool fp16 = p->precision == kFp16x3;
+  if (p->ct16 == 2) {
+    const int nunits = gemm16_units(p);
+    const dim3 gridw(nunits * mt16 * splits);
+    const size_t ldsw = gemm16_lds(2, 8, 2);
+    if (fp16)
+      hipLaunchKernelGGL(fle_gemm16w_kernel<true>, gridw, dim3(512), ldsw, stream, p->Xhi, p->Xlo,
+                         (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, kchunk, nunits, mt16, *p);
+    else
+      hipLaunchKernelGGL(fle_gemm16w_kernel<false>, gridw, dim3(512), ldsw, stream, p->Xhi, p->Xlo,
+                         (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, kchunk, nunits, mt16, *p);
+    *splits_out = splits;
+    return mch::hip_check(hipGetLastError(), "fle_gemm16w_kernel launch");
+  }
   const dim3 grid(ntn * mt16 * splits);
   const size_t lds = gemm16_lds(p->nst16, p->nw16);
-  const bool fp16 = p->precision == kFp16x3;
 #define MCH_GEMM16(F, S, W)                                                                        \
   hipLaunchKernelGGL((fle_gemm16_kernel<F, S, W>), grid, dim3(64 * W), lds, stream, p->Xhi, p->Xlo, \
                      (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, kchunk, ntn, mt16, *p)
@@ -1058,13 +1200,13 @@ int mch_fle_set_precision(void* plan, int precision) {
     MCH_TRY(hipMalloc(&p->rowmax, (size_t)p->Mpad * sizeof(float)), "hipMalloc(rowmax)");
     // dynamic LDS above 64 KB for every tile configuration that needs it
     for (int cand = 0; cand < kGemm16Candidates; ++cand) {
-      const int nst = kGemm16Cands[cand].nst, nw = kGemm16Cands[cand].nw;
-      const int lds = (int)gemm16_lds(nst, nw);
+      const int nst = kGemm16Cands[cand].nst, nw = kGemm16Cands[cand].nw, ct = kGemm16Cands[cand].ct;
+      const int lds = (int)gemm16_lds(nst, nw, ct);
       if (lds <= 64 * 1024) continue;
-      MCH_TRY(hipFuncSetAttribute(gemm16_fn<false>(nst, nw), hipFuncAttributeMaxDynamicSharedMemorySize, lds),
-              "hipFuncSetAttribute");
-      MCH_TRY(hipFuncSetAttribute(gemm16_fn<true>(nst, nw), hipFuncAttributeMaxDynamicSharedMemorySize, lds),
-              "hipFuncSetAttribute");
+      MCH_TRY(hipFuncSetAttribute(gemm16_fn<false>(nst, nw, ct), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  lds), "hipFuncSetAttribute");
+      MCH_TRY(hipFuncSetAttribute(gemm16_fn<true>(nst, nw, ct), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  lds), "hipFuncSetAttribute");
     }
     // partials for the largest K split any autotune candidate uses
     int cus = 0, dev = 0;
@@ -1076,8 +1218,8 @@ int mch_fle_set_precision(void* plan, int precision) {
       if (!gemm16_usable(p, cand)) continue;
       const Gemm16Cand& c = kGemm16Cands[cand];
       int occ = 0;
-      if ((rc = gemm16_occupancy(c.nst, c.nw, &occ))) return rc;
-      smax = std::max(smax, gemm16_splits(p, c.nw, occ * p->cus, c.rounds));
+      if ((rc = gemm16_occupancy(c.nst, c.nw, c.ct, &occ))) return rc;
+      smax = std::max(smax, gemm16_splits(p, c.nw, c.ct, occ * p->cus, c.rounds));
     }
     p->splits16 = smax;
     p->ksplit16 = smax;
@@ -1089,6 +1231,7 @@ int mch_fle_set_precision(void* plan, int precision) {
     MCH_TRY(hipMemset(p->Xlo, 0, nx * sizeof(uint16_t)), "memset(Xlo)");
     p->nst16 = 2;
     p->nw16 = 4;
+    p->ct16 = 1;
   }
   if (p->table_precision != precision) {
     const bool fp16 = precision == kFp16x3;
@@ -1120,15 +1263,16 @@ int mch_fle_set_precision(void* plan, int precision) {
     MCH_TRY(hipEventCreate(&e0), "hipEventCreate");
     MCH_TRY(hipEventCreate(&e1), "hipEventCreate");
     float best = 1e30f;
-    int best_nst = 2, best_nw = 4, best_split = p->splits16;
+    int best_nst = 2, best_nw = 4, best_ct = 1, best_split = p->splits16;
     for (int cand = 0; cand < kGemm16Candidates; ++cand) {
       if (!gemm16_usable(p, cand)) continue;
-      const int nst = kGemm16Cands[cand].nst, nw = kGemm16Cands[cand].nw;
+      const int nst = kGemm16Cands[cand].nst, nw = kGemm16Cands[cand].nw, ct = kGemm16Cands[cand].ct;
       int occ = 0;
-      if ((rc = gemm16_occupancy(nst, nw, &occ))) break;
+      if ((rc = gemm16_occupancy(nst, nw, ct, &occ))) break;
       p->nst16 = nst;
       p->nw16 = nw;
-      p->ksplit16 = gemm16_splits(p, nw, occ * p->cus, kGemm16Cands[cand].rounds);
+      p->ct16 = ct;
+      p->ksplit16 = gemm16_splits(p, nw, ct, occ * p->cus, kGemm16Cands[cand].rounds);
       int splits = 0;
       for (int rep = 0; rep < 2; ++rep)
         if ((rc = launch_gemm(p, p->max_batch, 0, &splits))) break;
@@ -1141,13 +1285,14 @@ int mch_fle_set_precision(void* plan, int precision) {
       MCH_TRY(hipEventSynchronize(e1), "autotune");
       float ms = 0.f;
       (void)hipEventElapsedTime(&ms, e0, e1);
-      if (ms < best) { best = ms; best_nst = nst; best_nw = nw; best_split = p->ksplit16; }
+      if (ms < best) { best = ms; best_nst = nst; best_nw = nw; best_ct = ct; best_split = p->ksplit16; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     p->nst16 = best_nst;
     p->nw16 = best_nw;
+    p->ct16 = best_ct;
     p->ksplit16 = best_split;
   }
 #undef MCH_TRY
@@ -1155,11 +1300,38 @@ int mch_fle_set_precision(void* plan, int precision) {
   return MCH_OK;
 }
 
-// Tile configuration the 16-bit GEMM was tuned to: 100 * K splits + 10 *
-// waves per workgroup + pipeline depth (0 before any 16-bit mode).
+// Pin the 16-bit GEMM to one tile configuration (after the autotune of the
+// first switch to a 16-bit mode): stages, waves per workgroup, 128-column tiles
+// per workgroup and K splits, which must be an autotune candidate usable for
+// the plan with splits within the plan's partial buffer.
+int mch_fle_set_gemm_tiles(void* plan, int stages, int waves, int column_tiles, int k_splits) {
+  FlePlan* p = (FlePlan*)plan;
+  if (!p || !p->nst16) {
+    mch::set_error("mch_fle_set_gemm_tiles: switch the plan to a 16-bit mode first");
+    return MCH_ERR_ARG;
+  }
+  bool found = false;
+  for (int cand = 0; cand < kGemm16Candidates && !found; ++cand)
+    found = gemm16_usable(p, cand) && kGemm16Cands[cand].nst == stages && kGemm16Cands[cand].nw == waves &&
+            kGemm16Cands[cand].ct == column_tiles;
+  if (!found || k_splits < 1 || k_splits > p->splits16 || k_splits > p->Kpad / BK16) {
+    mch::set_error("mch_fle_set_gemm_tiles: (%d stages, %d waves, %d column tiles, %d splits) is not a usable "
+                   "configuration (splits <= %d)", stages, waves, column_tiles, k_splits, p->splits16);
+    return MCH_ERR_ARG;
+  }
+  p->nst16 = stages;
+  p->nw16 = waves;
+  p->ct16 = column_tiles;
+  p->ksplit16 = k_splits;
+  return MCH_OK;
+}
+
+// Tile configuration the 16-bit GEMM was tuned to: 10000 * (column tiles per
+// workgroup - 1) + 100 * K splits + 10 * waves per workgroup + pipeline depth
+// (0 before any 16-bit mode).
 int mch_fle_tuned_depth(void* plan) {
   FlePlan* p = (FlePlan*)plan;
-  return p && p->nst16 ? 100 * p->ksplit16 + 10 * p->nw16 + p->nst16 : 0;
+  return p && p->nst16 ? 10000 * (p->ct16 - 1) + 100 * p->ksplit16 + 10 * p->nw16 + p->nst16 : 0;
 }
 
 // a (B, ne) complex64 rows of stride lda = B^H f for volumes (B, N^3) float32.

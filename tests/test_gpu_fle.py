@@ -191,3 +191,31 @@ def test_projection_at_shift_equals_projection_of_extracted_patch(gpu_device, mo
     # and no shift at all is the plain projection
     assert np.array_equal(f.evaluate_t(vols, shift_zyx=torch.zeros_like(sh)).cpu().numpy(),
                           f.evaluate_t(vols).cpu().numpy())
+
+
+@pytest.mark.parametrize("mode", ["bf16x3", "fp16x3"])
+def test_every_gemm_tile_configuration_matches_fp32(gpu_device, mode):
+    # batch 256 (512 GEMM rows): the 256-row tiles (8 waves) and the 256 x 256
+    # tiles (two 128-column tiles of one class; classes with an odd tile count
+    # end in a single-tile unit) are usable beside the 128-row ones; every
+    # configuration gives the FP32 projection to the split precision's bar
+    from matcha_amd.core.FLEBasis3D import FLEBasis3D
+    N, bl, radius, B = 96, 32, 45, 256
+    g = torch.Generator(device=gpu_device).manual_seed(5)
+    vols = torch.randn((B, N, N, N), device=gpu_device, generator=g)
+    f = FLEBasis3D(N, bl, 1e-4, batchsize=B, radius=radius, device=gpu_device)
+    plain = f.evaluate_t(vols).cpu().numpy()
+    scale = np.linalg.norm(plain, axis=1)
+    f.set_compute_precision(mode)
+    tested = []
+    for stages, waves, ctiles in [(2, 4, 1), (3, 4, 1), (2, 2, 1), (3, 2, 1), (2, 8, 1), (3, 8, 1), (2, 8, 2)]:
+        for splits in (1, 2):
+            f.set_gemm_tiles(stages, waves, ctiles, splits)
+            assert f.gemm_tiles() == {"column_tiles": ctiles, "k_splits": splits, "waves": waves, "stages": stages}
+            got = f.evaluate_t(vols).cpu().numpy()
+            err = np.max(np.abs(got - plain), axis=1) / scale
+            assert err.max() < SPLIT_TOL[mode] + 2e-5, (stages, waves, ctiles, splits, err.max())
+            tested.append((stages, waves, ctiles, splits))
+    assert len(tested) == 14
+    with pytest.raises(RuntimeError):
+        f.set_gemm_tiles(3, 8, 2, 1)   # not a candidate (the wide tile has two stages)
