@@ -139,6 +139,73 @@ __global__ void so3_repack_kernel(const float2* __restrict__ sig, long long ldsi
   }
 }
 
+// Order-preserving 32-bit image of a float: a > b iff key(a) > key(b), with
+// -0 and +0 mapped alike (a tie, as under a float compare).  0 is the image of
+// the all-ones NaN only and marks an empty (taken or padding) slot.
+__device__ __forceinline__ unsigned float_key(float v) {
+  const unsigned u = __float_as_uint(v + 0.0f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(unsigned k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+// C rounds of argmax over the values of a 256-thread workgroup, value
+// descending and flat index ascending among ties (the order torch.topk gives
+// the grid in SOFFT.ids_to_angles / Matcha.py:27-58 up to the tie order, which
+// is fixed here).  hk[i] holds float_key of the thread's i-th value, at flat
+// index qof(i) of the slice, increasing with i; round c writes
+// part_val/part_idx[orow + c] (index qbase + q, or -1 once the slice is
+// exhausted).  A thread's best is one 64-bit word (key, ~q), so a round is one
+// wave max-reduction and one four-way merge; each thread keeps its best word
+// and only the owner of the round's winner rescans its values.
+template <int NV, class QOf>
+__device__ __forceinline__ void block_top_c(unsigned (&hk)[NV], QOf qof, int C, long long orow,
+                                            int qbase, float* __restrict__ part_val,
+                                            int* __restrict__ part_idx) {
+  __shared__ unsigned long long red[2][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int ib = 0;
+  // first maximum over i (= smallest q among equal keys); empty slots (0) never win
+  auto local_best = [&]() {
+    unsigned bk = 0u;
+    ib = 0;
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+      if (hk[i] > bk) { bk = hk[i]; ib = i; }
+    return bk == 0u ? 0ull : ((unsigned long long)bk << 32) | (unsigned)~qof(ib);
+  };
+  unsigned long long best = local_best();
+  for (int c = 0; c < C; ++c) {
+    unsigned long long w = best;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const unsigned long long o = __shfl_xor(w, off, 64);
+      w = o > w ? o : w;
+    }
+    // two buffers: round c + 1 writes the other one, and round c + 2 starts
+    // after every thread has passed round c + 1's barrier, i.e. read round c's
+    if (lane == 0) red[c & 1][wave] = w;
+    __syncthreads();
+    w = red[c & 1][0];
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+      const unsigned long long o = red[c & 1][k];
+      w = o > w ? o : w;
+    }
+    if (w != 0ull && w == best) {  // the (unique) owner of the winner
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+        if (i == ib) hk[i] = 0u;
+      best = local_best();
+    }
+    if (threadIdx.x == 0) {
+      part_val[orow + c] = w != 0ull ? key_float((unsigned)(w >> 32)) : -INFINITY;
+      part_idx[orow + c] = w != 0ull ? qbase + (int)~(unsigned)w : -1;
+    }
+  }
+}
+
 // One workgroup per (particle, beta row): F(u,v) = sum_l d^l_{uv}(beta_j) sigma,
 // 2-D inverse DFT over (alpha, gamma), local top-C of the n*n slice.
 template <int MAXPER>
@@ -154,8 +221,6 @@ __global__ __launch_bounds__(256) void sofft_slice_kernel(
   float2* F = smem;            // W*W
   float2* G = F + W * W;       // W*n
   float2* tw = G + W * n;      // n
-  __shared__ float red_v[4];
-  __shared__ int red_i[4];
   const int tid = threadIdx.x;
   for (int k = tid; k < n; k += blockDim.x) tw[k] = twiddle[k];
   const float2* s = sig + (long long)b * ldsig;
@@ -193,7 +258,7 @@ __global__ __launch_bounds__(256) void sofft_slice_kernel(
   // in registers (static indexing: MAXPER >= ceil(n*n / 256))
   const float inv = 1.0f / ((float)n * (float)n);
   const int nn = n * n;
-  float vals[MAXPER];
+  unsigned vals[MAXPER];  // float_key of the values (0: padding)
 #pragma unroll
   for (int r = 0; r < MAXPER; ++r) {
     const int q = tid + r * 256;
@@ -210,41 +275,11 @@ __global__ __launch_bounds__(256) void sofft_slice_kernel(
       }
       acc *= inv;
     }
-    vals[r] = acc;
+    vals[r] = q < nn ? float_key(acc) : 0u;
     if (grid_out != nullptr && q < nn) grid_out[((long long)b * n + j) * nn + q] = acc;
   }
-  // C rounds of block argmax (value desc, flat index asc)
-  const int lane = tid & 63, wave = tid >> 6;
-  for (int c = 0; c < C; ++c) {
-    float bv = -INFINITY;
-    int bi = 0x7fffffff;
-#pragma unroll
-    for (int r = 0; r < MAXPER; ++r) {
-      const int q = tid + r * 256;
-      if (q < nn && (vals[r] > bv || (vals[r] == bv && q < bi))) { bv = vals[r]; bi = q; }
-    }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      const float ov = __shfl_xor(bv, off, 64);
-      const int oi = __shfl_xor(bi, off, 64);
-      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-    }
-    if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
-    __syncthreads();
-    bv = red_v[0]; bi = red_i[0];
-    for (int w = 1; w < 4; ++w) {
-      if (red_v[w] > bv || (red_v[w] == bv && red_i[w] < bi)) { bv = red_v[w]; bi = red_i[w]; }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < MAXPER; ++r)
-      if (tid + r * 256 == bi) vals[r] = -INFINITY;  // taken
-    if (tid == 0) {
-      const long long o = ((long long)b * n + j) * C + c;
-      part_val[o] = bv;
-      part_idx[o] = (bi == 0x7fffffff) ? -1 : j * nn + bi;
-    }
-  }
+  block_top_c(vals, [&](int r) { return tid + r * 256; }, C, ((long long)b * n + j) * C, j * nn,
+              part_val, part_idx);
 }
 
 // Same slice as sofft_slice_kernel for n = 16 R: the phase table
@@ -265,8 +300,6 @@ __global__ __launch_bounds__(256) void sofft_slice_blk_kernel(
   float2* F = smem;            // W*W
   float2* G = F + W * W;       // W*n   G[ui][g]
   float2* E = G + W * n;       // W*n   E[ui][x]
-  __shared__ float red_v[4];
-  __shared__ int red_i[4];
   const int tid = threadIdx.x;
   for (int q = tid; q < W * n; q += 256) {
     const int ui = q / n, xx = q - (q / n) * n;
@@ -320,51 +353,17 @@ __global__ __launch_bounds__(256) void sofft_slice_blk_kernel(
       for (int c = 0; c < R; ++c) vals[r * R + c] += e[r].x * gv[c].x - e[r].y * gv[c].y;
   }
   const float inv = 1.0f / ((float)n * (float)n);
+  unsigned keys[R * R];  // float_key of the values
 #pragma unroll
   for (int r = 0; r < R; ++r)
 #pragma unroll
     for (int c = 0; c < R; ++c) {
-      vals[r * R + c] *= inv;
-      if (grid_out != nullptr)
-        grid_out[((long long)b * n + j) * nn + (a0 + r) * n + g0 + c] = vals[r * R + c];
+      const float v = vals[r * R + c] * inv;
+      keys[r * R + c] = float_key(v);
+      if (grid_out != nullptr) grid_out[((long long)b * n + j) * nn + (a0 + r) * n + g0 + c] = v;
     }
-  // C rounds of block argmax (value desc, flat index asc)
-  const int lane = tid & 63, wave = tid >> 6;
-  for (int cc = 0; cc < C; ++cc) {
-    float bv = -INFINITY;
-    int bi = 0x7fffffff;
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-#pragma unroll
-      for (int c = 0; c < R; ++c) {
-        const int q = (a0 + r) * n + g0 + c;
-        const float v = vals[r * R + c];
-        if (v > bv || (v == bv && q < bi)) { bv = v; bi = q; }
-      }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      const float ov = __shfl_xor(bv, off, 64);
-      const int oi = __shfl_xor(bi, off, 64);
-      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-    }
-    if (lane == 0) { red_v[wave] = bv; red_i[wave] = bi; }
-    __syncthreads();
-    bv = red_v[0]; bi = red_i[0];
-    for (int w = 1; w < 4; ++w) {
-      if (red_v[w] > bv || (red_v[w] == bv && red_i[w] < bi)) { bv = red_v[w]; bi = red_i[w]; }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-#pragma unroll
-      for (int c = 0; c < R; ++c)
-        if ((a0 + r) * n + g0 + c == bi) vals[r * R + c] = -INFINITY;  // taken
-    if (tid == 0) {
-      const long long o = ((long long)b * n + j) * C + cc;
-      part_val[o] = bv;
-      part_idx[o] = (bi == 0x7fffffff) ? -1 : j * nn + bi;
-    }
-  }
+  block_top_c(keys, [&](int i) { return (a0 + i / R) * n + g0 + i % R; }, C,
+              ((long long)b * n + j) * C, j * nn, part_val, part_idx);
 }
 
 // Merge the per-row candidates into the global top-C per particle and convert
