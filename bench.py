@@ -51,8 +51,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 GEMM16_FETCH_BYTES = {("box96_L32", 64, 4, 1): 587089 * 1024 * 2, ("box96_L32", 512, 8, 1): 1720199 * 1024 * 2,
                       ("box96_L32", 512, 8, 2): 1434256 * 1024 * 2}
 # HBM bytes (FETCH_SIZE x 2.0 + WRITE_SIZE) per launch of the shift search's
-# column pass, keyed by (workload, batch); measured like the GEMM's.
-SHIFT_Z_TRAFFIC_BYTES = {("box96_L32", 512): 3553865 * 1024 * 2 + 2102084 * 1024}
+# column pass, keyed by (workload, batch, reference table precision); measured
+# like the GEMM's.
+SHIFT_Z_TRAFFIC_BYTES = {("box96_L32", 512, "fp32"): 3553865 * 1024 * 2 + 2102084 * 1024}
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
@@ -74,6 +75,8 @@ def parse():
     p.add_argument("--batch", type=int, default=None, help="plain-path batch (default: the workload's)")
     p.add_argument("--tuned-batch", type=int, default=None, help="tuned-path batch (default: the workload's)")
     p.add_argument("--tuned-precision", default="bf16x3", choices=["bf16x3", "fp16x3"])
+    p.add_argument("--tuned-shift-precision", default="fp16", choices=["fp32", "fp16"],
+                   help="table the tuned path's shift search samples the rotated reference from")
     p.add_argument("--tuned-streams", type=int, default=2,
                    help="streams the tuned path's batches alternate over (one plan set each)")
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP32 path")
@@ -186,7 +189,8 @@ def run_path(name, torch, dev, w, args, pool_vols, pool_R, pool_t, precision, B,
     ev[3].record(stream)
     torch.cuda.synchronize()
     return {
-        "precision": precision, "batch": B, "streams": nstreams, "volumes_per_s": vps,
+        "precision": precision, "shift_reference": sm.reference_precision, "batch": B, "streams": nstreams,
+        "volumes_per_s": vps,
         "ms_per_step": elapsed / args.steps * 1e3,
         "median_err_deg": float(np.median(errs)), "p90_err_deg": float(np.percentile(errs, 90)),
         "shift_exact_frac": float(shift_ok.mean()), "particles_checked": int(errs.size),
@@ -488,7 +492,7 @@ def main():
     t_setup = time.perf_counter()
     wl = Workload(w["N"], w["n_blobs"], w["sigma"], args.seed, dev, mask_radius=w["radius"])
 
-    def plans(B, precision):
+    def plans(B, precision, shift_precision):
         """Projection, search and shift plans sized for one path's batch."""
         c = CrossCorrelationMatcher(w["N"], dev, batchsize=B, bandlimit=w["bandlimit"], radius=w["radius"],
                                     compute_precision=precision)
@@ -496,14 +500,16 @@ def main():
         m = Matcha(B, dev, w["L_max"], {"Li": w["Li"], "candidates": w["C"], "num_steps": w["steps"],
                                          "oversampling_factor_K": w["K"]})
         sh = ShiftMatcher({"N": w["N"], "num_subtomograms_per_batch": B, "shift_upsample_factor": 1,
-                           "shift_search_radius": w["max_shift"], "voxel_size": 1.0}, dev)
+                           "shift_search_radius": w["max_shift"], "voxel_size": 1.0,
+                           "shift_reference_precision": shift_precision}, dev)
         sh.set_reference(wl.template.cpu().numpy())
         return c, m, sh
 
     # the 16-bit tables are built (and the GEMM tuned) here, one plan set per stream
-    tuned_sets = [plans(Bt, args.tuned_precision) for _ in range(max(1, args.tuned_streams))]
+    tuned_sets = [plans(Bt, args.tuned_precision, args.tuned_shift_precision)
+                  for _ in range(max(1, args.tuned_streams))]
     ccm, matcher, sm = tuned_sets[0]
-    plain_plans = None if args.no_plain else plans(Bp, "fp32")
+    plain_plans = None if args.no_plain else plans(Bp, "fp32", "fp32")
     P = max(args.pool, Bmax)
     chunks = [wl.particles(min(64, P - i), w["max_shift"], args.snr) for i in range(0, P, 64)]
     pool_vols = torch.cat([c[0] for c in chunks])
@@ -544,13 +550,15 @@ def main():
     # 8 ky columns: forward z FFT, rotated-reference sampling, cross power,
     # inverse z FFT).  Algorithmic bytes: the half spectrum read and written
     # (B H N^2 x 8 B each way) + the background spectrum XMy and the reference
-    # pair table once; algorithmic flops: 2 passes of H N length-N FFTs
+    # table once ((N + 2)^2 (N + 1) entries of 16 B, FP32 pairs or FP16 quads);
+    # algorithmic flops: 2 passes of H N length-N FFTs
     # (5 N log2 N each) + 38 per sampled frequency (trilinear complex sample,
     # norms, product)
     shift_ms = time_shift_kernels(torch, dev, sm, pool_vols[:Bt])
     N, H = w["N"], w["N"] // 2 + 1
     z_t = shift_ms[1] * 1e-3
-    z_bytes = 2.0 * Bt * H * N * N * 8 + H * N * N * 8 + N * N * (N + 1) * 16
+    z_h16 = args.tuned_shift_precision == "fp16"
+    z_bytes = 2.0 * Bt * H * N * N * 8 + H * N * N * 8 + (N + 2) ** 2 * (N + 1) * 16
     z_flops = Bt * H * (2 * N * 5.0 * N * math.log2(N) + N * N * 38.0)
     z_gbs = z_bytes / z_t / 1e9
 
@@ -584,8 +592,9 @@ def main():
             "setup_s": t_setup,
         },
         "roofline": {"bound": "hbm", "achieved": z_gbs, "peak": 8000.0, "unit": "GB/s",
-                     "frac": z_gbs / 8000.0, "traffic": SHIFT_Z_TRAFFIC_BYTES.get((args.workload, Bt)),
-                     "kernel": f"shift_z_kernel<{N}>", "avg_ms": z_t * 1e3, "bytes_per_launch": z_bytes,
+                     "frac": z_gbs / 8000.0,
+                     "traffic": SHIFT_Z_TRAFFIC_BYTES.get((args.workload, Bt, args.tuned_shift_precision)),
+                     "kernel": f"shift_z_kernel<{N}, {'true' if z_h16 else 'false'}>", "avg_ms": z_t * 1e3, "bytes_per_launch": z_bytes,
                      "flops_per_launch": z_flops, "fp32_vector_tflops": z_flops / z_t / 1e12,
                      "fp32_vector_frac": z_flops / z_t / 157.3e12,
                      "share_of_step": 2 * z_t * 1e3 / tuned["ms_per_step"],

@@ -189,6 +189,16 @@ int mch_shift_search(void* plan, const float* subtomos, long long ld_vol, int B,
                      const float* shift_zyx, const float* rot, const float* ctf, float* out_shift,
                      float* out_score, hipStream_t stream);
 
+/* Table the searches sample the rotated reference from (no reference
+ * counterpart: the reference rotates its complex64 spectrum with grid_sample,
+ * src/matcha/core/ShiftMatcher.py:491-537).  fp16 = 0 (default): complex FP32
+ * pairs, the reference's arithmetic.  fp16 = 1: FP16 quads scaled by a power
+ * of two, two 16-byte gathers per trilinear sample instead of four, weights
+ * rounded to FP16 and products accumulated in FP32 (a reduced-precision mode
+ * for throughput; shifts and scores then agree with the FP32 table to the
+ * tolerance its tests state, not bit for bit). */
+int mch_shift_set_reference_precision(void* plan, int fp16);
+
 /* Per-kernel timing of mch_shift_search (a measurement aid, no reference
  * counterpart): when enabled, every search records events around its four
  * launches on its stream; mch_shift_kernel_ms waits for the last search and

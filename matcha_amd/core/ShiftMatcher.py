@@ -13,6 +13,12 @@ power, inverse FFT, argmax) runs in ``shift.hip``.
 up > 1 gives the reference's local sub-voxel refinement (``compute_shift``,
 mode "local": the coarse lag is zeroed and a matrix DFT is evaluated on
 ceil(up * (2 * shift_search_radius + 1))^3 points around zero, 1/up apart).
+
+``shift_reference_precision`` (an extension, no reference counterpart): "fp32"
+(default) samples the rotated reference from complex FP32 values, the
+reference's complex64 arithmetic; "fp16" from FP16 values scaled by a power of
+two, with FP16 trilinear weights and FP32 accumulation -- half the gathers of
+the search's column pass, for the throughput configuration.
 """
 from __future__ import annotations
 
@@ -100,6 +106,10 @@ class ShiftMatcher:
         self.do_ctf = bool(cfg_get(config, "do_ctf_correction", True))
         self.box_size = float(cfg_get(config, "box_size", self.N))
         self.voxel_size = float(cfg_get(config, "voxel_size", 1.0))
+        self.reference_precision = str(cfg_get(config, "shift_reference_precision", "fp32"))
+        if self.reference_precision not in ("fp32", "fp16"):
+            raise ValueError(f"shift_reference_precision must be 'fp32' or 'fp16', got "
+                             f"{self.reference_precision!r}")
         self.reference_data_fourier = None
         self._plan = None
 
@@ -119,6 +129,18 @@ class ShiftMatcher:
         self._plan = h
         rc = _lib.lib().mch_shift_plan_set_upsample(self._plan, self.upsample, self.radius)
         _lib.check(rc, "mch_shift_plan_set_upsample")
+        self.set_reference_precision(self.reference_precision)
+
+    def set_reference_precision(self, mode: str):
+        """Select the table the searches sample the rotated reference from:
+        "fp32" (the reference's complex64 values) or "fp16" (reduced precision,
+        half the gathers)."""
+        if mode not in ("fp32", "fp16"):
+            raise ValueError(f"shift reference precision must be 'fp32' or 'fp16', got {mode!r}")
+        self.reference_precision = mode
+        if self._plan is not None:
+            rc = _lib.lib().mch_shift_set_reference_precision(self._plan, 1 if mode == "fp16" else 0)
+            _lib.check(rc, "mch_shift_set_reference_precision")
 
     def __del__(self):
         h = getattr(self, "_plan", None)

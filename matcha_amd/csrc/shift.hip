@@ -48,7 +48,10 @@ struct ShiftPlan {
   int N, H, max_batch;
   float radius, radius_p, width, wsum;  // cosine mask geometry, sum of background weights
   int lut_lo, lut_n;  // w(r2) tabulated for lut_lo <= r2 < lut_lo + lut_n; 0 below, 1 above
-  float4* refpair;  // (ref[z][y][x-1], ref[z][y][x]) of the centred reference, N x N x (N + 1) bricked (pair_index)
+  float4* refpair;  // FP32 pairs of the centred reference (table_index)
+  uint4* refh16;     // FP16 quads of the same reference, scaled by 2^e (table_index)
+  float h16_unscale;  // 2^-e
+  int ref_h16;      // 1: the searches sample refh16, 0 (default): refpair
   float2* twid;     // (N) e^{-2 pi i j / N}
   float* lut;       // (lut_n)
   float2* XM;       // (H, N, N): x-spectrum of the rolled background weight, [kx][z][y]
@@ -200,60 +203,164 @@ __global__ __launch_bounds__(row_threads<N>()) void shift_fwd_x_kernel(
 // ---------------------------------------------------------------------------
 // rotated reference sample
 // ---------------------------------------------------------------------------
-// The pair table holds, for every (z, y) and column xi = 0..N, the 16-byte pair
-// (ref[z][y][xi - 1], ref[z][y][xi]) (zero outside the volume), in bricks of
-// 2 x 2 (z, y): the four pairs of one brick and column fill one 64-byte
-// segment, so that the gathers of samples that are close in z and y (the 4 x
-// 4 x 4 blocks of frequencies a wave samples) share segments (the column pass
-// took 1.77 ms per batch of 512 with rows of pairs, 1.61 with these bricks;
-// 128-byte bricks of 2 x 2 x 2 gave no further gain).  Index of pair
-// (z, y, xi), 0 <= z, y < N, 0 <= xi <= N:
-__host__ __device__ __forceinline__ unsigned pair_index(int N, int z, int y, int xi) {
-  return (((unsigned)((z >> 1) * (N >> 1) + (y >> 1)) * (unsigned)(N + 1) + (unsigned)xi) << 2) |
-         ((unsigned)(z & 1) << 1) | (unsigned)(y & 1);
+// Two tables of the centred reference, one 16-byte entry per (z', y', x'),
+// 0 <= z', y' < N + 2, 0 <= x' <= N, for the point (z, y, x) = (z' - 1, y' - 1,
+// x' - 1), with a zero border around the volume: a sample whose lower corner
+// (z0, y0, x0) lies in [-1, N)^3 reads entry (z0 + 1, y0 + 1, x0 + 1) and its
+// neighbours, all inside the table.
+//   FP32 pairs  (ref[z][y][x], ref[z][y][x + 1]) as complex FP32 -- the
+//               reference's complex64 values; a trilinear sample reads the four
+//               (z, y) corners;
+//   FP16 quads  rows y, y + 1 and columns x, x + 1 of plane z, scaled by 2^e,
+//               as four FP16 pairs over x: (re, y), (im, y), (re, y + 1),
+//               (im, y + 1) -- reduced precision; a sample reads two entries
+//               (z0 and z0 + 1), half the gathers and half the bytes through
+//               the texture path (the column pass: 1.60 -> 1.45 ms per batch of
+//               512 at N = 96; a compact FP16 table of one 4-byte point per
+//               entry, read as eight 4-byte loads, kept 2.4 GB more of the
+//               gathers in L2 but took 1.65 ms).
+// Entries in bricks of 2 x 2 (z', y'): the four entries of one brick and
+// column fill one 64-byte segment, so that the gathers of samples that are
+// close in z and y (the 4 x 4 x 4 blocks of frequencies a wave samples) share
+// segments (the column pass took 1.77 ms per batch of 512 with rows of pairs,
+// 1.61 with these bricks; 128-byte bricks of 2 x 2 x 2 gave no further gain).
+__host__ __device__ constexpr unsigned table_index(int N, int zp, int yp, int xp) {
+  return (((unsigned)((zp >> 1) * ((N + 2) >> 1) + (yp >> 1)) * (unsigned)(N + 1) + (unsigned)xp) << 2) |
+         ((unsigned)(zp & 1) << 1) | (unsigned)(yp & 1);
 }
-// pairs in the table
-inline size_t pair_table_size(int N) { return (size_t)N * N * (N + 1); }
+// entries of either table (N even)
+constexpr size_t table_entries(int N) { return (size_t)(N + 2) * (N + 2) * (N + 1); }
 template <int N>
-__host__ __device__ __forceinline__ unsigned pair_offset(int z, int y, int xi) {
-  return pair_index(N, z, y, xi) * 16u;
+constexpr unsigned table_bytes() { return (unsigned)(table_entries(N) * 16u); }
+// byte steps to the entry of z' + 1 / y' + 1 from (z', y'): the next entry of
+// the brick for even z' / y', else the next brick along z / y
+template <int N>
+constexpr unsigned table_zbrick() { return (unsigned)((N + 2) / 2) * (unsigned)(N + 1) * 64u; }
+template <int N>
+constexpr unsigned table_ybrick() { return (unsigned)(N + 1) * 64u; }
+
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+
+// Byte offset of entry (z', y', x') from 24-bit products (every index of the
+// table stays below 2^24).
+template <int N>
+__device__ __forceinline__ unsigned table_off24(unsigned zp, unsigned yp, unsigned xp) {
+  const unsigned brick = __umul24(zp >> 1, (unsigned)((N + 2) >> 1)) + (yp >> 1);
+  return ((__umul24(brick, (unsigned)(N + 1)) + xp) << 6) | ((zp & 1u) << 5) | ((yp & 1u) << 4);
 }
 
-// Trilinear sample of the centred reference spectrum at the rotated position
-// (rx, ry, rz) (x the last axis), through the reference's grid_sample
-// normalisation with size_e = N + 1 (align_corners): i = ((r sc - 1) + 1) / 2 (N - 1).
-// The pair column x0 + 1 holds (ref[x0], ref[x0 + 1]) for -1 <= x0 < N; corners
-// outside the volume and samples outside the ball (inball false) read as zero.
-template <int N>
-__device__ __forceinline__ float2 ref_trilinear(__amdgpu_buffer_rsrc_t rref, float rx, float ry, float rz,
-                                                bool inball) {
-  const float sc = 2.0f / (float)(N - 1);  // 2 / (size_e - 2) with size_e = N + 1
-  const float um = (float)(N - 1);
-  const float ix = ((rx * sc - 1.0f) + 1.0f) * 0.5f * um;
-  const float iy = ((ry * sc - 1.0f) + 1.0f) * 0.5f * um;
-  const float iz = ((rz * sc - 1.0f) + 1.0f) * 0.5f * um;
+// The corners of one trilinear sample of the centred reference spectrum,
+// loaded (FP32: four pairs over x for the (z, y) corners; FP16: two quads for
+// z0 and z0 + 1), and the fractional position.  A sample goes in three steps,
+// so that a caller can keep the next sample's loads in flight while it
+// combines the current one, and skip the geometry of samples it knows to be
+// zero: ref_place (offset and fractions), ref_load (the loads, unconditional:
+// corners outside the volume and samples outside the ball take the
+// out-of-range offset and read as zero), ref_combine.
+template <bool H16>
+struct RefCorners {
+  v4u q[H16 ? 2 : 4];
+  float fx, fy, fz;
+  unsigned o0, dz, dy;  // offset of the lower corner's entry (kOutOfRange: all read zero), steps
+};
+
+// Sample at the rotated position (rx, ry, rz) (x the last axis).  FP32 table:
+// through the reference's grid_sample normalisation with size_e = N + 1
+// (align_corners), i = ((r sc - 1) + 1) / 2 (N - 1), rounded as the reference
+// rounds it.  FP16 table: that normalisation is the identity up to rounding
+// and is skipped (i = r).
+template <int N, bool H16>
+__device__ __forceinline__ void ref_place(RefCorners<H16>& r, float rx, float ry, float rz, bool inball) {
+  float ix = rx, iy = ry, iz = rz;
+  if constexpr (!H16) {
+    const float sc = 2.0f / (float)(N - 1);  // 2 / (size_e - 2) with size_e = N + 1
+    const float um = (float)(N - 1);
+    ix = ((rx * sc - 1.0f) + 1.0f) * 0.5f * um;
+    iy = ((ry * sc - 1.0f) + 1.0f) * 0.5f * um;
+    iz = ((rz * sc - 1.0f) + 1.0f) * 0.5f * um;
+  }
   const float fz0 = floorf(iz), fy0 = floorf(iy), fx0 = floorf(ix);
   const int z0 = (int)fz0, y0 = (int)fy0, x0 = (int)fx0;
-  const float fz = iz - fz0, fy = iy - fy0, fx = ix - fx0;
-  const bool xok = x0 >= -1 && x0 < N;
-  const int z1 = z0 + 1, y1 = y0 + 1, xi = x0 + 1;
-  const bool z0ok = z0 >= 0 && z0 < N, z1ok = z0 >= -1 && z0 < N - 1;
-  const bool y0ok = y0 >= 0 && y0 < N, y1ok = y0 >= -1 && y0 < N - 1;
-  // lanes outside the ball issue no loads
-  float4 q00 = make_float4(0.f, 0.f, 0.f, 0.f), q01 = q00, q10 = q00, q11 = q00;
-  if (inball) {
-    q00 = load_f4(rref, xok && z0ok && y0ok ? pair_offset<N>(z0, y0, xi) : kOutOfRange);
-    q01 = load_f4(rref, xok && z0ok && y1ok ? pair_offset<N>(z0, y1, xi) : kOutOfRange);
-    q10 = load_f4(rref, xok && z1ok && y0ok ? pair_offset<N>(z1, y0, xi) : kOutOfRange);
-    q11 = load_f4(rref, xok && z1ok && y1ok ? pair_offset<N>(z1, y1, xi) : kOutOfRange);
+  r.fz = iz - fz0;
+  r.fy = iy - fy0;
+  r.fx = ix - fx0;
+  // one range check for all corners: the lower corner in [-1, N)^3 (the
+  // border entries read as zero); the other corners are steps from it
+  const bool ok = inball && (unsigned)(z0 + 1) <= (unsigned)N && (unsigned)(y0 + 1) <= (unsigned)N &&
+                  (unsigned)(x0 + 1) <= (unsigned)N;
+  r.o0 = ok ? table_off24<N>((unsigned)(z0 + 1), (unsigned)(y0 + 1), (unsigned)(x0 + 1)) : kOutOfRange;
+  r.dz = ((z0 + 1) & 1) ? table_zbrick<N>() - 32u : 32u;
+  r.dy = ((y0 + 1) & 1) ? table_ybrick<N>() - 16u : 16u;
+}
+
+template <bool H16>
+__device__ __forceinline__ void ref_load(RefCorners<H16>& r, __amdgpu_buffer_rsrc_t rref) {
+  const bool ok = r.o0 != kOutOfRange;
+  if constexpr (H16) {
+    r.q[0] = __builtin_amdgcn_raw_buffer_load_b128(rref, r.o0, 0, 0);
+    r.q[1] = __builtin_amdgcn_raw_buffer_load_b128(rref, ok ? r.o0 + r.dz : kOutOfRange, 0, 0);
+  } else {
+    r.q[0] = __builtin_amdgcn_raw_buffer_load_b128(rref, r.o0, 0, 0);
+    r.q[1] = __builtin_amdgcn_raw_buffer_load_b128(rref, ok ? r.o0 + r.dy : kOutOfRange, 0, 0);
+    r.q[2] = __builtin_amdgcn_raw_buffer_load_b128(rref, ok ? r.o0 + r.dz : kOutOfRange, 0, 0);
+    r.q[3] = __builtin_amdgcn_raw_buffer_load_b128(rref, ok ? r.o0 + r.dz + r.dy : kOutOfRange, 0, 0);
   }
-  const float s00 = (1.f - fz) * (1.f - fy), s01 = (1.f - fz) * fy, s10 = fz * (1.f - fy), s11 = fz * fy;
-  float2 rv;
-  rv.x = s00 * fmaf(fx, q00.z - q00.x, q00.x) + s01 * fmaf(fx, q01.z - q01.x, q01.x) +
-         s10 * fmaf(fx, q10.z - q10.x, q10.x) + s11 * fmaf(fx, q11.z - q11.x, q11.x);
-  rv.y = s00 * fmaf(fx, q00.w - q00.y, q00.y) + s01 * fmaf(fx, q01.w - q01.y, q01.y) +
-         s10 * fmaf(fx, q10.w - q10.y, q10.y) + s11 * fmaf(fx, q11.w - q11.y, q11.y);
-  return rv;
+}
+
+// The trilinear value from loaded corners.  FP32: the reference's order of
+// operations (x, then the four (z, y) weights).  FP16: the eight weights
+// rounded to FP16 in x pairs, each (z, y) corner pair through one FP16 dot
+// product accumulated in FP32; the result is still scaled by the table's 2^e.
+template <bool H16>
+__device__ __forceinline__ float2 ref_combine(const RefCorners<H16>& r) {
+  const float fx = r.fx, fy = r.fy, fz = r.fz;
+  if constexpr (H16) {
+    const float a00 = (1.f - fz) * (1.f - fy), a01 = (1.f - fz) * fy, a10 = fz * (1.f - fy), a11 = fz * fy;
+    const float wx0 = 1.f - fx, wx1 = fx;
+    const half2v h00 = {(_Float16)(a00 * wx0), (_Float16)(a00 * wx1)};
+    const half2v h01 = {(_Float16)(a01 * wx0), (_Float16)(a01 * wx1)};
+    const half2v h10 = {(_Float16)(a10 * wx0), (_Float16)(a10 * wx1)};
+    const half2v h11 = {(_Float16)(a11 * wx0), (_Float16)(a11 * wx1)};
+    auto h2 = [](unsigned u) { return __builtin_bit_cast(half2v, u); };
+    float re = __builtin_amdgcn_fdot2(h00, h2(r.q[0].x), 0.f, false);
+    float im = __builtin_amdgcn_fdot2(h00, h2(r.q[0].y), 0.f, false);
+    re = __builtin_amdgcn_fdot2(h01, h2(r.q[0].z), re, false);
+    im = __builtin_amdgcn_fdot2(h01, h2(r.q[0].w), im, false);
+    re = __builtin_amdgcn_fdot2(h10, h2(r.q[1].x), re, false);
+    im = __builtin_amdgcn_fdot2(h10, h2(r.q[1].y), im, false);
+    re = __builtin_amdgcn_fdot2(h11, h2(r.q[1].z), re, false);
+    im = __builtin_amdgcn_fdot2(h11, h2(r.q[1].w), im, false);
+    return make_float2(re, im);
+  } else {
+    float4 q[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      q[i] = make_float4(__uint_as_float(r.q[i].x), __uint_as_float(r.q[i].y), __uint_as_float(r.q[i].z),
+                         __uint_as_float(r.q[i].w));
+    const float s00 = (1.f - fz) * (1.f - fy), s01 = (1.f - fz) * fy, s10 = fz * (1.f - fy), s11 = fz * fy;
+    float2 rv;
+    rv.x = s00 * fmaf(fx, q[0].z - q[0].x, q[0].x) + s01 * fmaf(fx, q[1].z - q[1].x, q[1].x) +
+           s10 * fmaf(fx, q[2].z - q[2].x, q[2].x) + s11 * fmaf(fx, q[3].z - q[3].x, q[3].x);
+    rv.y = s00 * fmaf(fx, q[0].w - q[0].y, q[0].y) + s01 * fmaf(fx, q[1].w - q[1].y, q[1].y) +
+           s10 * fmaf(fx, q[2].w - q[2].y, q[2].y) + s11 * fmaf(fx, q[3].w - q[3].y, q[3].y);
+    return rv;
+  }
+}
+
+// One sample, fetched and combined, in true units (the FP16 table's 2^e removed).
+template <int N, bool H16>
+__device__ __forceinline__ float2 ref_sample(__amdgpu_buffer_rsrc_t rref, float unscale, float rx, float ry,
+                                             float rz, bool inball) {
+  RefCorners<H16> r;
+  ref_place<N, H16>(r, rx, ry, rz, inball);
+  ref_load<H16>(r, rref);
+  float2 v = ref_combine<H16>(r);
+  if constexpr (H16) {
+    v.x *= unscale;
+    v.y *= unscale;
+  }
+  return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -262,11 +369,11 @@ __device__ __forceinline__ float2 ref_trilinear(__amdgpu_buffer_rsrc_t rref, flo
 // Leaves P = conj(R) S of plane kx of particle b in LDS, slot (sz, sy) holding
 // frequency (freq(sz), freq(sy)), the plane's shares of ||S||^2 and ||R||^2 in
 // snorm_part / rnorm_part, and ends on a barrier.  `red` holds 2 * NT / 64 floats.
-template <int N>
+template <int N, bool H16>
 __device__ __forceinline__ void plane_cross_power(
     float2* __restrict__ plane, float2* __restrict__ tw, float* __restrict__ red, int kx, int b,
     const float2* __restrict__ T, const float2* __restrict__ XM, const float* __restrict__ bg_part, float wsum,
-    const float4* __restrict__ refpair, const float* __restrict__ rot, const float* __restrict__ ctf,
+    const void* __restrict__ reftab, float unscale, const float* __restrict__ rot, const float* __restrict__ ctf,
     const float2* __restrict__ twg, float* __restrict__ rnorm_part, float* __restrict__ snorm_part) {
   constexpr int NT = plane_threads<N>(), S = N + 1, H = N / 2 + 1, ZSTEP = NT / N;
   for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
@@ -314,7 +421,7 @@ __device__ __forceinline__ void plane_cross_power(
   const int r2xy = ip * ip + jp * jp;
   const float by = (float)roll_half<N>(ky) - c;
   const float px = m0 * bx + m1 * by + c, py = m3 * bx + m4 * by + c, pz = m6 * bx + m7 * by + c;
-  const __amdgpu_buffer_rsrc_t rref = make_rsrc(refpair, (unsigned)(N * N * (N + 1)) * 16u);
+  const __amdgpu_buffer_rsrc_t rref = make_rsrc(reftab, table_bytes<N>());
   const float* ctfb = ctf ? ctf + (long long)b * N * N * H : nullptr;
   float ns = 0.f, nr = 0.f;
 #pragma unroll 2
@@ -324,7 +431,7 @@ __device__ __forceinline__ void plane_cross_power(
     const int kp = kz < N / 2 ? kz : kz - N;
     const bool inball = kp * kp + r2xy <= (N / 2) * (N / 2);
     const float bz = (float)roll_half<N>(kz) - c;
-    float2 rv = ref_trilinear<N>(rref, fmaf(m2, bz, px), fmaf(m5, bz, py), fmaf(m8, bz, pz), inball);
+    float2 rv = ref_sample<N, H16>(rref, unscale, fmaf(m2, bz, px), fmaf(m5, bz, py), fmaf(m8, bz, pz), inball);
     float cv = 1.f;
     if (ctfb && inball) cv = ctfb[(kz * N + ky) * H + kx];
     rv.x *= cv;
@@ -472,10 +579,10 @@ __global__ __launch_bounds__(slice_threads<N>()) void shift_fwd_xy_kernel(
 // back to T in place.  The cross power runs over wave tiles of 4 kx x 4 ky x
 // 4 kz, whose samples fall in a compact region of the reference (lanes share
 // its cache lines).  Norm partials per workgroup.
-template <int N>
+template <int N, bool H16>
 __global__ __launch_bounds__(kZThreads) void shift_z_kernel(
     float2* __restrict__ T, const float2* __restrict__ XMy, const float* __restrict__ bg_part, float wsum,
-    const float4* __restrict__ refpair, const float* __restrict__ rot, const float* __restrict__ ctf,
+    const void* __restrict__ reftab, float unscale, const float* __restrict__ rot, const float* __restrict__ ctf,
     const float2* __restrict__ twg, float* __restrict__ rnorm_part, float* __restrict__ snorm_part) {
   constexpr int NT = kZThreads, NC = kZKx * kZKy, SC = NC + 1, H = N / 2 + 1, NG = zgroups<N>(),
                 NKB = N / kZKy, ZSTEP = NT / NC;
@@ -530,34 +637,95 @@ __global__ __launch_bounds__(kZThreads) void shift_z_kernel(
   const bool kxok = kx < H;
   const int jp = kx < N / 2 ? kx : kx - H;
   const float bx = (float)(kx + N / 2) - c;
-  const __amdgpu_buffer_rsrc_t rref = make_rsrc(refpair, (unsigned)(N * N * (N + 1)) * 16u);
+  const __amdgpu_buffer_rsrc_t rref = make_rsrc(reftab, table_bytes<N>());
   const float* ctfb = ctf ? ctf + (long long)b * N * N * H : nullptr;
   float ns = 0.f, nr = 0.f;
   constexpr int TILES = 2 * (N / 4);  // (half of the ky block, group of 4 kz)
-#pragma unroll 2
-  for (int t = w; t < TILES; t += NT / 64) {
-    const int ly = 4 * (t & 1) + ty;
-    const int ky = ky0 + ly, kz = 4 * (t >> 1) + tz;
+  constexpr int WAVES = NT / 64;
+  static_assert(TILES % WAVES == 0, "every wave runs the same number of tiles");
+  // FP16 table: the products and ||R||^2 are taken on the scaled samples; the
+  // scales come back once, 1/N^3 and 2^-e on the cross power, 2^-2e on ||R||^2
+  const float pscale = H16 ? inv * unscale : 1.f;
+  struct Tile {
+    RefCorners<H16> rc;
+    float cv;
+    int ly, kz;
+    bool any;  // some lane of the wave samples inside the ball (wave-uniform)
+  };
+  // the CTF factor is loaded before the corners, so that waiting for a tile's
+  // loads never waits for the next tile's
+  auto fetch = [&](int t) {
+    Tile T;
+    T.ly = 4 * (t & 1) + ty;
+    T.kz = 4 * (t >> 1) + tz;
+    const int ky = ky0 + T.ly;
     const int ip = ky < N / 2 ? ky : ky - N;
-    const int kp = kz < N / 2 ? kz : kz - N;
+    const int kp = T.kz < N / 2 ? T.kz : T.kz - N;
     const bool inball = kxok && kp * kp + ip * ip + jp * jp <= (N / 2) * (N / 2);
-    const float by = (float)roll_half<N>(ky) - c, bz = (float)roll_half<N>(kz) - c;
-    const float px = m0 * bx + m1 * by + c, py = m3 * bx + m4 * by + c, pz = m6 * bx + m7 * by + c;
-    float2 rv = ref_trilinear<N>(rref, fmaf(m2, bz, px), fmaf(m5, bz, py), fmaf(m8, bz, pz), inball);
-    float cv = 1.f;
-    if (ctfb && inball) cv = ctfb[(kz * N + ky) * H + kx];
-    rv.x *= cv;
-    rv.y *= cv;
-    if (kxok) {
-      float2* slot = tile + mch::fft::slot_of_freq<N>(kz) * SC + tx * kZKy + ly;
-      float2 s = *slot;
-      s.x *= inv;
-      s.y *= inv;
-      ns += s.x * s.x + s.y * s.y;
-      nr += rv.x * rv.x + rv.y * rv.y;
-      // conj(R) * S
-      *slot = make_float2(rv.x * s.x + rv.y * s.y, rv.x * s.y - rv.y * s.x);
+    // tiles wholly outside the ball (38% of them at N = 96) sample nothing
+    T.any = __builtin_amdgcn_ballot_w64(inball) != 0;
+    T.cv = 1.f;
+    T.rc.o0 = kOutOfRange;
+    if (T.any) {
+      if (ctfb && inball) T.cv = ctfb[(T.kz * N + ky) * H + kx];
+      const float by = (float)roll_half<N>(ky) - c, bz = (float)roll_half<N>(T.kz) - c;
+      const float px = m0 * bx + m1 * by + c, py = m3 * bx + m4 * by + c, pz = m6 * bx + m7 * by + c;
+      ref_place<N, H16>(T.rc, fmaf(m2, bz, px), fmaf(m5, bz, py), fmaf(m8, bz, pz), inball);
     }
+    ref_load<H16>(T.rc, rref);  // outside the branch: the loads stay in flight across it
+    return T;
+  };
+  auto combine = [&](const Tile& T) {
+    if (!T.any) {  // R = 0: the cross power is zero, ||S||^2 still counts
+      if (kxok) {
+        float2* slot = tile + mch::fft::slot_of_freq<N>(T.kz) * SC + tx * kZKy + T.ly;
+        float2 s = *slot;
+        if constexpr (!H16) {
+          s.x *= inv;
+          s.y *= inv;
+        }
+        ns += s.x * s.x + s.y * s.y;
+        *slot = make_float2(0.f, 0.f);
+      }
+      return;
+    }
+    float2 rv = ref_combine<H16>(T.rc);
+    if (ctfb) {
+      rv.x *= T.cv;
+      rv.y *= T.cv;
+    }
+    if (kxok) {
+      float2* slot = tile + mch::fft::slot_of_freq<N>(T.kz) * SC + tx * kZKy + T.ly;
+      float2 s = *slot;
+      if constexpr (H16) {
+        ns += s.x * s.x + s.y * s.y;
+        nr += rv.x * rv.x + rv.y * rv.y;
+        // conj(R) * S
+        *slot = make_float2((rv.x * s.x + rv.y * s.y) * pscale, (rv.x * s.y - rv.y * s.x) * pscale);
+      } else {
+        s.x *= inv;
+        s.y *= inv;
+        ns += s.x * s.x + s.y * s.y;
+        nr += rv.x * rv.x + rv.y * rv.y;
+        // conj(R) * S
+        *slot = make_float2(rv.x * s.x + rv.y * s.y, rv.x * s.y - rv.y * s.x);
+      }
+    }
+  };
+  // two tiles per iteration from two buffers: one tile's loads are in flight
+  // while the other is combined, and no in-flight registers are copied (the
+  // last fetch repeats a tile and is not used)
+  static_assert((TILES / WAVES) % 2 == 0, "tiles per wave in pairs");
+  Tile ta = fetch(w);
+  for (int t = w; t < TILES; t += 2 * WAVES) {
+    const Tile tb = fetch(t + WAVES);
+    combine(ta);
+    ta = fetch(t + 2 * WAVES < TILES ? t + 2 * WAVES : t);
+    combine(tb);
+  }
+  if constexpr (H16) {
+    ns *= inv * inv;
+    nr *= unscale * unscale;
   }
   ns = mch::wave_sum(ns);
   nr = mch::wave_sum(nr);
@@ -682,10 +850,10 @@ __global__ __launch_bounds__(slice_threads<N>()) void shift_inv_xy_kernel(
 // plane kernel contracts y and z of each kx plane to R x R (Q), and
 // shift_up_kx_kernel contracts kx and takes the first argmax.
 // t01: (R, N) phases of the y / z axes in natural frequency order.
-template <int N>
+template <int N, bool H16>
 __global__ __launch_bounds__(plane_threads<N>()) void shift_yz_up_kernel(
     const float2* __restrict__ T, const float2* __restrict__ XM, const float* __restrict__ bg_part,
-    float wsum, const float4* __restrict__ refpair, const float* __restrict__ rot,
+    float wsum, const void* __restrict__ reftab, float unscale, const float* __restrict__ rot,
     const float* __restrict__ ctf, const float2* __restrict__ twg, const float2* __restrict__ t01, int R,
     float2* __restrict__ Qg, float* __restrict__ rnorm_part, float* __restrict__ snorm_part) {
   constexpr int NT = plane_threads<N>(), S = N + 1, H = N / 2 + 1;
@@ -701,8 +869,8 @@ __global__ __launch_bounds__(plane_threads<N>()) void shift_yz_up_kernel(
     const int u = q / N, k = q - (q / N) * N;
     ts[u * N + mch::fft::slot_of_freq<N>(k)] = t01[q];
   }
-  plane_cross_power<N>(plane, tw, red, kx, b, T, XM, bg_part, wsum, refpair, rot, ctf, twg, rnorm_part,
-                       snorm_part);
+  plane_cross_power<N, H16>(plane, tw, red, kx, b, T, XM, bg_part, wsum, reftab, unscale, rot, ctf, twg,
+                            rnorm_part, snorm_part);
   // Y[sz][u1] = sum_sy P[sz][sy] e[u1][sy], 4 x 4 register blocks
   const int nbz = N / 4, nbu = (R + 3) / 4;
   constexpr int MAXB = 2;  // blocks per thread (N / 4 * ceil(R / 4) <= 2 NT for R <= 2 N)
@@ -1021,7 +1189,9 @@ int configure_kernels() {
     return rc;
   if ((rc = raise((const void*)shift_fwd_xy_kernel<N>, fwd_xy_lds, "hipFuncSetAttribute(shift_fwd_xy_kernel)")))
     return rc;
-  if ((rc = raise((const void*)shift_z_kernel<N>, zcol_lds<N>(), "hipFuncSetAttribute(shift_z_kernel)")))
+  if ((rc = raise((const void*)shift_z_kernel<N, false>, zcol_lds<N>(), "hipFuncSetAttribute(shift_z_kernel)")))
+    return rc;
+  if ((rc = raise((const void*)shift_z_kernel<N, true>, zcol_lds<N>(), "hipFuncSetAttribute(shift_z_kernel)")))
     return rc;
   if ((rc = raise((const void*)shift_inv_xy_kernel<N>, slice_lds<N>(), "hipFuncSetAttribute(shift_inv_xy_kernel)")))
     return rc;
@@ -1032,7 +1202,8 @@ void free_plan(ShiftPlan* p) {
   if (!p) return;
   if (p->timing)
     for (int i = 0; i < 5; ++i) (void)hipEventDestroy(p->ev[i]);
-  (void)hipFree(p->refpair); (void)hipFree(p->twid); (void)hipFree(p->lut); (void)hipFree(p->XM);
+  (void)hipFree(p->refpair); (void)hipFree(p->refh16); (void)hipFree(p->twid); (void)hipFree(p->lut);
+  (void)hipFree(p->XM);
   (void)hipFree(p->XMy);
   (void)hipFree(p->T); (void)hipFree(p->rnorm_part); (void)hipFree(p->snorm_part);
   (void)hipFree(p->bg_part); (void)hipFree(p->amax_val); (void)hipFree(p->amax_idx);
@@ -1057,7 +1228,11 @@ int configure_up(int R) {
                    "%zu B of LDS)", R, N, plane_up_lds<N>(R));
     return MCH_ERR_UNSUPPORTED;
   }
-  return mch::hip_check(hipFuncSetAttribute((const void*)shift_yz_up_kernel<N>,
+  rc = mch::hip_check(hipFuncSetAttribute((const void*)shift_yz_up_kernel<N, false>,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)plane_up_lds<N>(R)),
+                     "hipFuncSetAttribute(shift_yz_up_kernel)");
+  if (rc) return rc;
+  return mch::hip_check(hipFuncSetAttribute((const void*)shift_yz_up_kernel<N, true>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize,
                                             (int)plane_up_lds<N>(R)),
                         "hipFuncSetAttribute(shift_yz_up_kernel)");
@@ -1090,9 +1265,14 @@ int launch_search(ShiftPlan* p, const SearchArgs& a, hipStream_t stream) {
                        a.shift_zyx, p->lut, p->lut_lo, p->lut_n, p->twid, p->T, p->bg_part);
     if ((rc = mch::hip_check(hipGetLastError(), "shift_fwd_x_kernel"))) return rc;
     mark(1);
-    hipLaunchKernelGGL(shift_yz_up_kernel<N>, dim3(H * a.B), dim3(plane_threads<N>()), plane_up_lds<N>(R),
-                       stream, p->T, p->XM, p->bg_part, p->wsum, p->refpair, a.rot, a.ctf, p->twid, p->t01, R,
-                       p->Qg, p->rnorm_part, p->snorm_part);
+    if (p->ref_h16)
+      hipLaunchKernelGGL((shift_yz_up_kernel<N, true>), dim3(H * a.B), dim3(plane_threads<N>()),
+                         plane_up_lds<N>(R), stream, p->T, p->XM, p->bg_part, p->wsum, p->refh16,
+                         p->h16_unscale, a.rot, a.ctf, p->twid, p->t01, R, p->Qg, p->rnorm_part, p->snorm_part);
+    else
+      hipLaunchKernelGGL((shift_yz_up_kernel<N, false>), dim3(H * a.B), dim3(plane_threads<N>()),
+                         plane_up_lds<N>(R), stream, p->T, p->XM, p->bg_part, p->wsum, p->refpair, 1.0f, a.rot,
+                         a.ctf, p->twid, p->t01, R, p->Qg, p->rnorm_part, p->snorm_part);
     if ((rc = mch::hip_check(hipGetLastError(), "shift_yz_up_kernel"))) return rc;
     mark(2);
     hipLaunchKernelGGL(shift_up_kx_kernel, dim3(R, a.B), dim3(256), (size_t)2 * H * R * sizeof(float2), stream,
@@ -1110,9 +1290,14 @@ int launch_search(ShiftPlan* p, const SearchArgs& a, hipStream_t stream) {
                      a.shift_zyx, p->lut, p->lut_lo, p->lut_n, p->twid, p->T, p->bg_part);
   if ((rc = mch::hip_check(hipGetLastError(), "shift_fwd_xy_kernel"))) return rc;
   mark(1);
-  hipLaunchKernelGGL(shift_z_kernel<N>, dim3(zparts<N>() * a.B), dim3(kZThreads), zcol_lds<N>(), stream, p->T,
-                     p->XMy, p->bg_part, p->wsum, p->refpair, a.rot, a.ctf, p->twid, p->rnorm_part,
-                     p->snorm_part);
+  if (p->ref_h16)
+    hipLaunchKernelGGL((shift_z_kernel<N, true>), dim3(zparts<N>() * a.B), dim3(kZThreads), zcol_lds<N>(), stream,
+                       p->T, p->XMy, p->bg_part, p->wsum, p->refh16, p->h16_unscale, a.rot, a.ctf, p->twid,
+                       p->rnorm_part, p->snorm_part);
+  else
+    hipLaunchKernelGGL((shift_z_kernel<N, false>), dim3(zparts<N>() * a.B), dim3(kZThreads), zcol_lds<N>(), stream,
+                       p->T, p->XMy, p->bg_part, p->wsum, p->refpair, 1.0f, a.rot, a.ctf, p->twid,
+                       p->rnorm_part, p->snorm_part);
   if ((rc = mch::hip_check(hipGetLastError(), "shift_z_kernel"))) return rc;
   mark(2);
   hipLaunchKernelGGL(shift_inv_xy_kernel<N>, dim3(N, a.B), dim3(slice_threads<N>()), slice_lds<N>(), stream,
@@ -1132,6 +1317,17 @@ int launch_search(ShiftPlan* p, const SearchArgs& a, hipStream_t stream) {
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
+
+int mch_shift_set_reference_precision(void* plan, int fp16) {
+  ShiftPlan* p = (ShiftPlan*)plan;
+  if (!p) { mch::set_error("mch_shift_set_reference_precision: NULL plan"); return MCH_ERR_ARG; }
+  if (fp16 != 0 && fp16 != 1) {
+    mch::set_error("mch_shift_set_reference_precision: fp16 must be 0 or 1 (got %d)", fp16);
+    return MCH_ERR_ARG;
+  }
+  p->ref_h16 = fp16;
+  return MCH_OK;
+}
 
 int mch_shift_plan_set_timing(void* plan, int enable) {
   ShiftPlan* p = (ShiftPlan*)plan;
@@ -1293,22 +1489,61 @@ int mch_shift_plan_create(int N, const void* ref_host /* complex64 (N,N,N) */, f
     tw[k] = make_float2((float)cos(a), (float)sin(a));
   }
   const size_t half = (size_t)N * N * H;
-  // pair table: column xi = 0..N holds (ref[xi - 1], ref[xi]), zero outside the volume
-  const size_t npair = pair_table_size(N);
-  std::vector<float4> pair(npair, make_float4(0.f, 0.f, 0.f, 0.f));
+  // both tables over (z', y', x') = (z + 1, y + 1, x + 1) with a zero border
+  const size_t nent = table_entries(N);
+  const float* rf = (const float*)ref_host;
+  // FP32 pairs: (ref[z][y][x], ref[z][y][x + 1]), (z, y, x) = (z' - 1, y' - 1, x' - 1)
+  std::vector<float4> pair(nent, make_float4(0.f, 0.f, 0.f, 0.f));
   {
-    const float2* r = (const float2*)ref_host;
-    const float2 zero = make_float2(0.f, 0.f);
-    for (size_t zy = 0; zy < (size_t)N * N; ++zy)
-      for (int xi = 0; xi <= N; ++xi) {
-        const float2 a = xi > 0 ? r[zy * N + xi - 1] : zero;
-        const float2 b = xi < N ? r[zy * N + xi] : zero;
-        pair[pair_index(N, (int)(zy / N), (int)(zy % N), xi)] = make_float4(a.x, a.y, b.x, b.y);
-      }
+    auto at = [&](int z, int y, int x) -> float2 {
+      if (z < 0 || z >= N || y < 0 || y >= N || x < 0 || x >= N) return make_float2(0.f, 0.f);
+      const size_t o = (((size_t)z * N + y) * N + x) * 2;
+      return make_float2(rf[o], rf[o + 1]);
+    };
+    for (int zp = 0; zp < N + 2; ++zp)
+      for (int yp = 0; yp < N + 2; ++yp)
+        for (int xp = 0; xp <= N; ++xp) {
+          const float2 a = at(zp - 1, yp - 1, xp - 1), b = at(zp - 1, yp - 1, xp);
+          pair[table_index(N, zp, yp, xp)] = make_float4(a.x, a.y, b.x, b.y);
+        }
+  }
+  // FP16 quads: rows y, y + 1 x columns x, x + 1 of plane z, scaled by 2^e so
+  // that the largest component stays below 2^14 (FP16 keeps 11 significant
+  // bits down to 2^-14)
+  std::vector<uint4> quad(nent, make_uint4(0u, 0u, 0u, 0u));
+  {
+    float maxabs = 0.f;
+    for (size_t i = 0; i < (size_t)2 * N * N * N; ++i) maxabs = fmaxf(maxabs, fabsf(rf[i]));
+    int ex = 0;
+    if (maxabs > 0.f) (void)frexpf(maxabs, &ex);  // maxabs < 2^ex
+    const int e = 14 - ex;
+    const float up = ldexpf(1.f, e);
+    p->h16_unscale = ldexpf(1.f, -e);
+    auto comp = [&](int z, int y, int x, int c) -> _Float16 {
+      if (z < 0 || z >= N || y < 0 || y >= N || x < 0 || x >= N) return (_Float16)0.f;
+      return (_Float16)(rf[(((size_t)z * N + y) * N + x) * 2 + c] * up);
+    };
+    auto pack = [](_Float16 lo, _Float16 hi) {
+      return (unsigned)__builtin_bit_cast(unsigned short, lo) |
+             ((unsigned)__builtin_bit_cast(unsigned short, hi) << 16);
+    };
+    for (int zp = 0; zp < N + 2; ++zp)
+      for (int yp = 0; yp < N + 2; ++yp)
+        for (int xp = 0; xp <= N; ++xp) {
+          const int z = zp - 1, y = yp - 1, x = xp - 1;
+          uint4 q;
+          q.x = pack(comp(z, y, x, 0), comp(z, y, x + 1, 0));
+          q.y = pack(comp(z, y, x, 1), comp(z, y, x + 1, 1));
+          q.z = pack(comp(z, y + 1, x, 0), comp(z, y + 1, x + 1, 0));
+          q.w = pack(comp(z, y + 1, x, 1), comp(z, y + 1, x + 1, 1));
+          quad[table_index(N, zp, yp, xp)] = q;
+        }
   }
 #define MCH_TRY(expr, what) do { if ((rc = mch::hip_check((expr), what)) != MCH_OK) goto fail; } while (0)
-  MCH_TRY(hipMalloc(&p->refpair, npair * sizeof(float4)), "hipMalloc(refpair)");
-  MCH_TRY(hipMemcpy(p->refpair, pair.data(), npair * sizeof(float4), hipMemcpyHostToDevice), "copy ref");
+  MCH_TRY(hipMalloc(&p->refpair, nent * sizeof(float4)), "hipMalloc(refpair)");
+  MCH_TRY(hipMemcpy(p->refpair, pair.data(), nent * sizeof(float4), hipMemcpyHostToDevice), "copy ref");
+  MCH_TRY(hipMalloc(&p->refh16, nent * sizeof(uint4)), "hipMalloc(refh16)");
+  MCH_TRY(hipMemcpy(p->refh16, quad.data(), nent * sizeof(uint4), hipMemcpyHostToDevice), "copy ref quads");
   MCH_TRY(hipMalloc(&p->twid, N * sizeof(float2)), "hipMalloc(twid)");
   MCH_TRY(hipMemcpy(p->twid, tw.data(), N * sizeof(float2), hipMemcpyHostToDevice), "copy twid");
   MCH_TRY(hipMalloc(&p->lut, lut.size() * sizeof(float)), "hipMalloc(lut)");
