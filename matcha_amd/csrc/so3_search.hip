@@ -488,6 +488,8 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
   const float hc = cosf(0.5f * beta), hs = sinf(0.5f * beta);
   const float lc = log2f(fabsf(hc)), ls = log2f(fabsf(hs));
   const bool cneg = hc < 0.f, sneg = hs < 0.f;
+  const bool ratios = fabsf(hc) >= 0x1p-20f && fabsf(hs) >= 0x1p-20f;
+  const float rcs = hc / hs, rsc = hs / hc;
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(sp), (short)0, (int)sp_bytes, 0x00020000);
   const int P = (Li + 1) * (Li + 1);
@@ -519,12 +521,28 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
       return v;
     };
     const float sgn = (bb & 1) ? -1.f : 1.f;
-    float d = sgn * term(a, bb);
-    float dp = sgn * 0.5f * ((float)bb * term(a + 1, bb - 1) - (float)a * term(a - 1, bb + 1));
+    // the derivative seeds need term(a +- k, bb -+ k) = term(a, bb) (c / s)^{+-k},
+    // k = 1, 2: away from beta = 0 and pi (|c|, |s| >= 2^-20, a workgroup-uniform
+    // test) by ratios, so that a task costs one exponential; every product then
+    // stays below 2^40 (|term(a, bb)| <= 1), also where its coefficient is zero
+    const float t0 = term(a, bb);
+    float tp1, tm1, tp2, tm2;  // term(a + 1, bb - 1), term(a - 1, bb + 1), term(a + 2, bb - 2), term(a - 2, bb + 2)
+    if (ratios) {
+      tp1 = t0 * rcs;
+      tm1 = t0 * rsc;
+      tp2 = tp1 * rcs;
+      tm2 = tm1 * rsc;
+    } else {
+      tp1 = term(a + 1, bb - 1);
+      tm1 = term(a - 1, bb + 1);
+      tp2 = term(a + 2, bb - 2);
+      tm2 = term(a - 2, bb + 2);
+    }
+    float d = sgn * t0;
+    float dp = sgn * 0.5f * ((float)bb * tp1 - (float)a * tm1);
     float dpp = sgn * 0.25f *
-                ((float)(bb * (bb - 1)) * term(a + 2, bb - 2) -
-                 (float)(bb * (a + 1) + a * (bb + 1)) * term(a, bb) +
-                 (float)(a * (a - 1)) * term(a - 2, bb + 2));
+                ((float)(bb * (bb - 1)) * tp2 - (float)(bb * (a + 1) + a * (bb + 1)) * t0 +
+                 (float)(a * (a - 1)) * tm2);
     float pd = 0.f, pdp = 0.f, pdpp = 0.f, Cprev = 0.f;
     // U[i] = sum_l sigma'_i d, U1 with d', U2 with d'' (i: merged images (mp, m), (m, mp))
     float2 U[2], U1[2], U2[2];
