@@ -53,7 +53,8 @@ GEMM16_FETCH_BYTES = {("box96_L32", 64, 4, 1): 587089 * 1024 * 2, ("box96_L32", 
 # HBM bytes (FETCH_SIZE x 2.0 + WRITE_SIZE) per launch of the shift search's
 # column pass, keyed by (workload, batch, reference table precision); measured
 # like the GEMM's.
-SHIFT_Z_TRAFFIC_BYTES = {("box96_L32", 512, "fp32"): 3553865 * 1024 * 2 + 2102084 * 1024}
+SHIFT_Z_TRAFFIC_BYTES = {("box96_L32", 512, "fp32"): 2958538 * 1024 * 2 + 1904087 * 1024,
+                         ("box96_L32", 512, "fp16"): 2974177 * 1024 * 2 + 1947942 * 1024}
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
