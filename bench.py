@@ -54,7 +54,7 @@ GEMM16_FETCH_BYTES = {("box96_L32", 64, 4, 1): 587089 * 1024 * 2, ("box96_L32", 
 # column pass, keyed by (workload, batch, reference table precision); measured
 # like the GEMM's.
 SHIFT_Z_TRAFFIC_BYTES = {("box96_L32", 512, "fp32"): 2958538 * 1024 * 2 + 1904087 * 1024,
-                         ("box96_L32", 512, "fp16"): 2974177 * 1024 * 2 + 1947942 * 1024}
+                         ("box96_L32", 512, "reduced"): 2411520 * 1024 * 2 + 913110 * 1024}
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
@@ -76,8 +76,9 @@ def parse():
     p.add_argument("--batch", type=int, default=None, help="plain-path batch (default: the workload's)")
     p.add_argument("--tuned-batch", type=int, default=None, help="tuned-path batch (default: the workload's)")
     p.add_argument("--tuned-precision", default="bf16x3", choices=["bf16x3", "fp16x3"])
-    p.add_argument("--tuned-shift-precision", default="fp16", choices=["fp32", "fp16"],
-                   help="table the tuned path's shift search samples the rotated reference from")
+    p.add_argument("--tuned-shift-precision", default="reduced", choices=["fp32", "reduced"],
+                   help="storage precision of the tuned path's shift search (reduced: FP16 reference table, "
+                        "BF16 work spectrum)")
     p.add_argument("--tuned-streams", type=int, default=2,
                    help="streams the tuned path's batches alternate over (one plan set each)")
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP32 path")
@@ -190,7 +191,7 @@ def run_path(name, torch, dev, w, args, pool_vols, pool_R, pool_t, precision, B,
     ev[3].record(stream)
     torch.cuda.synchronize()
     return {
-        "precision": precision, "shift_reference": sm.reference_precision, "batch": B, "streams": nstreams,
+        "precision": precision, "shift_precision": sm.precision, "batch": B, "streams": nstreams,
         "volumes_per_s": vps,
         "ms_per_step": elapsed / args.steps * 1e3,
         "median_err_deg": float(np.median(errs)), "p90_err_deg": float(np.percentile(errs, 90)),
@@ -216,6 +217,32 @@ def time_gemm(torch, dev, ccm, vols, precision, reps=20):
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def time_march(torch, dev, ccm, matcher, vols, reps=5):
+    """Average duration of one Newton march over the grid candidates of `vols`
+    (`Matcha.march`: the sigma repack and the march kernel, HIP events on the
+    current stream)."""
+    sigma, _ = ccm.get_sigma(vols, max_degree=matcher.Lmax)
+    a, b, g, _ = matcher.grid_candidates(sigma[:vols.shape[0]])
+    matcher.march(sigma[:vols.shape[0]], a, b, g)
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        matcher.march(sigma[:vols.shape[0]], a, b, g)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def march_flops(levels, num_steps, candidates, batch):
+    """FP32 operations of the fused march's shell work: per fundamental pair
+    (m' >= |m|) and shell l >= m', the three-term recurrences of d, d', d''
+    (29) and the twelve accumulating products with sigma (24), for every
+    evaluation (num_steps + 1 per level) of every candidate."""
+    pair_shells = lambda L: sum((2 * m + 1) * (L - m + 1) for m in range(L + 1))
+    return 53.0 * (num_steps + 1) * sum(pair_shells(L) for L in levels) * candidates * batch
 
 
 def time_shift_kernels(torch, dev, sm, vols, reps=10):
@@ -502,7 +529,7 @@ def main():
                                          "oversampling_factor_K": w["K"]})
         sh = ShiftMatcher({"N": w["N"], "num_subtomograms_per_batch": B, "shift_upsample_factor": 1,
                            "shift_search_radius": w["max_shift"], "voxel_size": 1.0,
-                           "shift_reference_precision": shift_precision}, dev)
+                           "shift_precision": shift_precision}, dev)
         sh.set_reference(wl.template.cpu().numpy())
         return c, m, sh
 
@@ -530,6 +557,8 @@ def main():
     # algorithmic flops = the three split products
     fle_ = ccm.fle
     gemm_t = time_gemm(torch, dev, ccm, pool_vols[:Bt], args.tuned_precision)
+    march_t = time_march(torch, dev, ccm, matcher, pool_vols[:Bt])
+    march_fl = march_flops(matcher.levels, matcher.num_steps, matcher.candidates, Bt)
     gemm_plain = time_gemm(torch, dev, plain_plans[0], pool_vols[:Bp], "fp32") if plain else None
     w_bytes = 4.0 * fle_.Ntot * fle_.Kpad          # W^T hi + lo, 2 + 2 B per element
     x_bytes = 4.0 * 8 * (2 * Bt) * fle_.Kpad       # X hi + lo of the 8 classes
@@ -547,19 +576,20 @@ def main():
     gemm_hbm = achieved / 8000.0
     gemm_mfma = flops16 / gemm_t / 1e12 / 2500.0
 
-    # the dominant kernel: the shift search's column pass (one block of 4 kx x
+    # the shift search's column pass (one block of 4 kx x
     # 8 ky columns: forward z FFT, rotated-reference sampling, cross power,
     # inverse z FFT).  Algorithmic bytes: the half spectrum read and written
-    # (B H N^2 x 8 B each way) + the background spectrum XMy and the reference
-    # table once ((N + 2)^2 (N + 1) entries of 16 B, FP32 pairs or FP16 quads);
+    # (B H N^2 elements each way, 8 B complex FP32 or 4 B complex BF16 in the
+    # reduced mode) + the background spectrum XMy and the reference table once
+    # ((N + 2)^2 (N + 1) entries of 16 B, FP32 pairs or FP16 quads);
     # algorithmic flops: 2 passes of H N length-N FFTs
     # (5 N log2 N each) + 38 per sampled frequency (trilinear complex sample,
     # norms, product)
     shift_ms = time_shift_kernels(torch, dev, sm, pool_vols[:Bt])
     N, H = w["N"], w["N"] // 2 + 1
     z_t = shift_ms[1] * 1e-3
-    z_h16 = args.tuned_shift_precision == "fp16"
-    z_bytes = 2.0 * Bt * H * N * N * 8 + H * N * N * 8 + (N + 2) ** 2 * (N + 1) * 16
+    z_red = args.tuned_shift_precision == "reduced"
+    z_bytes = 2.0 * Bt * H * N * N * (4 if z_red else 8) + H * N * N * 8 + (N + 2) ** 2 * (N + 1) * 16
     z_flops = Bt * H * (2 * N * 5.0 * N * math.log2(N) + N * N * 38.0)
     z_gbs = z_bytes / z_t / 1e9
 
@@ -592,14 +622,24 @@ def main():
             "gemm_tiles": tiles,
             "setup_s": t_setup,
         },
-        "roofline": {"bound": "hbm", "achieved": z_gbs, "peak": 8000.0, "unit": "GB/s",
-                     "frac": z_gbs / 8000.0,
-                     "traffic": SHIFT_Z_TRAFFIC_BYTES.get((args.workload, Bt, args.tuned_shift_precision)),
-                     "kernel": f"shift_z_kernel<{N}, {'true' if z_h16 else 'false'}>", "avg_ms": z_t * 1e3, "bytes_per_launch": z_bytes,
-                     "flops_per_launch": z_flops, "fp32_vector_tflops": z_flops / z_t / 1e12,
-                     "fp32_vector_frac": z_flops / z_t / 157.3e12,
-                     "share_of_step": 2 * z_t * 1e3 / tuned["ms_per_step"],
-                     "shift_kernels_ms": dict(zip(("fwd_xy", "z", "inv_xy", "final"), map(float, shift_ms)))},
+        # the dominant kernel (20% of the step): the fused Newton march, bound by
+        # FP32 vector issue; achieved = the fused algorithm's shell flops per
+        # launch (march_flops) / its live duration; peak = the published FP32
+        # rate, which counts packed FP32 (frac_nonpacked: against half of it,
+        # the rate without packed instructions, which this build does not use)
+        "roofline": {"bound": "fp32_vector", "achieved": march_fl / march_t / 1e12, "peak": 157.3,
+                     "unit": "TFLOP/s", "frac": march_fl / march_t / 1e12 / 157.3, "traffic": None,
+                     "frac_nonpacked": march_fl / march_t / 1e12 / 78.65,
+                     "kernel": "so3_march_kernel (with so3_repack_kernel)", "avg_ms": march_t * 1e3,
+                     "flops_per_launch": march_fl, "share_of_step": 2 * march_t * 1e3 / tuned["ms_per_step"]},
+        "roofline_shift_z": {"bound": "hbm", "achieved": z_gbs, "peak": 8000.0, "unit": "GB/s",
+                             "frac": z_gbs / 8000.0,
+                             "traffic": SHIFT_Z_TRAFFIC_BYTES.get((args.workload, Bt, args.tuned_shift_precision)),
+                             "kernel": f"shift_z_kernel<{N}, {'true' if z_red else 'false'}>", "avg_ms": z_t * 1e3,
+                             "bytes_per_launch": z_bytes, "flops_per_launch": z_flops,
+                             "fp32_vector_tflops": z_flops / z_t / 1e12, "fp32_vector_frac": z_flops / z_t / 157.3e12,
+                             "share_of_step": 2 * z_t * 1e3 / tuned["ms_per_step"],
+                             "shift_kernels_ms": dict(zip(("fwd_xy", "z", "inv_xy", "final"), map(float, shift_ms)))},
         "roofline_gemm16": {"bound": "mfma" if gemm_mfma > gemm_hbm else "hbm",
                             "achieved": flops16 / gemm_t / 1e12 if gemm_mfma > gemm_hbm else achieved,
                             "peak": 2500.0 if gemm_mfma > gemm_hbm else 8000.0,

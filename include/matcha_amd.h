@@ -189,15 +189,18 @@ int mch_shift_search(void* plan, const float* subtomos, long long ld_vol, int B,
                      const float* shift_zyx, const float* rot, const float* ctf, float* out_shift,
                      float* out_score, hipStream_t stream);
 
-/* Table the searches sample the rotated reference from (no reference
- * counterpart: the reference rotates its complex64 spectrum with grid_sample,
- * src/matcha/core/ShiftMatcher.py:491-537).  fp16 = 0 (default): complex FP32
- * pairs, the reference's arithmetic.  fp16 = 1: FP16 quads scaled by a power
- * of two, two 16-byte gathers per trilinear sample instead of four, weights
- * rounded to FP16 and products accumulated in FP32 (a reduced-precision mode
- * for throughput; shifts and scores then agree with the FP32 table to the
- * tolerance its tests state, not bit for bit). */
-int mch_shift_set_reference_precision(void* plan, int fp16);
+/* Storage precision of the shift searches (no reference counterpart: the
+ * reference keeps its rotated reference and the cross-correlation spectra in
+ * complex64, src/matcha/core/ShiftMatcher.py:171-282,491-537).  reduced = 0
+ * (default): complex FP32 throughout, the reference's values.  reduced = 1:
+ * the rotated reference sampled from FP16 quads scaled by a power of two (two
+ * 16-byte gathers per trilinear sample instead of four, weights rounded to
+ * FP16, products accumulated in FP32) and, on the integer-lag path, the work
+ * spectrum between passes stored as complex BF16 (half the bytes); all
+ * arithmetic stays FP32.  A throughput mode: lags agree with the FP32 path
+ * where the correlation has a clear peak, scores to about 2e-3, and lags of
+ * near-tied correlation values may differ by one voxel. */
+int mch_shift_set_precision(void* plan, int reduced);
 
 /* Per-kernel timing of mch_shift_search (a measurement aid, no reference
  * counterpart): when enabled, every search records events around its four

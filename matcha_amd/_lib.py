@@ -59,7 +59,7 @@ _SIGNATURES = [
     ("mch_shift_search", c_int, [c_void_p, c_void_p, c_ll, c_int, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),
     ("mch_shift_plan_set_timing", c_int, [c_void_p, c_int]),
-    ("mch_shift_set_reference_precision", c_int, [c_void_p, c_int]),
+    ("mch_shift_set_precision", c_int, [c_void_p, c_int]),
     ("mch_shift_kernel_ms", c_int, [c_void_p, ctypes.POINTER(ctypes.c_float)]),
     ("mch_extract_patches", c_int, [c_void_p, c_ll, c_int, c_int, c_void_p, c_void_p, c_ll, c_void_p]),
     ("mch_euler_to_shift_rot", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),

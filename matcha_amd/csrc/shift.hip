@@ -28,6 +28,7 @@
 #include "fft_lds.hpp"
 
 #include <math.h>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -51,7 +52,7 @@ struct ShiftPlan {
   float4* refpair;  // FP32 pairs of the centred reference (table_index)
   uint4* refh16;     // FP16 quads of the same reference, scaled by 2^e (table_index)
   float h16_unscale;  // 2^-e
-  int ref_h16;      // 1: the searches sample refh16, 0 (default): refpair
+  int reduced;      // 1: reduced precision (refh16, BF16 work spectrum), 0 (default): FP32
   float2* twid;     // (N) e^{-2 pi i j / N}
   float* lut;       // (lut_n)
   float2* XM;       // (H, N, N): x-spectrum of the rolled background weight, [kx][z][y]
@@ -241,6 +242,28 @@ constexpr unsigned table_ybrick() { return (unsigned)(N + 1) * 64u; }
 
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 typedef unsigned v4u __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// Work-spectrum elements: complex FP32, or (RED) complex BF16 in 4 bytes,
+// rounded to nearest even on store and widened exactly on load.
+template <bool RED>
+using SpecElem = typename std::conditional<RED, unsigned, float2>::type;
+template <bool RED>
+__device__ __forceinline__ SpecElem<RED> spec_pack(float2 v) {
+  if constexpr (RED) {
+    const bf16x2 h = {(__bf16)v.x, (__bf16)v.y};
+    return __builtin_bit_cast(unsigned, h);
+  } else {
+    return v;
+  }
+}
+template <bool RED>
+__device__ __forceinline__ float2 spec_unpack(SpecElem<RED> e) {
+  if constexpr (RED)
+    return make_float2(__uint_as_float(e << 16), __uint_as_float(e & 0xffff0000u));
+  else
+    return e;
+}
 
 // Byte offset of entry (z', y', x') from 24-bit products (every index of the
 // table stays below 2^24).
@@ -483,11 +506,11 @@ template <int N> constexpr int zparts() { return zgroups<N>() * (N / kZKy); }
 // Forward transform along x and y of one (b, z) slice.  Loads and the mask
 // split as shift_fwd_x_kernel; the x spectra are unpacked in registers,
 // re-laid out as lines kx, transformed along y and stored with ky natural.
-template <int N>
+template <int N, bool RED>
 __global__ __launch_bounds__(slice_threads<N>()) void shift_fwd_xy_kernel(
     const float* __restrict__ vols, long long ldv, const float* __restrict__ shift_zyx,
     const float* __restrict__ lutg, int lut_lo, int lut_n, const float2* __restrict__ twg,
-    float2* __restrict__ T, float* __restrict__ bg_part) {
+    SpecElem<RED>* __restrict__ T, float* __restrict__ bg_part) {
   constexpr int NT = slice_threads<N>(), NL = N / 2, S = N + 1, H = N / 2 + 1, YSTEP = NT / N;
   constexpr int KXP = (H + YSTEP - 1) / YSTEP;
   static_assert(NT % N == 0 && N % YSTEP == 0, "column split");
@@ -568,9 +591,9 @@ __global__ __launch_bounds__(slice_threads<N>()) void shift_fwd_xy_kernel(
   }
   __syncthreads();
   mch::fft::forward<N, H, NT, -1>(buf, S, 1, tw);  // along y: lines are kx
-  float2* out = T + (long long)b * H * N * N + (long long)z * N;
+  SpecElem<RED>* out = T + (long long)b * H * N * N + (long long)z * N;
   const int ky = threadIdx.x % N, sy = mch::fft::slot_of_freq<N>(ky);
-  for (int kx = kx0; kx < H; kx += YSTEP) out[(long long)kx * N * N + ky] = buf[kx * S + sy];
+  for (int kx = kx0; kx < H; kx += YSTEP) out[(long long)kx * N * N + ky] = spec_pack<RED>(buf[kx * S + sy]);
 }
 
 // One workgroup per (b, group of 4 kx, block of 8 ky): T + bg * XMy, FFT along z
@@ -581,7 +604,7 @@ __global__ __launch_bounds__(slice_threads<N>()) void shift_fwd_xy_kernel(
 // its cache lines).  Norm partials per workgroup.
 template <int N, bool H16>
 __global__ __launch_bounds__(kZThreads) void shift_z_kernel(
-    float2* __restrict__ T, const float2* __restrict__ XMy, const float* __restrict__ bg_part, float wsum,
+    SpecElem<H16>* __restrict__ T, const float2* __restrict__ XMy, const float* __restrict__ bg_part, float wsum,
     const void* __restrict__ reftab, float unscale, const float* __restrict__ rot, const float* __restrict__ ctf,
     const float2* __restrict__ twg, float* __restrict__ rnorm_part, float* __restrict__ snorm_part) {
   constexpr int NT = kZThreads, NC = kZKx * kZKy, SC = NC + 1, H = N / 2 + 1, NG = zgroups<N>(),
@@ -605,7 +628,7 @@ __global__ __launch_bounds__(kZThreads) void shift_z_kernel(
   const int lc = threadIdx.x % NC, zb = threadIdx.x / NC;
   const int lkx = kx0 + lc / kZKy, lky = ky0 + lc % kZKy;
   const bool lok = lkx < H;  // the last group of kx can be partial
-  float2* col = T + ((long long)b * H + (lok ? lkx : 0)) * N * N + lky;
+  SpecElem<H16>* col = T + ((long long)b * H + (lok ? lkx : 0)) * N * N + lky;
   const float2* mcol = XMy + (long long)(lok ? lkx : 0) * N * N + lky;
   {
     constexpr int PER = N / ZSTEP;
@@ -615,7 +638,7 @@ __global__ __launch_bounds__(kZThreads) void shift_z_kernel(
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int zz = zb + (base + u) * ZSTEP;
-        a[u] = lok ? col[zz * N] : make_float2(0.f, 0.f);
+        a[u] = lok ? spec_unpack<H16>(col[zz * N]) : make_float2(0.f, 0.f);
         mm[u] = lok ? mcol[zz * N] : make_float2(0.f, 0.f);
       }
 #pragma unroll
@@ -742,7 +765,7 @@ __global__ __launch_bounds__(kZThreads) void shift_z_kernel(
   }
   mch::fft::inverse<N, NC, NT, 1>(tile, 1, SC, tw);  // along z, natural order out
   if (lok)
-    for (int zz = zb; zz < N; zz += ZSTEP) col[zz * N] = tile[zz * SC + lc];
+    for (int zz = zb; zz < N; zz += ZSTEP) col[zz * N] = spec_pack<H16>(tile[zz * SC + lc]);
 }
 
 // Inverse transform of one (b, z) slice along y, then along x of the
@@ -751,9 +774,9 @@ __global__ __launch_bounds__(kZThreads) void shift_z_kernel(
 // y[n] / 2 with y the inverse DFT of the Hermitian row Y (Y_0 = 2 Re X_0,
 // Y_{N/2} = 2 Re X_{N/2}, Y_k = X_k, Y_{N-k} = conj X_k).  Two such real rows
 // go through one complex FFT as Y1 + i Y2: rows j and j + N/2 of the slice.
-template <int N>
+template <int N, bool RED>
 __global__ __launch_bounds__(slice_threads<N>()) void shift_inv_xy_kernel(
-    const float2* __restrict__ T, const float2* __restrict__ twg, float* __restrict__ amax_val,
+    const SpecElem<RED>* __restrict__ T, const float2* __restrict__ twg, float* __restrict__ amax_val,
     int* __restrict__ amax_idx) {
   constexpr int NT = slice_threads<N>(), NL = N / 2, S = N + 1, H = N / 2 + 1, YSTEP = NT / N;
   constexpr int KXP = (H + YSTEP - 1) / YSTEP, ITEMS = NL * H, IP = (ITEMS + NT - 1) / NT;
@@ -765,7 +788,7 @@ __global__ __launch_bounds__(slice_threads<N>()) void shift_inv_xy_kernel(
   __shared__ int ri[NT / 64];
   const int z = blockIdx.x, b = blockIdx.y;
   for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
-  const float2* in = T + (long long)b * H * N * N + (long long)z * N;
+  const SpecElem<RED>* in = T + (long long)b * H * N * N + (long long)z * N;
   {
     // rows kx of the slice, ky natural -> slot order for the inverse along y
     const int ky = threadIdx.x % N, sy = mch::fft::slot_of_freq<N>(ky), kx0 = threadIdx.x / N;
@@ -773,7 +796,7 @@ __global__ __launch_bounds__(slice_threads<N>()) void shift_inv_xy_kernel(
 #pragma unroll
     for (int i = 0; i < KXP; ++i) {
       const int kx = kx0 + i * YSTEP;
-      if (kx < H) v[i] = in[(long long)kx * N * N + ky];
+      if (kx < H) v[i] = spec_unpack<RED>(in[(long long)kx * N * N + ky]);
     }
 #pragma unroll
     for (int i = 0; i < KXP; ++i) {
@@ -1187,13 +1210,17 @@ int configure_kernels() {
   };
   if ((rc = raise((const void*)shift_fwd_x_kernel<N>, fwd_lds, "hipFuncSetAttribute(shift_fwd_x_kernel)")))
     return rc;
-  if ((rc = raise((const void*)shift_fwd_xy_kernel<N>, fwd_xy_lds, "hipFuncSetAttribute(shift_fwd_xy_kernel)")))
+  if ((rc = raise((const void*)shift_fwd_xy_kernel<N, false>, fwd_xy_lds, "hipFuncSetAttribute(shift_fwd_xy_kernel)")))
+    return rc;
+  if ((rc = raise((const void*)shift_fwd_xy_kernel<N, true>, fwd_xy_lds, "hipFuncSetAttribute(shift_fwd_xy_kernel)")))
     return rc;
   if ((rc = raise((const void*)shift_z_kernel<N, false>, zcol_lds<N>(), "hipFuncSetAttribute(shift_z_kernel)")))
     return rc;
   if ((rc = raise((const void*)shift_z_kernel<N, true>, zcol_lds<N>(), "hipFuncSetAttribute(shift_z_kernel)")))
     return rc;
-  if ((rc = raise((const void*)shift_inv_xy_kernel<N>, slice_lds<N>(), "hipFuncSetAttribute(shift_inv_xy_kernel)")))
+  if ((rc = raise((const void*)shift_inv_xy_kernel<N, false>, slice_lds<N>(), "hipFuncSetAttribute(shift_inv_xy_kernel)")))
+    return rc;
+  if ((rc = raise((const void*)shift_inv_xy_kernel<N, true>, slice_lds<N>(), "hipFuncSetAttribute(shift_inv_xy_kernel)")))
     return rc;
   return MCH_OK;
 }
@@ -1265,7 +1292,7 @@ int launch_search(ShiftPlan* p, const SearchArgs& a, hipStream_t stream) {
                        a.shift_zyx, p->lut, p->lut_lo, p->lut_n, p->twid, p->T, p->bg_part);
     if ((rc = mch::hip_check(hipGetLastError(), "shift_fwd_x_kernel"))) return rc;
     mark(1);
-    if (p->ref_h16)
+    if (p->reduced)
       hipLaunchKernelGGL((shift_yz_up_kernel<N, true>), dim3(H * a.B), dim3(plane_threads<N>()),
                          plane_up_lds<N>(R), stream, p->T, p->XM, p->bg_part, p->wsum, p->refh16,
                          p->h16_unscale, a.rot, a.ctf, p->twid, p->t01, R, p->Qg, p->rnorm_part, p->snorm_part);
@@ -1285,25 +1312,29 @@ int launch_search(ShiftPlan* p, const SearchArgs& a, hipStream_t stream) {
     mark(4);
     return rc;
   }
-  hipLaunchKernelGGL(shift_fwd_xy_kernel<N>, dim3(N, a.B), dim3(slice_threads<N>()),
-                     slice_lds<N>() + (size_t)p->lut_n * sizeof(float), stream, a.subtomos, a.ld_vol,
-                     a.shift_zyx, p->lut, p->lut_lo, p->lut_n, p->twid, p->T, p->bg_part);
-  if ((rc = mch::hip_check(hipGetLastError(), "shift_fwd_xy_kernel"))) return rc;
-  mark(1);
-  if (p->ref_h16)
-    hipLaunchKernelGGL((shift_z_kernel<N, true>), dim3(zparts<N>() * a.B), dim3(kZThreads), zcol_lds<N>(), stream,
-                       p->T, p->XMy, p->bg_part, p->wsum, p->refh16, p->h16_unscale, a.rot, a.ctf, p->twid,
-                       p->rnorm_part, p->snorm_part);
-  else
-    hipLaunchKernelGGL((shift_z_kernel<N, false>), dim3(zparts<N>() * a.B), dim3(kZThreads), zcol_lds<N>(), stream,
-                       p->T, p->XMy, p->bg_part, p->wsum, p->refpair, 1.0f, a.rot, a.ctf, p->twid,
-                       p->rnorm_part, p->snorm_part);
-  if ((rc = mch::hip_check(hipGetLastError(), "shift_z_kernel"))) return rc;
-  mark(2);
-  hipLaunchKernelGGL(shift_inv_xy_kernel<N>, dim3(N, a.B), dim3(slice_threads<N>()), slice_lds<N>(), stream,
-                     p->T, p->twid, p->amax_val, p->amax_idx);
-  if ((rc = mch::hip_check(hipGetLastError(), "shift_inv_xy_kernel"))) return rc;
-  mark(3);
+  // reduced precision: the FP16 reference table and a BF16 work spectrum
+  auto passes = [&](auto red) -> int {
+    constexpr bool RED = decltype(red)::value;
+    SpecElem<RED>* T = reinterpret_cast<SpecElem<RED>*>(p->T);
+    hipLaunchKernelGGL((shift_fwd_xy_kernel<N, RED>), dim3(N, a.B), dim3(slice_threads<N>()),
+                       slice_lds<N>() + (size_t)p->lut_n * sizeof(float), stream, a.subtomos, a.ld_vol,
+                       a.shift_zyx, p->lut, p->lut_lo, p->lut_n, p->twid, T, p->bg_part);
+    int e;
+    if ((e = mch::hip_check(hipGetLastError(), "shift_fwd_xy_kernel"))) return e;
+    mark(1);
+    hipLaunchKernelGGL((shift_z_kernel<N, RED>), dim3(zparts<N>() * a.B), dim3(kZThreads), zcol_lds<N>(), stream,
+                       T, p->XMy, p->bg_part, p->wsum, RED ? (const void*)p->refh16 : (const void*)p->refpair,
+                       RED ? p->h16_unscale : 1.0f, a.rot, a.ctf, p->twid, p->rnorm_part, p->snorm_part);
+    if ((e = mch::hip_check(hipGetLastError(), "shift_z_kernel"))) return e;
+    mark(2);
+    hipLaunchKernelGGL((shift_inv_xy_kernel<N, RED>), dim3(N, a.B), dim3(slice_threads<N>()), slice_lds<N>(),
+                       stream, T, p->twid, p->amax_val, p->amax_idx);
+    if ((e = mch::hip_check(hipGetLastError(), "shift_inv_xy_kernel"))) return e;
+    mark(3);
+    return MCH_OK;
+  };
+  rc = p->reduced ? passes(std::true_type()) : passes(std::false_type());
+  if (rc) return rc;
   hipLaunchKernelGGL(shift_final_kernel, dim3(a.B), dim3(256), 0, stream, p->amax_val, p->amax_idx,
                      p->ntile_x, p->rnorm_part, p->snorm_part, zparts<N>(), N, a.out_shift, a.out_score);
   rc = mch::hip_check(hipGetLastError(), "shift_final_kernel");
@@ -1318,14 +1349,14 @@ int launch_search(ShiftPlan* p, const SearchArgs& a, hipStream_t stream) {
 // ---------------------------------------------------------------------------
 extern "C" {
 
-int mch_shift_set_reference_precision(void* plan, int fp16) {
+int mch_shift_set_precision(void* plan, int reduced) {
   ShiftPlan* p = (ShiftPlan*)plan;
-  if (!p) { mch::set_error("mch_shift_set_reference_precision: NULL plan"); return MCH_ERR_ARG; }
-  if (fp16 != 0 && fp16 != 1) {
-    mch::set_error("mch_shift_set_reference_precision: fp16 must be 0 or 1 (got %d)", fp16);
+  if (!p) { mch::set_error("mch_shift_set_precision: NULL plan"); return MCH_ERR_ARG; }
+  if (reduced != 0 && reduced != 1) {
+    mch::set_error("mch_shift_set_precision: reduced must be 0 or 1 (got %d)", reduced);
     return MCH_ERR_ARG;
   }
-  p->ref_h16 = fp16;
+  p->reduced = reduced;
   return MCH_OK;
 }
 

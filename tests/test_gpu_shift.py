@@ -258,10 +258,10 @@ def test_upsampled_refinement_unsupported_grid_raises(gpu_device):
         sm.set_reference(np.zeros((32, 32, 32), dtype=np.float32))
 
 
-def _fp16_case(rng, N, B, ctf):
+def _reduced_case(rng, N, B, ctf, rotate):
     from matcha_amd.core.ShiftMatcher import tracker_to_matrices
     tmpl = fle.blob_volume(rng, N, max(12, N // 2), sigma_range=(1.0, 4.0))
-    q = _rand_quats(rng, B)
+    q = _rand_quats(rng, B) if rotate else np.tile([1.0, 0.0, 0.0, 0.0], (B, 1))
     rot = tracker_to_matrices(q)
     vols = np.stack([np.roll(tmpl, tuple(rng.integers(-6, 7, 3)), axis=(0, 1, 2))
                      + 0.5 * rng.standard_normal(tmpl.shape) for _ in range(B)]).astype(np.float32)
@@ -271,45 +271,65 @@ def _fp16_case(rng, N, B, ctf):
     return tmpl, q, rot, vols, prior, ctfs
 
 
-@pytest.mark.parametrize("N,B,ctf", [(48, 8, False), (48, 4, True), (96, 6, False)])
-def test_fp16_reference_table_matches_oracle(gpu_device, N, B, ctf):
-    # the reduced-precision sampling table (FP16 values and weights, FP32
-    # accumulation): every term of the cross power carries at most ~2^-11
-    # relative rounding, so the normalised scores stay within 1e-3 of the FP64
-    # oracle and the integer lags, whose peak stands far above that, are exact
-    rng = np.random.default_rng(300 + N + B)
-    tmpl, q, rot, vols, prior, ctfs = _fp16_case(rng, N, B, ctf)
-    sm = _matcher(gpu_device, N, B, shift_reference_precision="fp16")
+def _search_and_oracle(dev, N, B, tmpl, q, rot, vols, prior, ctfs, **cfg):
+    sm = _matcher(dev, N, B, **cfg)
     sm.set_reference(tmpl)
-    kw = dict(subtomos=torch.tensor(vols, device=gpu_device),
-              ctfs=None if ctfs is None else torch.tensor(ctfs, device=gpu_device),
-              shift_zyx=torch.tensor(prior, device=gpu_device), rotation_tracker=q)
+    kw = dict(subtomos=torch.tensor(vols, device=dev),
+              ctfs=None if ctfs is None else torch.tensor(ctfs, device=dev),
+              shift_zyx=torch.tensor(prior, device=dev), rotation_tracker=q)
     sh, sc = sm.search_shifts(**kw)
     refF = S.load_reference_full(tmpl, N, 1.0).astype(np.complex64).astype(np.complex128)
     rsh, rsc = S.estimate_shift(refF, vols.astype(np.float64), prior, rot, N=N,
                                 ctfs=None if ctfs is None else ctfs.astype(np.float64))
+    return sm, kw, sh, sc, rsh, rsc
+
+
+@pytest.mark.parametrize("N,B,ctf", [(48, 8, False), (48, 4, True), (96, 6, False)])
+def test_reduced_precision_clear_peaks_match_oracle(gpu_device, N, B, ctf):
+    # reduced precision (FP16 reference table, BF16 work spectrum, FP32
+    # arithmetic) where the correlation has a clear peak (the reference at the
+    # particles' own orientation): each stored value carries at most 2^-9
+    # relative rounding, so the normalised scores stay within 2e-3 of the FP64
+    # oracle and the lags, whose peak stands far above that, are exact
+    rng = np.random.default_rng(300 + N + B)
+    case = _reduced_case(rng, N, B, ctf, rotate=False)
+    sm, kw, sh, sc, rsh, rsc = _search_and_oracle(gpu_device, N, B, *case, shift_precision="reduced")
     assert np.array_equal(sh.cpu().numpy(), rsh.astype(np.float32))
-    assert np.max(np.abs(sc.cpu().numpy() - rsc)) < 1e-3
-    # back on the FP32 table: the FP32 path's own results, bit for bit
-    sm.set_reference_precision("fp32")
+    assert np.max(np.abs(sc.cpu().numpy() - rsc)) < 2e-3
+    # back on FP32: the FP32 path's own results, bit for bit
+    sm.set_precision("fp32")
     sh32, sc32 = sm.search_shifts(**kw)
     fresh = _matcher(gpu_device, N, B)
-    fresh.set_reference(tmpl)
+    fresh.set_reference(case[0])
     shf, scf = fresh.search_shifts(**kw)
     assert torch.equal(sh32, shf) and torch.equal(sc32, scf)
     assert np.array_equal(sh32.cpu().numpy(), rsh.astype(np.float32))
 
 
-def test_fp16_reference_table_upsampled(gpu_device):
-    # sub-voxel refinement on the FP16 table: within one refinement step (1/up)
+@pytest.mark.parametrize("N,B", [(48, 8), (96, 4)])
+def test_reduced_precision_flat_correlations(gpu_device, N, B):
+    # at orientations unrelated to the particles the correlation surface is
+    # flat and near-tied lags are common: reduced precision then returns a lag
+    # whose correlation is within 2e-3 of the oracle's maximum, here a
+    # neighbour of the oracle's lag
+    rng = np.random.default_rng(500 + N + B)
+    case = _reduced_case(rng, N, B, False, rotate=True)
+    _, _, sh, sc, rsh, rsc = _search_and_oracle(gpu_device, N, B, *case, shift_precision="reduced")
+    assert np.max(np.abs(sc.cpu().numpy() - rsc)) < 2e-3
+    assert np.max(np.abs(sh.cpu().numpy() - rsh)) <= 1.0
+
+
+def test_reduced_precision_upsampled(gpu_device):
+    # sub-voxel refinement in reduced precision (FP16 reference table; this
+    # path keeps its work spectrum in FP32): within one refinement step (1/up)
     # of the FP64 oracle on every axis, scores within 1e-3
     N, B, up, radius = 64, 4, 8, 2
     rng = np.random.default_rng(404)
-    tmpl, q, rot, vols, prior, _ = _fp16_case(rng, N, B, False)
+    tmpl, q, rot, vols, prior, _ = _reduced_case(rng, N, B, False, rotate=True)
     vols = np.stack([np.roll(tmpl, tuple(rng.integers(-radius, radius + 1, 3)), axis=(0, 1, 2))
                      + 0.4 * rng.standard_normal(tmpl.shape) for _ in range(B)]).astype(np.float32)
     sm = _matcher(gpu_device, N, B, shift_upsample_factor=up, shift_search_radius=radius,
-                  shift_reference_precision="fp16")
+                  shift_precision="reduced")
     sm.set_reference(tmpl)
     sh, sc = sm.search_shifts(subtomos=torch.tensor(vols, device=gpu_device), ctfs=None,
                               shift_zyx=torch.tensor(prior, device=gpu_device), rotation_tracker=q)
@@ -321,6 +341,6 @@ def test_fp16_reference_table_upsampled(gpu_device):
     assert np.max(np.abs(sc.cpu().numpy() - rsc)) < 1e-3
 
 
-def test_reference_precision_rejects_unknown_mode(gpu_device):
+def test_shift_precision_rejects_unknown_mode(gpu_device):
     with pytest.raises(ValueError):
-        _matcher(gpu_device, 32, 1, shift_reference_precision="bf16")
+        _matcher(gpu_device, 32, 1, shift_precision="fp16")
