@@ -593,6 +593,16 @@ def main():
     z_flops = Bt * H * (2 * N * 5.0 * N * math.log2(N) + N * N * 38.0)
     z_gbs = z_bytes / z_t / 1e9
 
+    # the fused Newton march, bound by FP32 vector issue: achieved = the fused
+    # algorithm's shell flops per launch (march_flops) / its live duration;
+    # peak = the published FP32 rate, which counts packed FP32 (frac_nonpacked:
+    # against half of it, the rate without packed instructions, which this
+    # build does not use)
+    roof_march = {"bound": "fp32_vector", "achieved": march_fl / march_t / 1e12, "peak": 157.3,
+                  "unit": "TFLOP/s", "frac": march_fl / march_t / 1e12 / 157.3, "traffic": None,
+                  "frac_nonpacked": march_fl / march_t / 1e12 / 78.65,
+                  "kernel": "so3_march_kernel (with so3_repack_kernel)", "avg_ms": march_t * 1e3,
+                  "flops_per_launch": march_fl, "share_of_step": 2 * march_t * 1e3 / tuned["ms_per_step"]}
     dtype = {"bf16x3": "bf16x3+f32acc", "fp16x3": "fp16x3+f32acc"}[args.tuned_precision]
     out = {
         "metric": "aligned volumes/s, box 96 L=32, at <=1 deg median error: tuned vs plain FP32",
@@ -622,16 +632,7 @@ def main():
             "gemm_tiles": tiles,
             "setup_s": t_setup,
         },
-        # the dominant kernel (20% of the step): the fused Newton march, bound by
-        # FP32 vector issue; achieved = the fused algorithm's shell flops per
-        # launch (march_flops) / its live duration; peak = the published FP32
-        # rate, which counts packed FP32 (frac_nonpacked: against half of it,
-        # the rate without packed instructions, which this build does not use)
-        "roofline": {"bound": "fp32_vector", "achieved": march_fl / march_t / 1e12, "peak": 157.3,
-                     "unit": "TFLOP/s", "frac": march_fl / march_t / 1e12 / 157.3, "traffic": None,
-                     "frac_nonpacked": march_fl / march_t / 1e12 / 78.65,
-                     "kernel": "so3_march_kernel (with so3_repack_kernel)", "avg_ms": march_t * 1e3,
-                     "flops_per_launch": march_fl, "share_of_step": 2 * march_t * 1e3 / tuned["ms_per_step"]},
+        "roofline_march": roof_march,
         "roofline_shift_z": {"bound": "hbm", "achieved": z_gbs, "peak": 8000.0, "unit": "GB/s",
                              "frac": z_gbs / 8000.0,
                              "traffic": SHIFT_Z_TRAFFIC_BYTES.get((args.workload, Bt, args.tuned_shift_precision)),
@@ -649,6 +650,11 @@ def main():
                             "avg_ms": gemm_t * 1e3, "bytes_per_launch": alg_bytes,
                             "flops_per_launch": flops16, "share_of_step": 2 * gemm_t * 1e3 / tuned["ms_per_step"]},
     }
+    # `roofline`: the dominant kernel of this run, the one of the three with the
+    # largest share of the step (the march, the GEMM and the shift search's
+    # column pass take 18-21% each; which leads can change from box to box)
+    lead = max(("roofline_march", "roofline_gemm16", "roofline_shift_z"), key=lambda k: out[k]["share_of_step"])
+    out["roofline"] = dict(out[lead], entry=lead)
     if args.diff:
         out["differentiation"] = diff_workload(torch, dev, args)
     if args.batching:

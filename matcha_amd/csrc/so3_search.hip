@@ -474,7 +474,23 @@ __device__ void fill_lrec(float4* lrec, int L) {
 // march l upwards from the task's smallest mp; the repacked sigma of shell l
 // starts at byte 4 pk_offset(l), so one wave-uniform offset (advanced by
 // 16 (l + 1)^2 bytes) plus the lane's 16 p addresses both merged image pairs.
-__device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int Li, float beta,
+// The evaluation's functions of beta, computed by one thread and read by all
+// from LDS: cos b, sin b, c = cos b/2, s = sin b/2, log2 |c|, log2 |s|, c / s,
+// s / c.
+constexpr int kBetaScalars = 8;
+__device__ __forceinline__ void beta_scalars(float beta, float* bs) {
+  const float hc = cosf(0.5f * beta), hs = sinf(0.5f * beta);
+  bs[0] = cosf(beta);
+  bs[1] = sinf(beta);
+  bs[2] = hc;
+  bs[3] = hs;
+  bs[4] = log2f(fabsf(hc));
+  bs[5] = log2f(fabsf(hs));
+  bs[6] = hc / hs;
+  bs[7] = hs / hc;
+}
+
+__device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int Li, const float* bs,
                            const float2* __restrict__ za, const float2* __restrict__ zg,
                            const float* __restrict__ lgf, const float4* __restrict__ lrec, float* red,
                            float out[10]) {
@@ -484,12 +500,10 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
   float acc[10];
 #pragma unroll
   for (int i = 0; i < 10; ++i) acc[i] = 0.f;
-  const float x = cosf(beta), sx = sinf(beta);
-  const float hc = cosf(0.5f * beta), hs = sinf(0.5f * beta);
-  const float lc = log2f(fabsf(hc)), ls = log2f(fabsf(hs));
+  const float x = bs[0], sx = bs[1], hc = bs[2], hs = bs[3], lc = bs[4], ls = bs[5];
   const bool cneg = hc < 0.f, sneg = hs < 0.f;
   const bool ratios = fabsf(hc) >= 0x1p-20f && fabsf(hs) >= 0x1p-20f;
-  const float rcs = hc / hs, rsc = hs / hc;
+  const float rcs = bs[6], rsc = bs[7];
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(sp), (short)0, (int)sp_bytes, 0x00020000);
   const int P = (Li + 1) * (Li + 1);
@@ -659,11 +673,13 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
     for (int i = 0; i < 10; ++i) red[i * kMarchWaves + wave] = acc[i];
   }
   __syncthreads();
+  if (tid == 0) {  // the callers read the sums in thread 0 only
 #pragma unroll
-  for (int i = 0; i < 10; ++i) {
-    float v = 0.f;
-    for (int w = 0; w < kMarchWaves; ++w) v += red[i * kMarchWaves + w];
-    out[i] = v;
+    for (int i = 0; i < 10; ++i) {
+      float v = 0.f;
+      for (int w = 0; w < kMarchWaves; ++w) v += red[i * kMarchWaves + w];
+      out[i] = v;
+    }
   }
 }
 
@@ -708,12 +724,17 @@ __device__ bool solve3(const double Hm[9], const double g[3], double x[3]) {
   return true;
 }
 
-__global__ __launch_bounds__(kMarchThreads) void so3_march_kernel(MarchArgs args,
+// At least five waves per SIMD (at most 96 VGPRs, a few values spilled): the
+// march's evaluation chains hide their latencies better with five resident
+// workgroups per CU than with the four that 124 VGPRs allowed (1.24 against
+// 1.35 ms per batch of 512; six waves spill 40 VGPRs and lose again).
+__global__ __launch_bounds__(kMarchThreads, 5) void so3_march_kernel(MarchArgs args,
                                                                    const float* __restrict__ lgf_g) {
   __shared__ float2 za[kMaxL + 1], zg[kMaxL + 1];
   __shared__ float lgf[2 * kMaxL + 2];
   __shared__ float red[10 * kMarchWaves];
   __shared__ float ang[4];  // trial alpha, beta, gamma
+  __shared__ float bsc[kBetaScalars];
   __shared__ float4 lrec[kMaxL + 2];
   const int nblk = args.B * args.C;
   const int q = mch::xcd_remap(blockIdx.x, nblk);
@@ -758,10 +779,12 @@ __global__ __launch_bounds__(kMarchThreads) void so3_march_kernel(MarchArgs args
       }
       __syncthreads();
       const float ta = ang[0], tb = ang[1], tg = ang[2];
+      // the last wave computes the beta scalars while the first fills the phases
+      if (tid == kMarchThreads - 1) beta_scalars(tb, bsc);
       phase_table((double)ta, Li, za);
       phase_table((double)tg, Li, zg);
       __syncthreads();
-      march_eval(sp, sp_bytes, Li, tb, za, zg, lgf, lrec, red, ev);
+      march_eval(sp, sp_bytes, Li, bsc, za, zg, lgf, lrec, red, ev);
       if (tid == 0) {
         const bool first = (st == 0);
         const bool acc = first || (ev[0] > opt[3]);
@@ -803,11 +826,13 @@ __global__ __launch_bounds__(kMarchThreads) void so3_eval_kernel(
   const int b = q / C;
   for (int k = threadIdx.x; k <= 2 * L + 1; k += blockDim.x) lgf[k] = lgf_g[k];
   fill_lrec(lrec, L);
+  __shared__ float bsc[kBetaScalars];
+  if (threadIdx.x == kMarchThreads - 1) beta_scalars(be[q], bsc);
   phase_table((double)al[q], L, za);
   phase_table((double)ga[q], L, zg);
   __syncthreads();
   float ev[10];
-  march_eval(sig_pk + (long long)b * pk_stride, (unsigned)pk_stride * 8u, L, be[q], za, zg, lgf, lrec, red,
+  march_eval(sig_pk + (long long)b * pk_stride, (unsigned)pk_stride * 8u, L, bsc, za, zg, lgf, lrec, red,
              ev);
   if (threadIdx.x == 0)
     for (int k = 0; k < 10; ++k) out[(long long)q * 10 + k] = ev[k];
