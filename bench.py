@@ -646,7 +646,9 @@ def main():
                             "peak": 2500.0 if gemm_mfma > gemm_hbm else 8000.0,
                             "unit": "TFLOP/s" if gemm_mfma > gemm_hbm else "GB/s",
                             "frac": max(gemm_mfma, gemm_hbm), "hbm_frac": gemm_hbm, "mfma_frac": gemm_mfma,
-                            "traffic": traffic, "kernel": f"fle_gemm16_kernel<{args.tuned_precision}>",
+                            "traffic": traffic,
+                            "kernel": (("fle_gemm16w_kernel" if tiles["column_tiles_per_workgroup"] == 2
+                                        else "fle_gemm16_kernel") + f" ({args.tuned_precision})"),
                             "avg_ms": gemm_t * 1e3, "bytes_per_launch": alg_bytes,
                             "flops_per_launch": flops16, "share_of_step": 2 * gemm_t * 1e3 / tuned["ms_per_step"]},
     }
