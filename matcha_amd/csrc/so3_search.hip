@@ -206,6 +206,33 @@ __device__ __forceinline__ void block_top_c(unsigned (&hk)[NV], QOf qof, int C, 
   }
 }
 
+// F(u, v) = sum over max(|u|, |v|) <= l < L0 of d^l_{uv}(beta_j) sigma^l_{uv},
+// the terms' loads issued eight at a time (indices clamped to the last shell,
+// surplus terms masked), so that a thread waits for one round trip per eight
+// terms.
+__device__ __forceinline__ float2 sofft_f_entry(const float2* __restrict__ s, const float* __restrict__ dj,
+                                                int L0, int u, int v) {
+  const int l0 = abs(u) > abs(v) ? abs(u) : abs(v);
+  float ax = 0.f, ay = 0.f;
+  for (int lb = l0; lb < L0; lb += 8) {
+    float dv[8];
+    float2 sv[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int id = mch::dindex(min(lb + k, L0 - 1), u, v);
+      dv[k] = dj[id];
+      sv[k] = s[id];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float w = lb + k < L0 ? dv[k] : 0.f;
+      ax += w * sv[k].x;
+      ay += w * sv[k].y;
+    }
+  }
+  return make_float2(ax, ay);
+}
+
 // One workgroup per (particle, beta row): F(u,v) = sum_l d^l_{uv}(beta_j) sigma,
 // 2-D inverse DFT over (alpha, gamma), local top-C of the n*n slice.
 template <int MAXPER>
@@ -225,19 +252,7 @@ __global__ __launch_bounds__(256) void sofft_slice_kernel(
   for (int k = tid; k < n; k += blockDim.x) tw[k] = twiddle[k];
   const float2* s = sig + (long long)b * ldsig;
   const float* dj = dgrid + (long long)j * nsz;
-  for (int q = tid; q < W * W; q += blockDim.x) {
-    const int u = q / W - (L0 - 1), v = q % W - (L0 - 1);
-    const int l0 = abs(u) > abs(v) ? abs(u) : abs(v);
-    float ax = 0.f, ay = 0.f;
-    for (int l = l0; l < L0; ++l) {
-      const int id = mch::dindex(l, u, v);
-      const float dv = dj[id];
-      const float2 sv = s[id];
-      ax += dv * sv.x;
-      ay += dv * sv.y;
-    }
-    F[q] = make_float2(ax, ay);
-  }
+  for (int q = tid; q < W * W; q += blockDim.x) F[q] = sofft_f_entry(s, dj, L0, q / W - (L0 - 1), q % W - (L0 - 1));
   __syncthreads();
   // stage A: G(u, g) = sum_v F(u, v) e^{2 pi i v g / n}
   for (int q = tid; q < W * n; q += blockDim.x) {
@@ -309,30 +324,44 @@ __global__ __launch_bounds__(256) void sofft_slice_blk_kernel(
   }
   const float2* s = sig + (long long)b * ldsig;
   const float* dj = dgrid + (long long)j * nsz;
-  for (int q = tid; q < W * W; q += 256) {
-    const int u = q / W - (L0 - 1), v = q % W - (L0 - 1);
-    const int l0 = abs(u) > abs(v) ? abs(u) : abs(v);
-    float ax = 0.f, ay = 0.f;
-    for (int l = l0; l < L0; ++l) {
-      const int id = mch::dindex(l, u, v);
-      const float dv = dj[id];
-      const float2 sv = s[id];
-      ax += dv * sv.x;
-      ay += dv * sv.y;
-    }
-    F[q] = make_float2(ax, ay);
-  }
+  for (int q = tid; q < W * W; q += 256) F[q] = sofft_f_entry(s, dj, L0, q / W - (L0 - 1), q % W - (L0 - 1));
   __syncthreads();
   // stage A: G(u, g) = sum_v F(u, v) e^{2 pi i v g / n}
-  for (int q = tid; q < W * n; q += 256) {
-    const int ui = q / n, g = q - (q / n) * n;
-    float ax = 0.f, ay = 0.f;
+  constexpr int SA = 256 % n == 0 ? 256 / n : 1, KR = 8;
+  if (256 % n == 0 && W <= KR * SA) {
+    // a thread owns column g of rows ui0, ui0 + S, ... (S = 256 / n, at most
+    // KR rows): one E read per term serves all its rows
+    constexpr int S = SA;
+    const int g = tid % n, ui0 = tid / n;
+    float2 acc[KR];
+#pragma unroll
+    for (int k = 0; k < KR; ++k) acc[k] = make_float2(0.f, 0.f);
     for (int vi = 0; vi < W; ++vi) {
-      const float2 f = F[ui * W + vi], t = E[vi * n + g];
-      ax += f.x * t.x - f.y * t.y;
-      ay += f.x * t.y + f.y * t.x;
+      const float2 t = E[vi * n + g];
+#pragma unroll
+      for (int k = 0; k < KR; ++k) {
+        const int ui = ui0 + k * S;
+        if (ui < W) {
+          const float2 f = F[ui * W + vi];
+          acc[k].x += f.x * t.x - f.y * t.y;
+          acc[k].y += f.x * t.y + f.y * t.x;
+        }
+      }
     }
-    G[q] = make_float2(ax, ay);
+#pragma unroll
+    for (int k = 0; k < KR; ++k)
+      if (ui0 + k * S < W) G[(ui0 + k * S) * n + g] = acc[k];
+  } else {
+    for (int q = tid; q < W * n; q += 256) {
+      const int ui = q / n, g = q - (q / n) * n;
+      float ax = 0.f, ay = 0.f;
+      for (int vi = 0; vi < W; ++vi) {
+        const float2 f = F[ui * W + vi], t = E[vi * n + g];
+        ax += f.x * t.x - f.y * t.y;
+        ay += f.x * t.y + f.y * t.x;
+      }
+      G[q] = make_float2(ax, ay);
+    }
   }
   __syncthreads();
   // stage B: f(a, g) = Re sum_u e^{2 pi i u a / n} G(u, g) / n^2 on the thread's block
