@@ -544,7 +544,8 @@ __global__ __launch_bounds__(slice_threads<N>()) void shift_fwd_xy_kernel(
     const bool zxin = zs >= 0 && zs < N && xs >= 0 && xs < N;
     const int zxoff = zs * N * N + xs;
     constexpr int PER = N / YSTEP;
-    constexpr int U = load_batch<PER>();
+    // the whole share of the slice's loads in flight at once (one HBM round trip)
+    constexpr int U = PER <= 32 ? PER : load_batch<PER>();
     for (int base = 0; base < PER; base += U) {
       float pv[U];
 #pragma unroll
