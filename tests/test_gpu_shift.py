@@ -284,7 +284,8 @@ def _search_and_oracle(dev, N, B, tmpl, q, rot, vols, prior, ctfs, **cfg):
     return sm, kw, sh, sc, rsh, rsc
 
 
-@pytest.mark.parametrize("N,B,ctf", [(48, 8, False), (48, 4, True), (96, 6, False)])
+@pytest.mark.parametrize("N,B,ctf", [(32, 4, True), (48, 8, False), (48, 4, True), (96, 6, False),
+                                     (128, 2, False)])
 def test_reduced_precision_clear_peaks_match_oracle(gpu_device, N, B, ctf):
     # reduced precision (FP16 reference table, BF16 work spectrum, FP32
     # arithmetic) where the correlation has a clear peak (the reference at the
