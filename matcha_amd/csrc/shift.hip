@@ -626,25 +626,45 @@ __global__ __launch_bounds__(kZThreads) void shift_z_kernel(
   for (int zz = threadIdx.x & 63; zz < N; zz += 64) bgs += bg_part[b * N + zz];
   bgs = mch::wave_sum(bgs);
   const float bg = wsum > 0.f ? bgs / wsum : 0.f;
-  const int lc = threadIdx.x % NC, zb = threadIdx.x / NC;
-  const int lkx = kx0 + lc / kZKy, lky = ky0 + lc % kZKy;
-  const bool lok = lkx < H;  // the last group of kx can be partial
-  SpecElem<H16>* col = T + ((long long)b * H + (lok ? lkx : 0)) * N * N + lky;
-  const float2* mcol = XMy + (long long)(lok ? lkx : 0) * N * N + lky;
+  // loads and stores of T and XMy move two adjacent ky columns per lane (8
+  // bytes of BF16 or 16 of FP32 T): half the requests of one column per lane
+  constexpr int NP = NC / 2, PSTEP = NT / NP;  // 16 column pairs, rows z = zp + PSTEP k
+  static_assert(N % PSTEP == 0 && kZKy % 2 == 0, "paired column split");
+  using Pair = typename std::conditional<H16, uint2, float4>::type;
+  const int pc = threadIdx.x % NP, zp = threadIdx.x / NP;
+  const int pkx = kx0 + pc / (kZKy / 2), pky = ky0 + 2 * (pc % (kZKy / 2));
+  const bool pok = pkx < H;  // the last group of kx can be partial
+  const int lc0 = (pc / (kZKy / 2)) * kZKy + 2 * (pc % (kZKy / 2));  // tile column of the pair's first
+  SpecElem<H16>* col = T + ((long long)b * H + (pok ? pkx : 0)) * N * N + pky;
+  const float2* mcol = XMy + (long long)(pok ? pkx : 0) * N * N + pky;
   {
-    constexpr int PER = N / ZSTEP;
-    constexpr int U = load_batch<PER>();
-    for (int base = 0; base < PER; base += U) {
-      float2 a[U], mm[U];
+    constexpr int PER = N / PSTEP;
+    Pair a[PER];
+    float4 mm[PER];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int zz = zb + (base + u) * ZSTEP;
-        a[u] = lok ? spec_unpack<H16>(col[zz * N]) : make_float2(0.f, 0.f);
-        mm[u] = lok ? mcol[zz * N] : make_float2(0.f, 0.f);
+    for (int u = 0; u < PER; ++u) {
+      const int zz = zp + u * PSTEP;
+      if (pok) {
+        a[u] = *reinterpret_cast<const Pair*>(col + zz * N);
+        mm[u] = *reinterpret_cast<const float4*>(mcol + zz * N);
+      } else {
+        a[u] = Pair{};
+        mm[u] = make_float4(0.f, 0.f, 0.f, 0.f);
       }
+    }
 #pragma unroll
-      for (int u = 0; u < U; ++u)
-        tile[(zb + (base + u) * ZSTEP) * SC + lc] = make_float2(a[u].x + bg * mm[u].x, a[u].y + bg * mm[u].y);
+    for (int u = 0; u < PER; ++u) {
+      float2 a0, a1;
+      if constexpr (H16) {
+        a0 = spec_unpack<true>(a[u].x);
+        a1 = spec_unpack<true>(a[u].y);
+      } else {
+        a0 = make_float2(a[u].x, a[u].y);
+        a1 = make_float2(a[u].z, a[u].w);
+      }
+      float2* t = tile + (zp + u * PSTEP) * SC + lc0;
+      t[0] = make_float2(a0.x + bg * mm[u].x, a0.y + bg * mm[u].y);
+      t[1] = make_float2(a1.x + bg * mm[u].z, a1.y + bg * mm[u].w);
     }
   }
   __syncthreads();
@@ -765,8 +785,16 @@ __global__ __launch_bounds__(kZThreads) void shift_z_kernel(
     rnorm_part[((long long)b * NG + g) * NKB + kb] = bsum;
   }
   mch::fft::inverse<N, NC, NT, 1>(tile, 1, SC, tw);  // along z, natural order out
-  if (lok)
-    for (int zz = zb; zz < N; zz += ZSTEP) col[zz * N] = spec_pack<H16>(tile[zz * SC + lc]);
+  if (pok)
+    for (int zz = zp; zz < N; zz += PSTEP) {
+      const float2 v0 = tile[zz * SC + lc0], v1 = tile[zz * SC + lc0 + 1];
+      Pair o;
+      if constexpr (H16)
+        o = make_uint2(spec_pack<true>(v0), spec_pack<true>(v1));
+      else
+        o = make_float4(v0.x, v0.y, v1.x, v1.y);
+      *reinterpret_cast<Pair*>(col + zz * N) = o;
+    }
 }
 
 // Inverse transform of one (b, z) slice along y, then along x of the
