@@ -592,9 +592,21 @@ __global__ __launch_bounds__(slice_threads<N>()) void shift_fwd_xy_kernel(
   }
   __syncthreads();
   mch::fft::forward<N, H, NT, -1>(buf, S, 1, tw);  // along y: lines are kx
+  // stored two adjacent ky per lane (half the store requests)
+  using Pair = typename std::conditional<RED, uint2, float4>::type;
   SpecElem<RED>* out = T + (long long)b * H * N * N + (long long)z * N;
-  const int ky = threadIdx.x % N, sy = mch::fft::slot_of_freq<N>(ky);
-  for (int kx = kx0; kx < H; kx += YSTEP) out[(long long)kx * N * N + ky] = spec_pack<RED>(buf[kx * S + sy]);
+  constexpr int PSTEP = NT / (N / 2);
+  const int j2 = 2 * (threadIdx.x % (N / 2)), kxp = threadIdx.x / (N / 2);
+  const int s0 = mch::fft::slot_of_freq<N>(j2), s1 = mch::fft::slot_of_freq<N>(j2 + 1);
+  for (int kx = kxp; kx < H; kx += PSTEP) {
+    const float2 v0 = buf[kx * S + s0], v1 = buf[kx * S + s1];
+    Pair o;
+    if constexpr (RED)
+      o = make_uint2(spec_pack<true>(v0), spec_pack<true>(v1));
+    else
+      o = make_float4(v0.x, v0.y, v1.x, v1.y);
+    *reinterpret_cast<Pair*>(out + (long long)kx * N * N + j2) = o;
+  }
 }
 
 // One workgroup per (b, group of 4 kx, block of 8 ky): T + bg * XMy, FFT along z
@@ -819,18 +831,30 @@ __global__ __launch_bounds__(slice_threads<N>()) void shift_inv_xy_kernel(
   for (int k = threadIdx.x; k < N; k += NT) tw[k] = twg[k];
   const SpecElem<RED>* in = T + (long long)b * H * N * N + (long long)z * N;
   {
-    // rows kx of the slice, ky natural -> slot order for the inverse along y
-    const int ky = threadIdx.x % N, sy = mch::fft::slot_of_freq<N>(ky), kx0 = threadIdx.x / N;
-    float2 v[KXP];
+    // rows kx of the slice, ky natural -> slot order for the inverse along y;
+    // two adjacent ky per lane (half the load requests)
+    using Pair = typename std::conditional<RED, uint2, float4>::type;
+    constexpr int PSTEP = NT / (N / 2), KP = (H + PSTEP - 1) / PSTEP;
+    const int j2 = 2 * (threadIdx.x % (N / 2)), kxp = threadIdx.x / (N / 2);
+    const int s0 = mch::fft::slot_of_freq<N>(j2), s1 = mch::fft::slot_of_freq<N>(j2 + 1);
+    Pair v[KP];
 #pragma unroll
-    for (int i = 0; i < KXP; ++i) {
-      const int kx = kx0 + i * YSTEP;
-      if (kx < H) v[i] = spec_unpack<RED>(in[(long long)kx * N * N + ky]);
+    for (int i = 0; i < KP; ++i) {
+      const int kx = kxp + i * PSTEP;
+      if (kx < H) v[i] = *reinterpret_cast<const Pair*>(in + (long long)kx * N * N + j2);
     }
 #pragma unroll
-    for (int i = 0; i < KXP; ++i) {
-      const int kx = kx0 + i * YSTEP;
-      if (kx < H) buf[kx * S + sy] = v[i];
+    for (int i = 0; i < KP; ++i) {
+      const int kx = kxp + i * PSTEP;
+      if (kx < H) {
+        if constexpr (RED) {
+          buf[kx * S + s0] = spec_unpack<true>(v[i].x);
+          buf[kx * S + s1] = spec_unpack<true>(v[i].y);
+        } else {
+          buf[kx * S + s0] = make_float2(v[i].x, v[i].y);
+          buf[kx * S + s1] = make_float2(v[i].z, v[i].w);
+        }
+      }
     }
   }
   __syncthreads();
