@@ -76,6 +76,16 @@ int mch_newton_march(void* plan, const void* sigma, long long ld_sigma, int B, i
                      const int* levels, int n_levels, int num_steps, int newton, float base_lr,
                      float* alpha, float* beta, float* gamma, float* score, hipStream_t stream);
 
+/* Batch-global early exit of the march (the reference's stop_early config
+ * field): when set, a level's remaining Newton steps are skipped once one of
+ * its steps accepted no candidate of the batch.  mch_newton_march and
+ * mch_search_orientations then run one launch per (level, step), reading the
+ * accept counts of the previous launches on the device (no host sync);
+ * num_steps is then limited to 63.  Replaces the stop_early branch of
+ * Matcha._take_step and the break in Matcha.optimize
+ * (src/matcha/core/Matcha.py:198-199,358-361). */
+int mch_so3_set_stop_early(void* plan, int stop_early);
+
 /* Best candidate per particle (first maximum).  Replaces reduce_over_candidates
  * (src/matcha/core/Matcha.py:19-24). */
 int mch_reduce_candidates(const float* alpha, const float* beta, const float* gamma,

@@ -31,6 +31,7 @@ _SIGNATURES = [
     ("mch_sofft_eval", c_int, [c_void_p, c_void_p, c_ll, c_int, c_void_p, c_void_p]),
     ("mch_newton_march", c_int, [c_void_p, c_void_p, c_ll, c_int, c_int, c_int_p, c_int, c_int,
                                  c_int, c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("mch_so3_set_stop_early", c_int, [c_void_p, c_int]),
     ("mch_reduce_candidates", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p,
                                       c_void_p, c_void_p, c_void_p, c_void_p]),
     ("mch_search_orientations", c_int, [c_void_p, c_void_p, c_ll, c_int, c_int, c_int_p, c_int, c_int,

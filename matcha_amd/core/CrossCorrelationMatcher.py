@@ -59,7 +59,10 @@ class CrossCorrelationMatcher:
         self.device = torch.device(device)
         self.dtype_real = dtype
         self.dtype = _to_complex(dtype)
-        self.radius = radius if radius is not None else N // 2
+        # the reference passes radius=None through to FLEBasis3D, whose
+        # h = radius ** -1.5 then fails; its caller clamps the radius to
+        # min(N // 2, int(N // 2 * 0.95)) (align_subtomograms.py:94), the default here
+        self.radius = radius if radius is not None else min(N // 2, int(N // 2 * 0.95))
         if bandlimit is None:
             raise ValueError("bandlimit must be given")
         self.fle = FLEBasis3D(N=N, bandlimit=bandlimit, eps=expansion_epsilon,

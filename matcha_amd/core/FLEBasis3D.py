@@ -21,6 +21,8 @@ GEMMs, with the 16-element symmetry group D4h of the voxel grid folded in:
 
 The basis therefore spans ~V_ball/16 voxels and only m >= 0, i.e. about 1/8
 of the dense complex basis.  Tables are built here in FP64 and stored FP32.
+Orbit images outside the array (coordinate +N/2, which a ball of radius N/2
+reaches) enter the orbit sums as zeros.
 """
 from __future__ import annotations
 
@@ -131,23 +133,38 @@ def _orbit(v):
 def fundamental_voxels(N: int, radius: float):
     """Fundamental voxels of the ball |v| <= radius (1 + 1e-13), v = index - N/2.
 
-    Returns coords (V0, 3) int, orbit flat indices (V0, 16) int32 (axis 0 = x),
-    weights 1/|stabiliser| (V0,)."""
+    The array holds v in [-N/2, N/2 - 1] per axis and the ball is its part
+    within the radius, as the reference's mask (``np.arange(-R, R + N % 2)``,
+    FLEBasis3D.py:502-515): at radius N/2 the -N/2 boundary voxels are in the
+    ball and their +N/2 mirror images are not in the array.  Such images get
+    the index -1 (read as 0); orbits with no image in the array are dropped.
+
+    Returns coords (V0, 3) int, orbit flat indices (V0, 16) int32 (axis 0 = x,
+    -1 outside the array), weights 1/|stabiliser| (V0,)."""
+    if not radius > 0:
+        raise ValueError(f"radius must be positive, got {radius}")
     R = int(math.floor(radius * (1 + 1e-13)))
-    if N // 2 < R:
-        raise ValueError(f"radius {radius} does not fit a box of {N} with symmetric coordinates")
+    c = N // 2
     coords, orbits, wts = [], [], []
     lim = radius * (1 + 1e-13)
+
+    def flat(g):
+        if min(g) < -c or max(g) >= N - c:
+            return -1
+        return ((g[0] + c) * N + (g[1] + c)) * N + (g[2] + c)
+
     for x in range(0, R + 1):
         for y in range(0, x + 1):
             for z in range(0, R + 1):
                 if math.sqrt(x * x + y * y + z * z) > lim:
                     continue
                 imgs = _orbit((x, y, z))
+                ids = [flat(g) for g in imgs]
+                if max(ids) < 0:
+                    continue
                 stab = sum(1 for g in imgs if g == (x, y, z))
                 coords.append((x, y, z))
-                orbits.append([((gx + N // 2) * N + (gy + N // 2)) * N + (gz + N // 2)
-                               for gx, gy, gz in imgs])
+                orbits.append(ids)
                 wts.append(1.0 / stab)
     return np.asarray(coords, dtype=np.int64), np.asarray(orbits, dtype=np.int32), np.asarray(wts)
 

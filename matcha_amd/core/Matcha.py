@@ -10,8 +10,10 @@ keys and defaults (``_assert_config``, ``Matcha.py:120-148``), same
 The whole search is three HIP launches per call (grid top-C, fused Newton
 march over every level, candidate reduction) instead of the reference's
 per-step D/bank materialisation.  ``stop_early`` (a batch-global early exit,
-``Matcha.py:198-199``) runs the march one step per launch and checks the
-accept count on the host, exactly as the reference synchronises there.
+``Matcha.py:198-199``: a level stops after the first step that accepted no
+candidate of the batch) runs the march one launch per (level, step); each
+launch reads the accept counts of the previous ones on the device, so unlike
+the reference no host synchronisation is needed.
 """
 from __future__ import annotations
 
@@ -80,6 +82,10 @@ class Matcha:
                             K=int(cfg_get(cfg, "oversampling_factor_K")),
                             batchsize=self.batchsize, candidates=self.candidates,
                             device=self.device)
+        if self.stop_early:
+            if self.num_steps > 63:
+                raise ValueError("stop_early supports at most 63 num_steps")
+            _lib.check(_lib.lib().mch_so3_set_stop_early(self.plan.handle, 1), "mch_so3_set_stop_early")
         self._levels_c = (ctypes.c_int * len(self.levels))(*self.levels)
         self._truncate_dsize_cache = {int(l): Dsize(int(l)) for l in set(self.Li)}
 
@@ -121,8 +127,6 @@ class Matcha:
 
     @torch.no_grad()
     def search_orientations(self, sigma: torch.Tensor):
-        if self.stop_early:
-            return self._search_stop_early(sigma)
         sig = _check_sigma(sigma, self.device, Dsize(self.Lmax), self.batchsize)
         B = sig.shape[0]
         outs = [torch.empty(B, dtype=torch.float32, device=self.device) for _ in range(4)]
@@ -132,10 +136,6 @@ class Matcha:
             float(self.base_lr), *[o.data_ptr() for o in outs], _lib.stream_handle(self.device))
         _lib.check(rc, "mch_search_orientations")
         return tuple(outs)
-
-    def _search_stop_early(self, sigma):
-        raise NotImplementedError(
-            "stop_early=True (batch-global early exit) is not implemented in matcha_amd yet")
 
 
 def so3_evaluate(matcher: "Matcha", sigma: torch.Tensor, alphas, betas, gammas, L: int):

@@ -155,16 +155,20 @@ __global__ void fle_symmetrize_kernel(const float* __restrict__ vols, long long 
         make_shift_spec(shift_zyx, b, N, sp);
         for (int i = 0; i < 16; ++i) {
           const int id = orbit[(long long)v * 16 + i];
-          g[i] = patch_value(f, N, sp, id / (N * N), (id / N) % N, id % N);
+          g[i] = id < 0 ? 0.f : patch_value(f, N, sp, id / (N * N), (id / N) % N, id % N);
         }
       } else {
+        // index -1: an image outside the array (radius >= N/2), read as zero
+        // through the buffer range check
+        const __amdgpu_buffer_rsrc_t rv = make_rsrc(f, (unsigned)(N * N * N) * 4u);
+        auto at = [&](int id) { return load_f1(rv, id < 0 ? kOutOfRange : (unsigned)id * 4u); };
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int4 id = o4[i];
-          g[4 * i + 0] = f[id.x];
-          g[4 * i + 1] = f[id.y];
-          g[4 * i + 2] = f[id.z];
-          g[4 * i + 3] = f[id.w];
+          g[4 * i + 0] = at(id.x);
+          g[4 * i + 1] = at(id.y);
+          g[4 * i + 2] = at(id.z);
+          g[4 * i + 3] = at(id.w);
         }
       }
       // g[p + 4q + 8t]
@@ -267,17 +271,20 @@ __global__ __launch_bounds__(256) void fle_symmetrize_bf16_kernel(
     // an integral shift under which every image of the thread's voxels stays in
     // the volume reads them directly at an offset; otherwise through the buffer
     // range check (zero outside), or trilinear for a fractional shift
-    int shoff = 0;
-    bool direct = !SHIFTED;
-    if (SHIFTED && sp.integral) {
-      int m = 0, mz = 0;
+    // (and of the patch: a ball of radius >= N/2 reaches past the array,
+    // where images are zero)
+    int m = 0, mz = 0;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        m = max(m, max(pcs[u] & 1023, (pcs[u] >> 10) & 1023));
-        mz = max(mz, pcs[u] >> 20);
-      }
-      direct = c - m + sp.si[0] >= 0 && c + m + sp.si[0] < N && c - m + sp.si[1] >= 0 && c + m + sp.si[1] < N &&
-               c - mz + sp.si[2] >= 0 && c + mz + sp.si[2] < N;
+    for (int u = 0; u < 4; ++u) {
+      m = max(m, max(pcs[u] & 1023, (pcs[u] >> 10) & 1023));
+      mz = max(mz, pcs[u] >> 20);
+    }
+    const bool in_patch = c - m >= 0 && c + m < N && c - mz >= 0 && c + mz < N;
+    int shoff = 0;
+    bool direct = !SHIFTED && in_patch;
+    if (SHIFTED && sp.integral) {
+      direct = in_patch && c - m + sp.si[0] >= 0 && c + m + sp.si[0] < N && c - m + sp.si[1] >= 0 &&
+               c + m + sp.si[1] < N && c - mz + sp.si[2] >= 0 && c + mz + sp.si[2] < N;
       shoff = (sp.si[0] * N + sp.si[1]) * N + sp.si[2];
     }
     const __amdgpu_buffer_rsrc_t rv = make_rsrc(f, (unsigned)(N * N * N) * 4u);
@@ -308,11 +315,15 @@ __global__ __launch_bounds__(256) void fle_symmetrize_bf16_kernel(
     } else if (sp.integral) {
       fill([&](int a0, int a1, int a2) {
         const int i0 = a0 + sp.si[0], i1 = a1 + sp.si[1], i2 = a2 + sp.si[2];
-        const bool in = (unsigned)i0 < (unsigned)N && (unsigned)i1 < (unsigned)N && (unsigned)i2 < (unsigned)N;
+        const bool in = (unsigned)i0 < (unsigned)N && (unsigned)i1 < (unsigned)N && (unsigned)i2 < (unsigned)N &&
+                        (unsigned)a0 < (unsigned)N && (unsigned)a1 < (unsigned)N && (unsigned)a2 < (unsigned)N;
         return load_f1(rv, in ? (unsigned)((i0 * N + i1) * N + i2) * 4u : kOutOfRange);
       });
     } else {
-      fill([&](int a0, int a1, int a2) { return patch_value(f, N, sp, a0, a1, a2); });
+      fill([&](int a0, int a1, int a2) {
+        const bool in = (unsigned)a0 < (unsigned)N && (unsigned)a1 < (unsigned)N && (unsigned)a2 < (unsigned)N;
+        return in ? patch_value(f, N, sp, a0, a1, a2) : 0.f;
+      });
     }
     // class (eps, rho): G_t = sum_q (-1)^{q eps} sum_p (-i)^{rho p} f[p, q, t]
 #pragma unroll
@@ -1097,7 +1108,7 @@ int mch_fle_plan_create(int N, int V0, int Kpad, int Ntot, const int* class_colp
     return MCH_ERR_SIZE;
   }
   for (long long i = 0; i < (long long)V0 * 16; ++i) {
-    if (orbit[i] < 0 || orbit[i] >= N * N * N) {
+    if (orbit[i] < -1 || orbit[i] >= N * N * N) {
       mch::set_error("orbit index out of range");
       return MCH_ERR_ARG;
     }
@@ -1126,14 +1137,32 @@ int mch_fle_plan_create(int N, int V0, int Kpad, int Ntot, const int* class_colp
   MCH_TRY(hipMalloc(&p->orbit, (size_t)V0 * 16 * sizeof(int)), "hipMalloc(orbit)");
   MCH_TRY(hipMemcpy(p->orbit, orbit, (size_t)V0 * 16 * sizeof(int), hipMemcpyHostToDevice), "copy orbit");
   {
-    // the identity image (first orbit entry) of each fundamental voxel, as centred coordinates
+    // each fundamental voxel (x >= y >= 0, z >= 0) as centred coordinates, from
+    // any of its images inside the array (the images are the signed
+    // permutations of (x, y) with +-z); then every entry of the table must be
+    // its image p + 4q + 8t, or -1 exactly where that image is outside the array
     std::vector<int> fc(V0);
     const int c = N / 2;
     for (int v = 0; v < V0; ++v) {
-      const int id = orbit[(size_t)v * 16];
-      const int x = id / (N * N) - c, y = (id / N) % N - c, z = id % N - c;
-      if (x < 0 || y < 0 || z < 0 || x > 1023 || y > 1023 || z > 1023) {
-        mch::set_error("orbit table: first image of a fundamental voxel must have x, y, z >= 0");
+      const int* o = orbit + (size_t)v * 16;
+      int id = -1;
+      for (int i = 0; i < 16 && id < 0; ++i) id = o[i];
+      const int gx = id / (N * N) - c, gy = (id / N) % N - c, gz = id % N - c;
+      const int x = std::max(abs(gx), abs(gy)), y = std::min(abs(gx), abs(gy)), z = abs(gz);
+      bool ok = id >= 0 && x <= 1023 && z <= 1023;
+      for (int t = 0; t < 2 && ok; ++t)
+        for (int q = 0; q < 2 && ok; ++q)
+          for (int pp = 0; pp < 4 && ok; ++pp) {
+            const int y1 = t ? -y : y, zz = q ? -z : z;
+            const int a0 = pp == 0 ? x : (pp == 1 ? -y1 : (pp == 2 ? -x : y1));
+            const int a1 = pp == 0 ? y1 : (pp == 1 ? x : (pp == 2 ? -y1 : -x));
+            const bool in = a0 >= -c && a0 < N - c && a1 >= -c && a1 < N - c && zz >= -c && zz < N - c;
+            const int want = in ? ((a0 + c) * N + (a1 + c)) * N + (zz + c) : -1;
+            ok = o[pp + 4 * q + 8 * t] == want;
+          }
+      if (!ok) {
+        mch::set_error("orbit table: row %d is not the D4h orbit (images p + 4q + 8t, -1 outside the array) "
+                       "of a voxel with some image in the array", v);
         rc = MCH_ERR_ARG;
         goto fail;
       }

@@ -92,7 +92,12 @@ struct So3Plan {
   int* part_idx;         // (max_batch, n, max_cand)
   float* cand;           // (max_batch, max_cand, 4)
   float* lgf;            // log2 factorial table, size 2*Lmax+2
+  int stop_early;        // batch-global early exit of the march (stepped launches)
+  double* state;         // (max_batch * max_cand, 16) optimiser state between stepped launches
+  int* accept;           // (kMaxLevels * (kMaxStopSteps + 1)) accept counts of the stepped march
 };
+constexpr int kMaxLevels = 16;
+constexpr int kMaxStopSteps = 63;
 
 }  // namespace mch
 
@@ -476,6 +481,13 @@ struct MarchArgs {
   int newton;    // 1: Newton step, 0: gradient step
   float base_lr;
   int Lmax;
+  // stepped launches (stop_early): this launch runs level li0, step st0 only,
+  // with the optimiser state kept between launches in state (B*C x 16
+  // doubles) and accept[li * (num_steps + 1) + st] counting the candidates of
+  // the whole batch that step st of level li accepted (zeroed per search)
+  int li0, st0;
+  double* state;
+  int* accept;
 };
 
 constexpr int kMarchThreads = 256;
@@ -757,8 +769,19 @@ __device__ bool solve3(const double Hm[9], const double g[3], double x[3]) {
 // march's evaluation chains hide their latencies better with five resident
 // workgroups per CU than with the four that 124 VGPRs allowed (1.24 against
 // 1.35 ms per batch of 512; six waves spill 40 VGPRs and lose again).
+//
+// STEPPED (stop_early, Matcha.py:198-199): one launch per (level, step); a
+// level's remaining steps are skipped once one of its steps accepted no
+// candidate of the batch, read from the counts of the previous launches (the
+// launches are ordered on the stream, so no host synchronisation is needed).
+template <bool STEPPED>
 __global__ __launch_bounds__(kMarchThreads, 5) void so3_march_kernel(MarchArgs args,
                                                                    const float* __restrict__ lgf_g) {
+  if (STEPPED && args.st0 >= 2) {
+    const int* acc_l = args.accept + args.li0 * (args.num_steps + 1);
+    for (int st = 1; st < args.st0; ++st)
+      if (acc_l[st] == 0) return;  // the whole workgroup (uniform): the level has stopped
+  }
   __shared__ float2 za[kMaxL + 1], zg[kMaxL + 1];
   __shared__ float lgf[2 * kMaxL + 2];
   __shared__ float red[10 * kMarchWaves];
@@ -777,16 +800,26 @@ __global__ __launch_bounds__(kMarchThreads, 5) void so3_march_kernel(MarchArgs a
   // holds no registers in the other lanes during the evaluation
   __shared__ double gcur[3], Hcur[9];
   __shared__ float opt[5];  // current alpha, beta, gamma, best score, lr
+  double* stq = STEPPED ? args.state + (long long)q * 16 : nullptr;
   if (tid == 0) {
-    opt[0] = args.alpha[q]; opt[1] = args.beta[q]; opt[2] = args.gamma[q];
-    opt[3] = 0.f; opt[4] = 0.f;
-    for (int i = 0; i < 3; ++i) gcur[i] = 0.0;
-    for (int i = 0; i < 9; ++i) Hcur[i] = 0.0;
+    if (STEPPED && (args.li0 > 0 || args.st0 > 0)) {
+      for (int i = 0; i < 5; ++i) opt[i] = (float)stq[i];
+      for (int i = 0; i < 3; ++i) gcur[i] = stq[5 + i];
+      Hcur[0] = stq[8]; Hcur[1] = Hcur[3] = stq[9]; Hcur[2] = Hcur[6] = stq[10];
+      Hcur[4] = stq[11]; Hcur[5] = Hcur[7] = stq[12]; Hcur[8] = stq[13];
+    } else {
+      opt[0] = args.alpha[q]; opt[1] = args.beta[q]; opt[2] = args.gamma[q];
+      opt[3] = 0.f; opt[4] = 0.f;
+      for (int i = 0; i < 3; ++i) gcur[i] = 0.0;
+      for (int i = 0; i < 9; ++i) Hcur[i] = 0.0;
+    }
   }
   float ev[10];
-  for (int li = 0; li < args.n_levels; ++li) {
+  const int li_begin = STEPPED ? args.li0 : 0, li_end = STEPPED ? args.li0 + 1 : args.n_levels;
+  for (int li = li_begin; li < li_end; ++li) {
     const int Li = args.levels[li];
-    for (int st = 0; st <= args.num_steps; ++st) {
+    const int st_begin = STEPPED ? args.st0 : 0, st_end = STEPPED ? args.st0 : args.num_steps;
+    for (int st = st_begin; st <= st_end; ++st) {
       // thread 0 proposes the point to evaluate; everybody reads it from LDS
       if (tid == 0) {
         const float ca = opt[0], cb = opt[1], cg = opt[2], lr = opt[4];
@@ -829,9 +862,16 @@ __global__ __launch_bounds__(kMarchThreads, 5) void so3_march_kernel(MarchArgs a
         }
         if (first) lr = args.base_lr;
         opt[4] = lr;
+        if (STEPPED && !first && acc) atomicAdd(args.accept + li * (args.num_steps + 1) + st, 1);
       }
       __syncthreads();
     }
+  }
+  if (STEPPED && tid == 0) {
+    for (int i = 0; i < 5; ++i) stq[i] = (double)opt[i];
+    for (int i = 0; i < 3; ++i) stq[5 + i] = gcur[i];
+    stq[8] = Hcur[0]; stq[9] = Hcur[1]; stq[10] = Hcur[2];
+    stq[11] = Hcur[4]; stq[12] = Hcur[5]; stq[13] = Hcur[8];
   }
   if (tid == 0) {
     args.alpha[q] = opt[0];
@@ -865,6 +905,10 @@ __global__ __launch_bounds__(kMarchThreads) void so3_eval_kernel(
              ev);
   if (threadIdx.x == 0)
     for (int k = 0; k < 10; ++k) out[(long long)q * 10 + k] = ev[k];
+}
+
+__global__ void so3_zero_kernel(int* __restrict__ p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
 // argmax over candidates (first maximum), Matcha.reduce_over_candidates.
@@ -940,6 +984,9 @@ int mch_so3_plan_create(int L_max, int L0, int K, int max_batch, int max_candida
   MCH_TRY(hipMalloc(&p->part_val, (size_t)max_batch * n * max_candidates * sizeof(float)), "hipMalloc(part_val)");
   MCH_TRY(hipMalloc(&p->part_idx, (size_t)max_batch * n * max_candidates * sizeof(int)), "hipMalloc(part_idx)");
   MCH_TRY(hipMalloc(&p->cand, (size_t)max_batch * max_candidates * 4 * sizeof(float)), "hipMalloc(cand)");
+  MCH_TRY(hipMalloc(&p->state, (size_t)max_batch * max_candidates * 16 * sizeof(double)), "hipMalloc(state)");
+  MCH_TRY(hipMalloc(&p->accept, (size_t)mch::kMaxLevels * (mch::kMaxStopSteps + 1) * sizeof(int)),
+          "hipMalloc(accept)");
   if (lds > 64 * 1024) {
     MCH_TRY(hipFuncSetAttribute((const void*)sofft_slice_kernel<16>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
@@ -954,6 +1001,7 @@ int mch_so3_plan_create(int L_max, int L0, int K, int max_batch, int max_candida
 fail:
   (void)hipFree(p->d_grid); (void)hipFree(p->twiddle); (void)hipFree(p->lgf); (void)hipFree(p->sig_pk);
   (void)hipFree(p->part_val); (void)hipFree(p->part_idx); (void)hipFree(p->cand);
+  (void)hipFree(p->state); (void)hipFree(p->accept);
   delete p;
   return rc;
 }
@@ -963,6 +1011,7 @@ int mch_so3_plan_destroy(void* plan) {
   if (!p) return MCH_OK;
   (void)hipFree(p->d_grid); (void)hipFree(p->twiddle); (void)hipFree(p->lgf); (void)hipFree(p->sig_pk);
   (void)hipFree(p->part_val); (void)hipFree(p->part_idx); (void)hipFree(p->cand);
+  (void)hipFree(p->state); (void)hipFree(p->accept);
   delete p;
   return MCH_OK;
 }
@@ -1075,8 +1124,36 @@ int mch_newton_march(void* plan, const void* sigma, long long ld_sigma, int B, i
   a.newton = newton;
   a.base_lr = base_lr;
   a.Lmax = p->Lmax;
-  hipLaunchKernelGGL(so3_march_kernel, dim3(B * C), dim3(kMarchThreads), 0, stream, a, p->lgf);
-  return mch::hip_check(hipGetLastError(), "so3_march_kernel launch");
+  if (!p->stop_early) {
+    hipLaunchKernelGGL(so3_march_kernel<false>, dim3(B * C), dim3(kMarchThreads), 0, stream, a, p->lgf);
+    return mch::hip_check(hipGetLastError(), "so3_march_kernel launch");
+  }
+  // batch-global early exit: one launch per (level, step) after zeroing the
+  // accept counts (a fill kernel, so that the sequence can be graph-captured)
+  if (num_steps > mch::kMaxStopSteps) {
+    mch::set_error("mch_newton_march: stop_early supports at most %d steps per level", mch::kMaxStopSteps);
+    return MCH_ERR_SIZE;
+  }
+  const int ncount = n_levels * (num_steps + 1);
+  hipLaunchKernelGGL(so3_zero_kernel, dim3(1), dim3(256), 0, stream, p->accept, ncount);
+  if ((rc = mch::hip_check(hipGetLastError(), "so3_zero_kernel launch"))) return rc;
+  a.state = p->state;
+  a.accept = p->accept;
+  for (int li = 0; li < n_levels; ++li)
+    for (int st = 0; st <= num_steps; ++st) {
+      a.li0 = li;
+      a.st0 = st;
+      hipLaunchKernelGGL(so3_march_kernel<true>, dim3(B * C), dim3(kMarchThreads), 0, stream, a, p->lgf);
+      if ((rc = mch::hip_check(hipGetLastError(), "so3_march_kernel launch"))) return rc;
+    }
+  return MCH_OK;
+}
+
+int mch_so3_set_stop_early(void* plan, int stop_early) {
+  So3Plan* p = (So3Plan*)plan;
+  if (!p) { mch::set_error("mch_so3_set_stop_early: NULL plan"); return MCH_ERR_ARG; }
+  p->stop_early = stop_early ? 1 : 0;
+  return MCH_OK;
 }
 
 int mch_so3_evaluate(void* plan, const void* sigma, long long ld_sigma, int B, int C, int L,

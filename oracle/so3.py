@@ -23,8 +23,13 @@ Restated reference functions (swing-research/Matcha):
 
 Parity pins: ``derivatives`` vs central finite differences of the score
 (closed form, FP64); ``sofft_grid`` vs direct evaluation of Re sum D sigma / n^2
-on every grid point; the march vs monotone-score and fixed-point invariants.
-The exact accept/reject trajectory of the FP32 reference is unpinned.
+on every grid point; the march vs monotone-score and fixed-point invariants,
+and its rules (accept iff s' > best, lr x1.1 / x0.5, per-level reset of lr
+and best, the Newton step lr H^-1 g subtracted, the gradient step, the
+batch-global stop_early exit) by closed-form trajectories of scores
+A cos(beta) + B P2(cos(beta)) and by a Newton step from finite differences.
+The accept/reject trajectory of the FP32 reference where the decision rests
+on round-off (near convergence) is unpinned.
 """
 from __future__ import annotations
 
@@ -212,11 +217,13 @@ def _solve_or_skip(H: np.ndarray, g: np.ndarray):
 
 
 def optimize(alphas, betas, gammas, sigma, ell, num_steps, base_lr=1.0, stop_early=False,
-             step_type="newton"):
+             step_type="newton", trace=None):
     """``Matcha.optimize`` + ``_take_step`` for one level ``ell`` (FP64).
 
     alphas/betas/gammas: (B, C); sigma: (B, >= dsize(ell)).
-    Returns refined (alpha, beta, gamma, score) each (B, C).
+    Returns refined (alpha, beta, gamma, score) each (B, C).  ``trace`` (a
+    list, optional) receives one dict per step taken: the trial point, its
+    score, the accept mask and the learning rate after the step.
     """
     alphas = np.array(alphas, dtype=np.float64)
     betas = np.array(betas, dtype=np.float64)
@@ -247,6 +254,9 @@ def optimize(alphas, betas, gammas, sigma, ell, num_steps, base_lr=1.0, stop_ear
         upd = ev > best
         lr[~upd] *= 0.5
         lr[upd] *= 1.1
+        if trace is not None:
+            trace.append({"trial": (ta.copy(), tb.copy(), tg.copy()), "score": ev.copy(),
+                          "accept": upd.copy(), "lr": lr[..., 0].copy()})
         ta = np.where(upd, ta, alphas)
         tb = np.where(upd, tb, betas)
         tg = np.where(upd, tg, gammas)
@@ -273,7 +283,20 @@ def march_schedule(Li, L_max):
     return out
 
 
-def search_orientations(sigma, L_max, Li, candidates=10, num_steps=5, K=2, step_type="newton"):
+def march(alphas, betas, gammas, sigma, levels, num_steps=5, base_lr=1.0, stop_early=False,
+          step_type="newton"):
+    """``optimize`` over a level schedule, each level from the previous one's
+    best points (Matcha.py:381-398, before the candidate reduction)."""
+    a, b, g = alphas, betas, gammas
+    s = None
+    for ell in levels:
+        a, b, g, s = optimize(a, b, g, sigma, ell, num_steps, base_lr=base_lr, stop_early=stop_early,
+                              step_type=step_type)
+    return a, b, g, s
+
+
+def search_orientations(sigma, L_max, Li, candidates=10, num_steps=5, K=2, step_type="newton",
+                        stop_early=False):
     """``Matcha.search_orientations`` (Matcha.py:369-403) in FP64."""
     sigma = np.atleast_2d(sigma)
     sched = march_schedule(Li, L_max)
@@ -284,7 +307,8 @@ def search_orientations(sigma, L_max, Li, candidates=10, num_steps=5, K=2, step_
     s = None
     base_lr = 1.0 if step_type == "newton" else 5e-7
     for ell in sched:
-        a, b, g, s = optimize(a, b, g, sigma, ell, num_steps, base_lr=base_lr, step_type=step_type)
+        a, b, g, s = optimize(a, b, g, sigma, ell, num_steps, base_lr=base_lr, step_type=step_type,
+                              stop_early=stop_early)
     idx = np.argmax(s, axis=1)
     r = np.arange(s.shape[0])
     return a[r, idx], b[r, idx], g[r, idx], s[r, idx]

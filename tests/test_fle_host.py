@@ -40,23 +40,27 @@ def test_roots_pinned_by_mpmath():
         assert abs(lam - ref) < 1e-11 * ref
 
 
-def test_fundamental_voxels_partition_the_ball():
-    N, radius = 32, 13
+@pytest.mark.parametrize("N,radius", [(32, 13), (32, 16), (20, 10), (16, 9.5)])
+def test_fundamental_voxels_partition_the_ball(N, radius):
+    # radius N/2: the ball keeps the -N/2 boundary voxels, whose +N/2 mirror
+    # images are outside the array (index -1)
     coords, orbits, w = P.fundamental_voxels(N, radius)
     x, y, z, flat = fle.ball_voxels(N, radius)
     # each ball voxel appears in exactly one orbit; weights count multiplicity
     counts = {}
     for orb, wt in zip(orbits, w):
         for i in orb:
-            counts[int(i)] = counts.get(int(i), 0) + wt
+            if i >= 0:
+                counts[int(i)] = counts.get(int(i), 0) + wt
     assert set(counts) == set(int(i) for i in flat)
     assert all(abs(c - 1.0) < 1e-12 for c in counts.values())
+    assert (orbits.min() >= 0) == (radius < N // 2)
 
 
 def emulate_projection(t, vols, ne):
     B = vols.shape[0]
-    f = vols.reshape(B, -1)
-    g = f[:, t["orbits"]]
+    f = np.concatenate([vols.reshape(B, -1), np.zeros((B, 1))], axis=1)
+    g = f[:, t["orbits"]]  # index -1 (image outside the array) reads the zero column
     V0, Kpad, Ntot = t["V0"], t["Kpad"], t["Ntot"]
     X = np.zeros((8, 2 * B, Kpad))
     for eps in range(2):
@@ -99,7 +103,7 @@ def emulate_projection(t, vols, ne):
     return a
 
 
-@pytest.mark.parametrize("N,bl,radius", [(24, 8, 11), (20, 10, 9.5)])
+@pytest.mark.parametrize("N,bl,radius", [(24, 8, 11), (20, 10, 9.5), (16, 6, 8), (16, 6, 9.5)])
 def test_symmetric_gemm_emulation_matches_brute_force(N, bl, radius):
     sp = P.ball_spectrum(bl)
     ls, ks, ms, mds, lmds, cs, ne = sp

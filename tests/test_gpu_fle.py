@@ -28,6 +28,29 @@ def test_projection_small_exact(gpu_device, N, bl, radius, B):
     assert np.max(np.abs(got - ref)) < 2e-5 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("mode", ["fp32", "bf16x3", "fp16x3"])
+@pytest.mark.parametrize("N,bl,radius", [(32, 12, 16), (24, 8, 12.5)])
+def test_projection_ball_reaching_the_box_boundary(gpu_device, mode, N, bl, radius):
+    # radius >= N/2: the reference's ball keeps the -N/2 boundary voxels, whose
+    # +N/2 mirror images are not in the array (FLEBasis3D.py:502-515); they enter
+    # the orbit sums as zeros.  Also read at shifts (the patch is zero outside).
+    from matcha_amd.core.FLEBasis3D import FLEBasis3D
+    from matcha_amd.core.ShiftMatcher import extract_shifted_patches
+    B = 3
+    rng = np.random.default_rng(N + int(radius))
+    vols = rng.standard_normal((B, N, N, N)).astype(np.float32)
+    f = FLEBasis3D(N, bl, 1e-4, batchsize=B, radius=radius, device=gpu_device, compute_precision=mode)
+    tv = torch.tensor(vols, device=gpu_device)
+    got = f.evaluate_t(tv).cpu().numpy()
+    ref = fle.evaluate_t(vols.astype(np.float64), N, bl, radius)
+    tol = SPLIT_TOL.get(mode, 2e-5)
+    assert np.max(np.abs(got - ref)) < tol * np.max(np.abs(ref))
+    sh = torch.tensor([[1, -2, 0], [0, 0, 0], [0.5, 1.0, -1.5]], dtype=torch.float32, device=gpu_device)
+    direct = f.evaluate_t(tv, shift_zyx=sh).cpu().numpy()
+    via_patch = f.evaluate_t(extract_shifted_patches(tv, sh)).cpu().numpy()
+    assert np.array_equal(direct, via_patch)
+
+
 @pytest.mark.parametrize("N,bl,radius", [(64, 16, 28), (96, 32, 45)])
 def test_projection_benchmark_sizes_subset(gpu_device, N, bl, radius):
     rng = np.random.default_rng(N)

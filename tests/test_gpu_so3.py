@@ -151,3 +151,142 @@ def test_host_wigner_table_matches_oracle(gpu_device):
         _lib.check(_lib.lib().mch_wigner_d_host(beta, L, buf))
         ref = W.wigner_d(np.array([beta]), L)[0]
         assert np.max(np.abs(np.array(buf[:]) - ref)) < 1e-12
+
+
+# --- the march's rules on the GPU: closed-form trajectories (gradient mode) ---
+# Scores A cos(beta) + B P2(cos beta) (sigma entries (1,0,0) = A, (2,0,0) = B):
+# the accept/reject sequence is fixed by cos and sin with decisive margins
+# (>= 2e-2 here), so the FP32 march must follow it exactly.
+from tests.test_oracle_so3 import _closed_form, _cos_sigma  # noqa: E402
+
+
+def _march_cos(dev, pairs, betas, levels, steps, lr0, stop_early):
+    from matcha_amd.core.Matcha import Matcha
+    n = len(pairs)
+    m = Matcha(n, dev, 2, {"Li": [1], "candidates": 1, "num_steps": steps, "step_type": "gradient",
+                           "stop_early": stop_early})
+    m.base_lr = lr0
+    t = lambda x: torch.tensor(np.asarray(x, dtype=np.float32).reshape(n, 1), device=dev)
+    A, Bt, G, S = m.march(_sig_t(_cos_sigma(pairs), dev), t([0.3] * n), t(betas), t([1.7] * n), levels=levels)
+    return Bt.cpu().numpy()[:, 0].astype(np.float64), S.cpu().numpy()[:, 0].astype(np.float64)
+
+
+@pytest.mark.parametrize("stop_early", [False, True])
+@pytest.mark.parametrize("case", ["rules", "stop", "mixed"])
+def test_march_follows_closed_form_trajectory(gpu_device, case, stop_early):
+    pairs, betas = {"rules": ([(1.0, 0.3)], [1.0]),
+                    "stop": ([(1.0, 0.3), (1.0, 0.3)], [1.0, 1.4]),
+                    "mixed": ([(1.0, 0.3), (0.4, 0.1)], [1.0, 1.0])}[case]
+    levels, steps, lr0 = [1, 2], 3, 3.0
+    _, wb, ws = _closed_form(pairs, betas, levels, steps, lr0, stop_early=stop_early)
+    gb, gs = _march_cos(gpu_device, pairs, betas, levels, steps, lr0, stop_early)
+    assert np.max(np.abs(gb - np.array(wb))) < 1e-5, (gb, wb)
+    assert np.max(np.abs(gs - np.array(ws))) < 1e-5, (gs, ws)
+
+
+def test_stop_early_search_matches_oracle(gpu_device):
+    rng = np.random.default_rng(77)
+    B, L_max, Li = 3, 16, [8, 12]
+    truth = [(1.0, 0.8, 2.0), (4.0, 2.2, 5.5), (0.3, 1.6, 3.9)]
+    sig = np.stack([peaked_sigma(rng, L_max, *t, noise=0.03) for t in truth])
+    from matcha_amd.core.Matcha import Matcha
+    m = Matcha(B, gpu_device, L_max, {"Li": Li, "candidates": 10, "num_steps": 5, "stop_early": True})
+    A, Bt, G, S = (x.cpu().numpy().astype(np.float64) for x in m.search_orientations(_sig_t(sig, gpu_device)))
+    ra, rb, rg, rs = so3.search_orientations(sig, L_max, Li, candidates=10, num_steps=5, K=2, stop_early=True)
+    assert geodesic_deg(A, Bt, G, ra, rb, rg).max() < 0.1
+    assert np.max(np.abs(S - rs) / np.abs(rs)) < 1e-4
+
+
+def test_search_orientations_gradient_step_matches_oracle(gpu_device):
+    # step_type "gradient" (base lr 5e-7, Matcha.py:14-15,96-99): sigma scaled so
+    # that the steps move the candidates
+    rng = np.random.default_rng(5)
+    B, L_max, Li = 2, 16, [8, 12]
+    truth = [(2.0, 1.1, 0.6), (5.1, 2.0, 4.4)]
+    sig = 3e3 * np.stack([peaked_sigma(rng, L_max, *t, noise=0.03) for t in truth])
+    from matcha_amd.core.Matcha import Matcha
+    m = Matcha(B, gpu_device, L_max, {"Li": Li, "candidates": 10, "num_steps": 5, "step_type": "gradient"})
+    A, Bt, G, S = (x.cpu().numpy().astype(np.float64) for x in m.search_orientations(_sig_t(sig, gpu_device)))
+    ra, rb, rg, rs = so3.search_orientations(sig, L_max, Li, candidates=10, num_steps=5, K=2,
+                                             step_type="gradient")
+    grid = so3.search_orientations(sig, L_max, Li, candidates=10, num_steps=0, K=2)
+    assert np.all(rs > grid[3])  # the gradient steps improved on the grid points
+    assert geodesic_deg(A, Bt, G, ra, rb, rg).max() < 0.1
+    assert np.max(np.abs(S - rs) / np.abs(rs)) < 1e-4
+
+
+# --- larger degrees and grids: cfg4 (L = 48, lmax 67) and the reference's
+# production search (Li = [30, 40, 65], K = 2: grid n = 120; SOFFTMatcher
+# L_max 40, K 1: n = 80), which run the generic SOFFT slice kernel ---
+
+@pytest.mark.parametrize("L", [48, 67])
+def test_evaluate_score_grad_hessian_high_degree(gpu_device, L):
+    rng = np.random.default_rng(L)
+    B, C = 1, 3
+    sig = random_sigma(rng, B, 67, decay=0.03)
+    a = rng.uniform(0, 2 * np.pi, (B, C))
+    b = np.array([[0.0, 1.3, 2.9]])
+    g = rng.uniform(0, 2 * np.pi, (B, C))
+    m = _matcha(gpu_device, B, 67, [8], C=C)
+    from matcha_amd.core.Matcha import so3_evaluate
+    out = so3_evaluate(m, _sig_t(sig, gpu_device), torch.tensor(a, device=gpu_device),
+                       torch.tensor(b, device=gpu_device), torch.tensor(g, device=gpu_device), L)
+    out = out.cpu().numpy().astype(np.float64)
+    a32, b32, g32 = (x.astype(np.float32).astype(np.float64) for x in (a, b, g))
+    D = W.wigner_D(a32.ravel(), b32.ravel(), g32.ravel(), L).reshape(B, C, -1)
+    s_ref = np.einsum('bcn,bn->bc', D, sig[:, :W.dsize(L)]).real
+    grad, H = so3.derivatives(D, so3.coeff_bank(sig, L), a32)
+    ref = np.stack([s_ref, grad[..., 0], grad[..., 1], grad[..., 2], H[..., 0, 0], H[..., 0, 1],
+                    H[..., 0, 2], H[..., 1, 1], H[..., 1, 2], H[..., 2, 2]], axis=-1)
+    scale = np.sqrt((np.abs(sig[:, :W.dsize(L)]) ** 2).sum(axis=1))[:, None, None]
+    colscale = np.array([1, L, L, L, L * L, L * L, L * L, L * L, L * L, L * L], dtype=np.float64)
+    err = np.abs(out - ref) / (scale * colscale)
+    assert err.max() < 2e-5, (err.max(axis=(0, 1)))
+
+
+@pytest.mark.parametrize("L0,K", [(40, 1), (30, 2)])
+def test_sofft_grid_generic_kernel_matches_oracle(gpu_device, L0, K):
+    from matcha_amd.core.SOFFT import SOFFT
+    rng = np.random.default_rng(L0 + K)
+    B = 1
+    sig = random_sigma(rng, B, L0, decay=0.05)
+    sof = SOFFT(L0, gpu_device, B, oversampling_factor=K)
+    f = sof.eval(_sig_t(sig, gpu_device)).cpu().numpy()
+    ref = so3.sofft_grid(sig, L0, K)
+    assert f.shape == ref.shape == (B, 2 * L0 * K, 2 * L0 * K, 2 * L0 * K)
+    assert np.max(np.abs(f - ref)) < 1e-5 * (ref.max() - ref.min())
+
+
+def test_sofft_matcher_shipped_config_matches_oracle(gpu_device):
+    # SOFFTMatcher with the reference example's sofft_config (L_max 40,
+    # oversampling 1 -> n = 80, example.py:42-67,289-298): the grid argmax
+    from matcha_amd.core.SOFFT import SOFFTMatcher
+    rng = np.random.default_rng(80)
+    truth = [(1.0, 0.8, 2.0), (4.0, 2.2, 5.5)]
+    sig = np.stack([peaked_sigma(rng, 40, *t, noise=0.05, decay=0.05) for t in truth])
+    sm = SOFFTMatcher(2, gpu_device, 40, 1)
+    A, Bt, G, S = (x.cpu().numpy().astype(np.float64) for x in sm.search_orientations(_sig_t(sig, gpu_device)))
+    ra, rb, rg, rs = so3.sofft_search(sig, 40, 1)
+    f = so3.sofft_grid(sig, 40, 1).reshape(2, -1)
+    top2 = np.sort(f, axis=1)[:, -2:]
+    for i in range(2):
+        assert abs(S[i] - rs[i]) < 1e-5 * (f[i].max() - f[i].min())
+        if top2[i, 1] - top2[i, 0] > 1e-4 * (f[i].max() - f[i].min()):  # no near-tie
+            assert (A[i], Bt[i], G[i]) == pytest.approx((ra[i], rb[i], rg[i]), abs=1e-6)
+    # a grid of 80 points per axis is 4.5 degrees apart: the argmax is within a cell
+    assert geodesic_deg(A, Bt, G, *[np.array(x) for x in zip(*truth)]).max() < 6.0
+
+
+def test_production_search_config_matches_oracle(gpu_device):
+    # the reference's config.yaml search: Li = [30, 40, 65], K = 2, march to L = 65
+    rng = np.random.default_rng(65)
+    B, L_max, Li = 2, 65, [30, 40, 65]
+    truth = [(2.2, 0.9, 1.4), (0.4, 2.6, 5.0)]
+    sig = np.stack([peaked_sigma(rng, L_max, *t, noise=0.03, decay=0.05) for t in truth])
+    m = _matcha(gpu_device, B, L_max, Li, C=10, K=2)
+    assert m.levels == [30, 40, 65] and m.SOFFT.num_betas == 120
+    A, Bt, G, S = (x.cpu().numpy().astype(np.float64) for x in m.search_orientations(_sig_t(sig, gpu_device)))
+    ra, rb, rg, rs = so3.search_orientations(sig, L_max, Li, candidates=10, num_steps=5, K=2)
+    assert geodesic_deg(A, Bt, G, ra, rb, rg).max() < 0.1
+    assert np.max(np.abs(S - rs) / np.abs(rs)) < 1e-4
+    assert geodesic_deg(A, Bt, G, *[np.array(x) for x in zip(*truth)]).max() < 0.5
