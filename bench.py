@@ -3,11 +3,13 @@
 Headline (the metric's own definition): the tuned path -- batched (512
 particles per batch, consecutive batches alternating over two streams so that
 independent batches overlap on the device) with the projection GEMM on 16-bit
-hi/lo split operands (three MFMA products, FP32 accumulation) -- against the
-plain FP32 path at batch 64 on one stream, both measured in this run on the
-same synthetic particles, each required to reach <= 1 deg median geodesic
-error.  ``value`` is the tuned path's throughput; the plain path's throughput
-and the ratio are in ``config``.
+hi/lo split operands (three MFMA products, FP32 accumulation) and every other
+stage in FP32 (the shift search keeps complex64 storage, so its integer shifts
+are the reference's) -- against the plain FP32 path at batch 64 on one
+stream, both measured in this run on the same synthetic particles, each
+required to reach <= 1 deg median geodesic error.  ``value`` is the tuned
+path's throughput; the plain path's throughput and the ratio are in
+``config``; ``dtype`` names the precision of every stage that is not FP32.
 
 A step = one batch of particles aligned on the GPU by two rotation/shift
 alternations (matcha_amd.align.Alternation), each:
@@ -24,9 +26,16 @@ the radius-45 mask; FLE bandlimit 32 (ne 8252, lmax 42), radius 45; march
 Li = 8, 16, 24 -> L_max 32; SOFFT grid 64^3 (L0 = 8, K = 4); 10 candidates;
 5 Newton steps; plain path batch 64, tuned path batch 512.
 
-``roofline`` describes the kernel with the largest share of the step (the
-shift search's column pass), timed live with HIP events on its stream;
-``roofline_gemm16`` the tuned projection GEMM.
+Three kernels take 18-21% of the step each: the tuned projection GEMM
+(``roofline_gemm16``: MFMA bound, ``frac`` on the algorithmic one-product
+FLOPs, the issue rate of the three split products in ``mfma_issue_frac``),
+the fused Newton march (``roofline_march``: FP32 vector bound) and the shift
+search's column pass (``roofline_shift_z``: HBM bound); each is timed live
+with HIP events on its stream.  ``roofline`` repeats the one with the largest
+share of the step in this run.  ``cpu_baseline`` is a torch-CPU FP32
+restatement of the reference's algorithm (NUFFT projection, SHT, radial
+interpolation, sigma, SOFFT, Newton march, FFT shift search) timed on this
+host on a slice of the same workload.
 """
 from __future__ import annotations
 
@@ -76,9 +85,10 @@ def parse():
     p.add_argument("--batch", type=int, default=None, help="plain-path batch (default: the workload's)")
     p.add_argument("--tuned-batch", type=int, default=None, help="tuned-path batch (default: the workload's)")
     p.add_argument("--tuned-precision", default="bf16x3", choices=["bf16x3", "fp16x3"])
-    p.add_argument("--tuned-shift-precision", default="reduced", choices=["fp32", "reduced"],
-                   help="storage precision of the tuned path's shift search (reduced: FP16 reference table, "
-                        "BF16 work spectrum)")
+    p.add_argument("--tuned-shift-precision", default="fp32", choices=["fp32", "reduced"],
+                   help="storage precision of the tuned path's shift search (fp32: complex64 as the reference; "
+                        "reduced: FP16 reference table and BF16 work spectrum, lags of near-tied "
+                        "correlations may differ by one voxel)")
     p.add_argument("--tuned-streams", type=int, default=2,
                    help="streams the tuned path's batches alternate over (one plan set each)")
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP32 path")
@@ -554,7 +564,8 @@ def main():
 
     # the tuned path's projection GEMM: algorithmic bytes = W^T hi + lo (4 B per
     # element) + the hi/lo X operand (4 B per element), each read once;
-    # algorithmic flops = the three split products
+    # algorithmic flops = one product 2 (2B) (2 V0) sum(columns); the GEMM issues
+    # three (hi hi + hi lo + lo hi), reported as mfma_issue_frac
     fle_ = ccm.fle
     gemm_t = time_gemm(torch, dev, ccm, pool_vols[:Bt], args.tuned_precision)
     march_t = time_march(torch, dev, ccm, matcher, pool_vols[:Bt])
@@ -564,7 +575,8 @@ def main():
     x_bytes = 4.0 * 8 * (2 * Bt) * fle_.Kpad       # X hi + lo of the 8 classes
     alg_bytes = w_bytes + x_bytes
     achieved = alg_bytes / gemm_t / 1e9
-    flops16 = 3 * 2.0 * (2 * Bt) * (2 * fle_.V0) * sum(fle_.class_cols)
+    flops_alg = 2.0 * (2 * Bt) * (2 * fle_.V0) * sum(fle_.class_cols)   # one product
+    flops16 = 3 * flops_alg                                              # the three split products issued
     fle_.set_compute_precision(args.tuned_precision)
     depth = int(_lib_depth(fle_))
     tiles = {"k_splits": depth % 10000 // 100, "waves_per_workgroup": depth % 100 // 10,
@@ -574,7 +586,8 @@ def main():
     rows = -(-2 * Bt // 128) * 128
     traffic = fetch + tiles["k_splits"] * rows * fle_.Ntot * 4.0 if fetch is not None else None
     gemm_hbm = achieved / 8000.0
-    gemm_mfma = flops16 / gemm_t / 1e12 / 2500.0
+    gemm_mfma = flops_alg / gemm_t / 1e12 / 2500.0
+    gemm_issue = flops16 / gemm_t / 1e12 / 2500.0
 
     # the shift search's column pass (one block of 4 kx x
     # 8 ky columns: forward z FFT, rotated-reference sampling, cross power,
@@ -603,7 +616,10 @@ def main():
                   "frac_nonpacked": march_fl / march_t / 1e12 / 78.65,
                   "kernel": "so3_march_kernel (with so3_repack_kernel)", "avg_ms": march_t * 1e3,
                   "flops_per_launch": march_fl, "share_of_step": 2 * march_t * 1e3 / tuned["ms_per_step"]}
-    dtype = {"bf16x3": "bf16x3+f32acc", "fp16x3": "fp16x3+f32acc"}[args.tuned_precision]
+    dtype = {"bf16x3": "bf16x3+f32acc", "fp16x3": "fp16x3+f32acc"}[args.tuned_precision] + " projection GEMM"
+    if args.tuned_shift_precision == "reduced":
+        dtype += ", fp16 reference table + bf16 spectrum in the shift search"
+    dtype += ", fp32 elsewhere (fp64 Wigner/Bessel tables and Newton solve)"
     out = {
         "metric": "aligned volumes/s, box 96 L=32, at <=1 deg median error: tuned vs plain FP32",
         "value": tuned["volumes_per_s"],
@@ -642,15 +658,17 @@ def main():
                              "share_of_step": 2 * z_t * 1e3 / tuned["ms_per_step"],
                              "shift_kernels_ms": dict(zip(("fwd_xy", "z", "inv_xy", "final"), map(float, shift_ms)))},
         "roofline_gemm16": {"bound": "mfma" if gemm_mfma > gemm_hbm else "hbm",
-                            "achieved": flops16 / gemm_t / 1e12 if gemm_mfma > gemm_hbm else achieved,
+                            "achieved": flops_alg / gemm_t / 1e12 if gemm_mfma > gemm_hbm else achieved,
                             "peak": 2500.0 if gemm_mfma > gemm_hbm else 8000.0,
                             "unit": "TFLOP/s" if gemm_mfma > gemm_hbm else "GB/s",
                             "frac": max(gemm_mfma, gemm_hbm), "hbm_frac": gemm_hbm, "mfma_frac": gemm_mfma,
+                            "mfma_issue_frac": gemm_issue, "split_products": 3,
                             "traffic": traffic,
                             "kernel": (("fle_gemm16w_kernel" if tiles["column_tiles_per_workgroup"] == 2
                                         else "fle_gemm16_kernel") + f" ({args.tuned_precision})"),
                             "avg_ms": gemm_t * 1e3, "bytes_per_launch": alg_bytes,
-                            "flops_per_launch": flops16, "share_of_step": 2 * gemm_t * 1e3 / tuned["ms_per_step"]},
+                            "flops_per_launch": flops_alg, "flops_issued_per_launch": flops16,
+                            "share_of_step": 2 * gemm_t * 1e3 / tuned["ms_per_step"]},
     }
     # `roofline`: the dominant kernel of this run, the one of the three with the
     # largest share of the step (the march, the GEMM and the shift search's

@@ -156,7 +156,7 @@ def test_host_wigner_table_matches_oracle(gpu_device):
 # --- the march's rules on the GPU: closed-form trajectories (gradient mode) ---
 # Scores A cos(beta) + B P2(cos beta) (sigma entries (1,0,0) = A, (2,0,0) = B):
 # the accept/reject sequence is fixed by cos and sin with decisive margins
-# (>= 2e-2 here), so the FP32 march must follow it exactly.
+# (every |s' - best| >= 6e-3 in these cases), so the FP32 march must follow it.
 from tests.test_oracle_so3 import _closed_form, _cos_sigma  # noqa: E402
 
 
@@ -176,7 +176,7 @@ def _march_cos(dev, pairs, betas, levels, steps, lr0, stop_early):
 def test_march_follows_closed_form_trajectory(gpu_device, case, stop_early):
     pairs, betas = {"rules": ([(1.0, 0.3)], [1.0]),
                     "stop": ([(1.0, 0.3), (1.0, 0.3)], [1.0, 1.4]),
-                    "mixed": ([(1.0, 0.3), (0.4, 0.1)], [1.0, 1.0])}[case]
+                    "mixed": ([(1.0, 0.3), (0.5, 0.2)], [1.0, 2.2])}[case]
     levels, steps, lr0 = [1, 2], 3, 3.0
     _, wb, ws = _closed_form(pairs, betas, levels, steps, lr0, stop_early=stop_early)
     gb, gs = _march_cos(gpu_device, pairs, betas, levels, steps, lr0, stop_early)

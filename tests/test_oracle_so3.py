@@ -182,7 +182,7 @@ def test_stop_early_is_batch_global():
     assert _same_log(got, want)
     assert [L for L, _ in want] == [1, 2] and np.allclose(gb, betas)
     # one particle accepting keeps the whole batch marching
-    pairs2, betas2 = [(1.0, 0.3), (0.4, 0.1)], [1.0, 1.0]
+    pairs2, betas2 = [(1.0, 0.3), (0.5, 0.2)], [1.0, 2.2]
     want2, _, _ = _closed_form(pairs2, betas2, levels, steps, lr0, stop_early=True)
     got2, gb2, _ = _oracle_log(pairs2, betas2, levels, steps, lr0, stop_early=True)
     assert _same_log(got2, want2)
