@@ -294,6 +294,17 @@ int mch_score_gradients(const int* idx, int lmax, int kdim, const void* sigma, l
                         const void* aV, long long ld_aV, int L, const double* angles, const double* jr,
                         float* partial, float* out, void* G, long long ldG, hipStream_t stream);
 
+/* mch_score_gradients without the partial argument and without allocating:
+ * the per-shell partials live in a static device array of the library
+ * (B * (L + 1) <= 131072, 4 MiB per GPU for the life of the process).  No
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (HIP graph capture, serving loops).  Calls on one GPU share that
+ * array, so calls that overlap on one GPU (different streams or host
+ * threads) race; calls ordered on one stream do not. */
+int mch_score_gradients_noalloc(const int* idx, int lmax, int kdim, const void* sigma, long long ld_sigma, int B,
+                                const void* aV, long long ld_aV, int L, const double* angles, const double* jr,
+                                float* out, void* G, long long ldG, hipStream_t stream);
+
 /* Backward of the score under upstream weights (B,): d_omega (B, 3) and d_t (B, 3)
  * (either may be NULL) from out of mch_score_gradients, and with G the template
  * gradient d_aT (ne,) complex64 = sum_b weights[b] G_b (d/dRe + i d/dIm). */

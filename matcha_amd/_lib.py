@@ -72,6 +72,8 @@ _SIGNATURES = [
     ("mch_shift_tangent", c_int, [c_void_p, c_ll, c_int, c_int, c_int, c_void_p, c_void_p, c_ll, c_void_p]),
     ("mch_score_gradients", c_int, [c_void_p, c_int, c_int, c_void_p, c_ll, c_int, c_void_p, c_ll, c_int,
                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_ll, c_void_p]),
+    ("mch_score_gradients_noalloc", c_int, [c_void_p, c_int, c_int, c_void_p, c_ll, c_int, c_void_p, c_ll,
+                                            c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_ll, c_void_p]),
     ("mch_score_backward", c_int, [c_void_p, c_void_p, c_int, c_void_p, c_ll, c_int, c_void_p, c_void_p,
                                    c_void_p, c_void_p]),
     ("mch_wigner_d_host", c_int, [c_double, c_int, ctypes.POINTER(c_double)]),

@@ -31,6 +31,8 @@
 
 #include <math.h>
 
+#include <mutex>
+
 namespace {
 
 constexpr int kThreads = 256;
@@ -322,6 +324,36 @@ __global__ void score_backward_kernel(const float* __restrict__ weights, const f
   d_aT[j] = make_float2((float)re, (float)im);
 }
 
+// Static scratch of mch_score_gradients_noalloc: the per-shell partials
+// (B (L + 1) 8 floats), at most kScratchFloats.  Each GPU holds its own copy
+// for the life of the process; host code reaches it only through the device
+// address from hipGetSymbolAddress, looked up once per device.
+constexpr size_t kScratchFloats = size_t(1) << 20;  // 4 MiB
+__device__ __align__(256) float scratch[kScratchFloats];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices];
+
+int scratch_address(float** out) {
+  int dev = -1;
+  int rc = mch::hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= kMaxDevices) {
+    mch::set_error("device %d beyond the %d devices the static scratch is tracked for", dev, kMaxDevices);
+    return MCH_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    rc = mch::hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (rc) return rc;
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return MCH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -371,6 +403,26 @@ int mch_score_gradients(const int* idx, int lmax, int kdim, const void* sigma, l
   if (rc) return rc;
   hipLaunchKernelGGL(score_reduce_kernel, dim3((B + 255) / 256), dim3(256), 0, stream, partial, B, L, jr, out);
   return mch::hip_check(hipGetLastError(), "score_reduce_kernel");
+}
+
+int mch_score_gradients_noalloc(const int* idx, int lmax, int kdim, const void* sigma, long long ld_sigma, int B,
+                                const void* aV, long long ld_aV, int L, const double* angles, const double* jr,
+                                float* out, void* G, long long ldG, hipStream_t stream) {
+  if (B < 1 || L < 0) {
+    mch::set_error("mch_score_gradients_noalloc: bad arguments (B=%d L=%d)", B, L);
+    return MCH_ERR_ARG;
+  }
+  const size_t need = (size_t)B * (size_t)(L + 1) * kParts;
+  if (need > kScratchFloats) {
+    mch::set_error("mch_score_gradients_noalloc: B (L + 1) = %zu exceeds the static scratch (at most %zu)",
+                   (size_t)B * (size_t)(L + 1), kScratchFloats / kParts);
+    return MCH_ERR_SIZE;
+  }
+  float* partial = nullptr;
+  const int rc = scratch_address(&partial);
+  if (rc) return rc;
+  return mch_score_gradients(idx, lmax, kdim, sigma, ld_sigma, B, aV, ld_aV, L, angles, jr, partial, out, G, ldG,
+                             stream);
 }
 
 int mch_score_backward(const float* weights, const float* grads, int B, const void* G, long long ldG, int ne,

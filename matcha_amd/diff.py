@@ -58,7 +58,8 @@ class ScoreEngine:
         """Ball-harmonic coefficients (ne,) complex64 of a template volume."""
         return self.ccm.fle.evaluate_t(template.reshape(1, self.N, self.N, self.N).to(self.device))[0]
 
-    def _forward(self, omega, shift, aT, particles, want_template: bool):
+    def _prepare(self, omega, shift, aT, particles):
+        """Inputs of mch_score_gradients: (B, angles, jr, coeffs, sigma)."""
         B = omega.shape[0]
         if B > self.batchsize:
             raise ValueError(f"batch {B} exceeds batchsize {self.batchsize}")
@@ -87,6 +88,12 @@ class ScoreEngine:
         _lib.check(lib.mch_compute_sigma_degree(self.ccm._corr, a_t.data_ptr(), coeffs.data_ptr(),
                                                 coeffs.stride(0), self.ne, 4 * B, self.L, sigma.data_ptr(),
                                                 sigma.stride(0), None, stream), "mch_compute_sigma")
+        return B, angles, jr, coeffs, sigma
+
+    def _forward(self, omega, shift, aT, particles, want_template: bool):
+        B, angles, jr, coeffs, sigma = self._prepare(omega, shift, aT, particles)
+        stream = _lib.stream_handle(self.device)
+        lib = _lib.lib()
         partial = torch.empty((B, self.L + 1, 8), dtype=torch.float32, device=self.device)
         out = torch.empty((B, 7), dtype=torch.float32, device=self.device)
         G = torch.empty((B, self.ne), dtype=torch.complex64, device=self.device) if want_template else None
