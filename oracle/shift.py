@@ -26,7 +26,12 @@ Restated reference functions (swing-research/Matcha,
 
 Parity pins: an integer circular shift of a volume against itself gives that
 shift exactly (closed form); the cross-correlation equals a direct lag sum
-on small grids; rotation by the identity reproduces the half spectrum.
+on small grids; rotation by the identity reproduces the half spectrum;
+``apply_soft_mask`` scales single Fourier modes inside, on and beyond its
+cosine band by the closed-form factors; ``gridding_correct`` returns deltas
+divided by sinc^2 at known radii (1, pi^2/8, pi^2/4); ``get_mask_full``
+equals a brute-force count of wrapped frequencies; the upsampled refinement
+peaks at the closed-form Dirichlet-kernel maximum.
 """
 from __future__ import annotations
 

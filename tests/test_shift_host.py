@@ -72,3 +72,65 @@ def test_oracle_upsampled_refinement_closed_form(up, s0, s1, s2):
     shift, score = S.compute_shift_local(ref, sub, upsample=up, window=2 * 2 + 1)
     assert np.array_equal(shift, np.array([s0, s1, s2]))
     assert score == pytest.approx(1.0, rel=1e-12)
+
+
+# --- reference preparation (ShiftMatcher.py:377-453), pinned by single modes ---
+
+def _mode(N, k):
+    ax = np.arange(N)
+    Z, Y, X = np.meshgrid(ax, ax, ax, indexing="ij")
+    return np.cos(2 * np.pi * (k[0] * Z + k[1] * Y + k[2] * X) / N)
+
+
+@pytest.mark.parametrize("k,factor", [
+    ((3, 0, 0), 1.0),                  # inside the pass band (radius 7)
+    ((0, 7, 0), 1.0),                  # on its edge: 0.5 - 0.5 cos(pi) = 1
+    ((0, 0, 8), 0.5),                  # mid cosine band: 0.5 - 0.5 cos(pi / 2)
+    ((2, -2, -8), None),               # in the band, |k| = sqrt(72): 0.5 - 0.5 cos(pi (9 - |k|) / 2)
+    ((9, 1, 0), 0.0),                  # beyond radius_p = 9
+])
+def test_soft_mask_scales_a_single_fourier_mode(k, factor):
+    # box 32 at 1 A/px low-passed at 4 A with a 2 px edge: radius 32 / 4 - 1 = 7,
+    # radius_p = 9 (in Fourier pixels); a real mode cos(2 pi k.x / N) has its two
+    # components at +-k, both at |k|, so it comes back scaled by the mask there
+    N = 32
+    if factor is None:
+        factor = 0.5 - 0.5 * np.cos(np.pi * (9 - np.sqrt(72.0)) / 2)
+    v = _mode(N, k)
+    out = S.apply_soft_mask(v, N, 1.0, 4.0, 2.0)
+    assert np.max(np.abs(out - factor * v)) < 1e-12
+
+
+def test_soft_mask_off_for_non_positive_resolution():
+    v = _mode(16, (1, 2, 3))
+    assert S.apply_soft_mask(v, 16, 1.0, 0.0) is v
+
+
+@pytest.mark.parametrize("pos,value", [
+    ((16, 16, 16), 1.0),               # the centre: sinc(0) = 1
+    ((16, 16, 24), np.pi ** 2 / 8),    # r = 8 / 32 = 1/4: sinc = 2 sqrt(2) / pi
+    ((8, 16, 16), np.pi ** 2 / 8),     # the same along z, on the negative side
+    ((0, 16, 16), np.pi ** 2 / 4),     # r = 1/2: sinc = 2 / pi
+])
+def test_gridding_correction_divides_by_sinc_squared(pos, value):
+    # a delta at array index pos (centre N/2) comes back as 1 / sinc^2(|pos - N/2| / N)
+    N = 32
+    v = np.zeros((N, N, N))
+    v[pos] = 1.0
+    out = S.gridding_correct(v, N, 1.0)
+    assert out[pos] == pytest.approx(value, rel=1e-12)
+    out[pos] = 0
+    assert not out.any()
+
+
+def test_full_mask_is_the_ball_of_radius_half_box():
+    # mask_full over unshifted FFT indices: frequencies (k mod N, wrapped to
+    # [-N/2, N/2)) with |k|^2 <= (N/2)^2, counted by brute force
+    import itertools
+    N = 8
+    m = S.get_mask_full((N, N, N))
+    want = np.zeros((N, N, N), dtype=bool)
+    for i, j, l in itertools.product(range(N), repeat=3):
+        f = [x if x < N // 2 else x - N for x in (i, j, l)]
+        want[i, j, l] = sum(x * x for x in f) <= (N // 2) ** 2
+    assert np.array_equal(m, want)
