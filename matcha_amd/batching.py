@@ -10,12 +10,15 @@ particles with per-particle (box, march end L) is aligned in one pass:
   autotuned per box when its tuned precision is enabled), each bucket its own
   ``Matcha`` (march schedule ``Li`` truncated below L, then L) and alternation
   harness;
-* a bucket runs full batches from static device buffers; with ``graphs=True``
-  one alternation pass over a full batch (projection, sigma, SOFFT grid, the
-  fused Newton march -- every level and step in one launch -- and the shift
-  search, twice) is captured once into a HIP graph and replayed per batch, so
-  a batch is one graph launch; the ragged last batch of a bucket is padded
-  with empty particles, as the reference pads its last batch;
+* by default a bucket launches its kernels eagerly per batch (about twenty
+  long kernels per batch: launch overhead is not the bound, and a graph
+  replay measured slower for the copies into its static buffers); with
+  ``graphs=True`` (for callers that must replay a captured graph) a bucket runs
+  full batches from static device buffers: one alternation pass over a full
+  batch (projection, sigma, SOFFT grid, the fused Newton march -- every level
+  and step in one launch -- and the shift search, twice) is captured once
+  into a HIP graph and replayed per batch, the ragged last batch of a bucket
+  padded with empty particles, as the reference pads its last batch;
 * missing-wedge or CTF weights go to the shift search per particle, as the
   reference's ``ctfs`` (``ShiftMatcher.py:637-638``).
 """
@@ -112,7 +115,7 @@ class UnevenAligner:
 
     def __init__(self, templates: dict, device, radius: dict | None = None, bandlimit: int = 32, batch: int = 64,
                  L_ends=(16, 24, 32), Li=(8, 16, 24), K: int = 4, candidates: int = 10, num_steps: int = 5,
-                 alternations: int = 2, precision: str = "bf16x3", graphs: bool = True, max_shift: int = 4):
+                 alternations: int = 2, precision: str = "bf16x3", graphs: bool = False, max_shift: int = 4):
         self.device = torch.device(device)
         self.batch = int(batch)
         self.cfg = {"Li": tuple(Li), "K": int(K), "candidates": int(candidates), "num_steps": int(num_steps)}

@@ -137,15 +137,26 @@ def wedge_mask_half(N: int, tilt_max_deg: float, axis_offset_deg: float, device)
     return keep.to(torch.float32).expand(N, N, N // 2 + 1).contiguous()
 
 
+# Blob templates per box size, as the single-box workloads of that size: box 64
+# 40 blobs with sigma U(1.5, 4) px, box 96 120 blobs and box 128 200 blobs with
+# sigma U(1, 5) px (other sizes: 120 blobs scaled by the box area, sigma U(1, 5))
+BOX_TEMPLATES = {64: (40, (1.5, 4.0)), 96: (120, (1.0, 5.0)), 128: (200, (1.0, 5.0))}
+
+
+def box_template_spec(N: int):
+    return BOX_TEMPLATES.get(int(N), (max(20, int(120 * (N / 96) ** 2)), (1.0, 5.0)))
+
+
 class UnevenWorkload:
-    """Particles of several box sizes, each box with its own blob template; per
+    """Particles of several box sizes, each box with its own blob template
+    (``BOX_TEMPLATES`` unless ``n_blobs`` / ``sigma_range`` are given); per
     particle a Haar rotation, an integer shift, a missing wedge (tilt range
     +-U(tilt_range) about y offset by U(-90, 90) degrees), white noise at
     U(snr_range) dB (signal power over the box's spherical mask, noise added
     before the wedge) and a march end L drawn from ``L_ends``."""
 
     def __init__(self, boxes=(64, 96, 128), ratio=(2, 2, 1), L_ends=(16, 24, 32), seed: int = 0, device="cuda",
-                 n_blobs=None, sigma_range=(1.0, 5.0), tilt_range=(45.0, 65.0), snr_range=(-12.0, 0.0),
+                 n_blobs=None, sigma_range=None, tilt_range=(45.0, 65.0), snr_range=(-12.0, 0.0),
                  max_shift: int = 4):
         self.device = torch.device(device)
         self.boxes = tuple(int(b) for b in boxes)
@@ -154,8 +165,9 @@ class UnevenWorkload:
         self.rng = np.random.default_rng(seed)
         self.tilt_range, self.snr_range, self.max_shift = tilt_range, snr_range, int(max_shift)
         self.radius = {N: float(min(N // 2, int(0.95 * N / 2)) - 2) for N in self.boxes}
-        nb = n_blobs or {N: max(20, int(120 * (N / 96) ** 2)) for N in self.boxes}
-        self.work = {N: Workload(N, nb[N], sigma_range, seed + 100 + N, self.device, mask_radius=self.radius[N])
+        nb = n_blobs or {N: box_template_spec(N)[0] for N in self.boxes}
+        sr = {N: sigma_range or box_template_spec(N)[1] for N in self.boxes}
+        self.work = {N: Workload(N, nb[N], sr[N], seed + 100 + N, self.device, mask_radius=self.radius[N])
                      for N in self.boxes}
 
     def draw(self, n: int):

@@ -69,3 +69,35 @@ def test_uneven_set_is_aligned(gpu_device, workload):
     assert np.mean(np.all(res["shift"].astype(int) == tt[:, ::-1], axis=1)) >= 0.8
     # every bucket that had a full batch was replayed from a graph
     assert any(b.graph is not None for b in al.buckets.values())
+
+
+def test_benchmark_boxes_and_schedules_are_aligned(gpu_device):
+    # the batching workload's shapes: boxes 64 / 96 / 128 (each its own
+    # template), march ends 16 / 24 / 32, missing wedges of +-60 deg, 0 dB,
+    # shifts in [-4, 4]^3; every (box, L) bucket gets particles, and the planted
+    # rotations and integer shifts are recovered
+    from matcha_amd.batching import UnevenAligner
+    from matcha_amd.synthetic import UnevenWorkload, euler_zyz_to_matrix, geodesic_deg
+    wl = UnevenWorkload(seed=21, device=gpu_device, tilt_range=(60.0, 60.0), snr_range=(0.0, 0.0))
+    per = 8
+    specs = [(N, L) for N in wl.boxes for L in wl.L_ends for _ in range(per)]
+    n = len(specs)
+    particles, ctfs = [None] * n, [None] * n
+    Rt = np.zeros((n, 3, 3))
+    tt = np.zeros((n, 3), dtype=int)
+    for N in wl.boxes:
+        idx = [i for i, s in enumerate(specs) if s[0] == N]
+        vols, wedges, R, t, snr, tilt = wl.particles(N, len(idx))
+        for j, i in enumerate(idx):
+            particles[i], ctfs[i], Rt[i], tt[i] = vols[j], wedges[j], R[j], t[j]
+    al = UnevenAligner({N: wl.work[N].template for N in wl.boxes}, gpu_device, radius=wl.radius, bandlimit=32,
+                       batch=16, L_ends=wl.L_ends, precision="bf16x3", graphs=True, max_shift=wl.max_shift)
+    res = al.align(specs, particles, ctfs)
+    err = geodesic_deg(Rt, euler_zyz_to_matrix(res["alpha"].astype(np.float64), res["beta"].astype(np.float64),
+                                               res["gamma"].astype(np.float64)))
+    ok = np.all(res["shift"].astype(int) == tt[:, ::-1], axis=1)
+    assert len(al.buckets) == 9
+    for N in wl.boxes:
+        m = np.array([s[0] == N for s in specs])
+        assert np.median(err[m]) < 1.5, (N, np.sort(err[m]))
+        assert ok[m].mean() >= 0.9, (N, ok[m].mean())
