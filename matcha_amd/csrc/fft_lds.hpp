@@ -14,7 +14,8 @@
 // barrier between its loads and stores, only one after it.  Consumers of the
 // forward output (pointwise products, argmax) map slots to frequencies with
 // freq_of_slot().  The register DFTs are radix-4/3/2 compositions with
-// compile-time twiddles; W_N^{j} comes from an LDS table tw[j] = e^{-2 pi i j/N}.
+// compile-time twiddles (odd primes 5, 7 by their symmetric pairs); W_N^{j}
+// comes from an LDS table tw[j] = e^{-2 pi i j/N}.
 //
 // Work items are laid out with lanes along lines (consecutive lanes handle
 // consecutive lines), which is conflict-free in LDS when lines are contiguous
@@ -98,13 +99,17 @@ MCH_HD void static_for(F&& f) {
   static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
+// the first factor of a composite register DFT: 4, then 3, then 2
+constexpr int dft_split(int R) { return R % 4 == 0 ? 4 : (R % 3 == 0 ? 3 : (R % 2 == 0 ? 2 : 1)); }
+
 // In-register DFT of R points, natural order in and out:
-// X[k] = sum_n v[n] e^{SIGN 2 pi i n k / R}.
+// X[k] = sum_n v[n] e^{SIGN 2 pi i n k / R}.  Composite R = A B (A = 4, 3 or
+// 2) as A-point DFTs, twiddles and B-point DFTs; R = 2, 3, 4, 5, 7 directly.
 template <int R, int SIGN>
 struct Dft {
-  static_assert(R % 4 == 0 && R > 4, "composite register DFT sizes are multiples of 4");
-  static constexpr int A = 4;
-  static constexpr int B = R / 4;
+  static constexpr int A = dft_split(R);
+  static constexpr int B = R / A;
+  static_assert(A > 1 && B > 1, "register DFT sizes: products of 2, 3, 4, 5 and 7");
   static MCH_HD void run(float2* v) {
     float2 t[R];
     static_for<B>([&](auto n2c) {
@@ -135,6 +140,42 @@ struct Dft {
     });
   }
 };
+
+// Odd prime R: X[k] = v0 + sum_{n <= (R-1)/2} (v_n + v_{R-n}) cos(2 pi n k / R)
+//                     + SIGN i (v_n - v_{R-n}) sin(2 pi n k / R).
+template <int R, int SIGN>
+struct DftOdd {
+  static MCH_HD void run(float2* v) {
+    constexpr int H = (R - 1) / 2;
+    float2 s[H + 1], d[H + 1];
+    static_for<H>([&](auto nc) {
+      constexpr int n = decltype(nc)::value + 1;
+      s[n] = cadd(v[n], v[R - n]);
+      d[n] = csub(v[n], v[R - n]);
+    });
+    float2 out[R];
+    float2 x0 = v[0];
+    static_for<H>([&](auto nc) { x0 = cadd(x0, s[decltype(nc)::value + 1]); });
+    out[0] = x0;
+    static_for<H>([&](auto kc) {
+      constexpr int k = decltype(kc)::value + 1;
+      float2 re = v[0], im = make_float2(0.f, 0.f);
+      static_for<H>([&](auto nc) {
+        constexpr int n = decltype(nc)::value + 1;
+        constexpr double a = tw_angle(n * k, R, 1);
+        constexpr float c = (float)ccos(a), sn = (float)csin(a);
+        re = make_float2(re.x + c * s[n].x, re.y + c * s[n].y);
+        im = make_float2(im.x + sn * d[n].x, im.y + sn * d[n].y);
+      });
+      const float2 q = cmuli(im, (float)SIGN);  // SIGN i im
+      out[k] = cadd(re, q);
+      out[R - k] = csub(re, q);
+    });
+    static_for<R>([&](auto kc) { v[decltype(kc)::value] = out[decltype(kc)::value]; });
+  }
+};
+template <int SIGN> struct Dft<5, SIGN> : DftOdd<5, SIGN> {};
+template <int SIGN> struct Dft<7, SIGN> : DftOdd<7, SIGN> {};
 
 template <int SIGN>
 struct Dft<2, SIGN> {
@@ -172,7 +213,7 @@ struct Dft<4, SIGN> {
   }
 };
 
-// N = N1 * N2 with both factors multiples of 4 (supported line lengths)
+// N = N1 * N2 (supported line lengths; the factors are register DFT sizes)
 template <int N>
 struct Shape;
 template <> struct Shape<16> { static constexpr int N1 = 4, N2 = 4; };
@@ -181,6 +222,11 @@ template <> struct Shape<48> { static constexpr int N1 = 4, N2 = 12; };
 template <> struct Shape<64> { static constexpr int N1 = 8, N2 = 8; };
 template <> struct Shape<96> { static constexpr int N1 = 8, N2 = 12; };
 template <> struct Shape<128> { static constexpr int N1 = 8, N2 = 16; };
+template <> struct Shape<80> { static constexpr int N1 = 8, N2 = 10; };
+template <> struct Shape<112> { static constexpr int N1 = 7, N2 = 16; };
+template <> struct Shape<144> { static constexpr int N1 = 12, N2 = 12; };
+template <> struct Shape<160> { static constexpr int N1 = 10, N2 = 16; };
+template <> struct Shape<192> { static constexpr int N1 = 12, N2 = 16; };
 
 // frequency held by slot s of a forward (transposed-order) output, and back
 template <int N>

@@ -43,7 +43,7 @@ template <int N> constexpr int plane_threads() { return 4 * N; }
 // loads issued back to back before their values are used
 template <int PER> constexpr int load_batch() { return PER % 8 == 0 ? 8 : (PER % 4 == 0 ? 4 : (PER % 2 == 0 ? 2 : 1)); }
 
-constexpr int kMaxLut = 2048;  // background-weight table entries (enough for N <= 128)
+constexpr int kMaxLut = 2048;  // background-weight table entries (the 5-voxel cosine shell up to N = 192)
 
 struct ShiftPlan {
   int N, H, max_batch;
@@ -1480,10 +1480,16 @@ int mch_shift_plan_create(int N, const void* ref_host /* complex64 (N,N,N) */, f
     case 32: rc = configure_kernels<32>(); break;
     case 48: rc = configure_kernels<48>(); break;
     case 64: rc = configure_kernels<64>(); break;
+    case 80: rc = configure_kernels<80>(); break;
     case 96: rc = configure_kernels<96>(); break;
+    case 112: rc = configure_kernels<112>(); break;
     case 128: rc = configure_kernels<128>(); break;
+    case 144: rc = configure_kernels<144>(); break;
+    case 160: rc = configure_kernels<160>(); break;
+    case 192: rc = configure_kernels<192>(); break;
     default:
-      mch::set_error("mch_shift_plan_create: box N=%d unsupported (16, 32, 48, 64, 96 or 128)", N);
+      mch::set_error("mch_shift_plan_create: box N=%d unsupported (16, 32, 48, 64, 80, 96, 112, 128, 144, 160 "
+                     "or 192)", N);
       return MCH_ERR_UNSUPPORTED;
   }
   if (rc) return rc;
@@ -1681,9 +1687,15 @@ int mch_shift_plan_set_upsample(void* plan, int up, int radius) {
     case 32: rc = configure_up<32>(R); break;
     case 48: rc = configure_up<48>(R); break;
     case 64: rc = configure_up<64>(R); break;
+    case 80: rc = configure_up<80>(R); break;
     case 96: rc = configure_up<96>(R); break;
+    case 112: rc = configure_up<112>(R); break;
     case 128: rc = configure_up<128>(R); break;
-    default: rc = MCH_ERR_UNSUPPORTED; break;
+    default:
+      // the refinement's plane pass holds a whole (y, z) plane in LDS
+      mch::set_error("mch_shift_plan_set_upsample: sub-voxel refinement supports boxes up to 128 (N=%d)", N);
+      rc = MCH_ERR_UNSUPPORTED;
+      break;
   }
   if (rc) return rc;
   // phases in FP64: e^{2 pi i (u - floor(R/2)) f}, f = fftfreq(n, d=up)
@@ -1744,8 +1756,13 @@ int mch_shift_search(void* plan, const float* subtomos, long long ld_vol, int B,
     case 32: return launch_search<32>(p, a, stream);
     case 48: return launch_search<48>(p, a, stream);
     case 64: return launch_search<64>(p, a, stream);
+    case 80: return launch_search<80>(p, a, stream);
     case 96: return launch_search<96>(p, a, stream);
+    case 112: return launch_search<112>(p, a, stream);
     case 128: return launch_search<128>(p, a, stream);
+    case 144: return launch_search<144>(p, a, stream);
+    case 160: return launch_search<160>(p, a, stream);
+    case 192: return launch_search<192>(p, a, stream);
     default: break;
   }
   mch::set_error("mch_shift_search: plan with unsupported N=%d", N);

@@ -44,7 +44,7 @@ def test_circular_shift_closed_form(gpu_device, N):
     assert np.all(sc.cpu().numpy() > 0.9)
 
 
-@pytest.mark.parametrize("N,B", [(32, 4), (64, 2)])
+@pytest.mark.parametrize("N,B", [(32, 4), (64, 2), (80, 2), (112, 2)])
 def test_matches_oracle_random_rotations(gpu_device, N, B):
     rng = np.random.default_rng(100 + N)
     tmpl = fle.blob_volume(rng, N, 15, sigma_range=(1.5, 3.5))
@@ -109,6 +109,36 @@ def test_full_size_box96(gpu_device):
     rsh, rsc = S.estimate_shift(refF, vols.astype(np.float64), np.zeros((B, 1, 3)), rot, N=N)
     assert np.array_equal(sh.cpu().numpy(), rsh.astype(np.float32))
     assert np.max(np.abs(sc.cpu().numpy() - rsc) / np.abs(rsc)) < 1e-4
+
+
+@pytest.mark.parametrize("N", [144, 160, 192])
+def test_large_boxes_match_oracle(gpu_device, N):
+    # boxes beyond 128 (line lengths 12 x 12, 10 x 16, 12 x 16 in the LDS FFTs):
+    # a planted integer shift under a rotation near the identity (a clear
+    # peak) and one under a random rotation, bit-exact against the oracle
+    from matcha_amd.core.ShiftMatcher import tracker_to_matrices
+    rng = np.random.default_rng(N)
+    tmpl = fle.blob_volume(rng, N, 16, sigma_range=(2.0, 5.0))
+    q = np.array([[np.cos(0.01), np.sin(0.01), 0.0, 0.0], _rand_quats(rng, 1)[0]])
+    rot = tracker_to_matrices(q)
+    t = (5, -7, 3)
+    vols = np.stack([np.roll(tmpl, t, axis=(0, 1, 2)) + 0.2 * rng.standard_normal(tmpl.shape),
+                     np.roll(tmpl, (-2, 1, 4), axis=(0, 1, 2))]).astype(np.float32)
+    sm = _matcher(gpu_device, N, 2)
+    sm.set_reference(tmpl)
+    sh, sc = sm.search_shifts(subtomos=torch.tensor(vols, device=gpu_device), ctfs=None, shift_zyx=None,
+                              rotation_tracker=q)
+    refF = S.load_reference_full(tmpl, N, 1.0).astype(np.complex64).astype(np.complex128)
+    rsh, rsc = S.estimate_shift(refF, vols.astype(np.float64), np.zeros((2, 1, 3)), rot, N=N)
+    assert np.array_equal(sh.cpu().numpy(), rsh.astype(np.float32))
+    assert tuple(sh.cpu().numpy()[0].astype(int)) == t[::-1]
+    assert np.max(np.abs(sc.cpu().numpy() - rsc) / np.abs(rsc)) < 1e-4
+
+
+def test_subvoxel_refinement_beyond_box128_is_rejected(gpu_device):
+    sm = _matcher(gpu_device, 144, 1, shift_upsample_factor=2)
+    with pytest.raises(RuntimeError, match="up to 128"):
+        sm.set_reference(np.zeros((144, 144, 144), dtype=np.float32))
 
 
 def test_extract_patches_matches_oracle(gpu_device):
