@@ -17,7 +17,10 @@ GEMMs, with the 16-element symmetry group D4h of the voxel grid folded in:
   the 16 orbit values that depend only on the class ((l+m) mod 2, m mod 4);
 * a_{lkm} (m >= 0) = sum_{v0} X . W[v0, (l,k,m)] with
   W = h c_lk w(v0) j_l(lambda_lk r0/radius) P_lm(x0) (cos | sin)(m phi0),
-  w = 1/|stabiliser(v0)|; a_{lk,-m} = (-1)^m conj(a_{lkm}) (f is real).
+  w = 1/|stabiliser(v0)|; a_{lk,-m} = (-1)^m conj(a_{lkm}) (f is real);
+* the class m = 3 mod 4 is the conjugate of m = 1 mod 4 (its X2, X3 are
+  negated), so it reuses that operand with its sin rows of W negated and
+  its imaginary output negated: 6 operand slabs instead of 8.
 
 The basis therefore spans ~V_ball/16 voxels and only m >= 0, i.e. about 1/8
 of the dense complex basis.  Tables are built here in FP64 and stored FP32.
@@ -235,8 +238,12 @@ def projection_tables(N: int, radius: float, spectrum, lmax: int, BK: int = 16):
             cc = np.array([cs[ip] for (l, k, m, ip, im) in ch])
             rad = (h * w)[:, None] * J * Pm * cc[None, :]
             ang = phi0[:, None] * mm[None, :]
+            # classes (eps, 3) read the (eps, 1) operand, which has X2 and X3
+            # negated: their sin rows are negated here and their imaginary
+            # output in the kernel's finalize
+            ssign = -1.0 if cls % 4 == 3 else 1.0
             Wb[0:K:2, off:off + n] = rad * np.cos(ang)
-            Wb[1:K:2, off:off + n] = rad * np.sin(ang)
+            Wb[1:K:2, off:off + n] = ssign * rad * np.sin(ang)
             for j, (l, k, m, ip, im) in enumerate(ch):
                 chan_plus[off + j] = ip
                 chan_minus[off + j] = im if m > 0 else -1

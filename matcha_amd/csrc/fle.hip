@@ -78,6 +78,12 @@ struct CorrPlan {
   int* idx;   // (lmax+1, 2*lmax+1, kdim): coefficient index of (l, m, k) or -1
 };
 
+// The operand slab a class reads.  Class (eps, 3) is class (eps, 1) with X2
+// and X3 negated (rho = 3 is the conjugate of rho = 1): it reads the (eps, 1)
+// slab against a basis whose sin rows are negated, and its imaginary output
+// is negated back in the finalize.  Slabs 3 and 7 are never written.
+__host__ __device__ __forceinline__ int x_slab(int cls) { return (cls & 3) == 3 ? cls - 2 : cls; }
+
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -195,7 +201,7 @@ __global__ void fle_symmetrize_kernel(const float* __restrict__ vols, long long 
           A[tt][3] = s1r; Bi[tt][3] = -s1i;  // rho = 3: conjugate of rho = 1
         }
 #pragma unroll
-        for (int rho = 0; rho < 4; ++rho) {
+        for (int rho = 0; rho < 3; ++rho) {  // rho = 3 reads the rho = 1 slab
           const int cls = eps * 4 + rho;
           const float X1 = A[0][rho] + A[1][rho];
           const float X2 = Bi[0][rho] - Bi[1][rho];
@@ -326,10 +332,11 @@ __global__ __launch_bounds__(256) void fle_symmetrize_bf16_kernel(
       });
     }
     // class (eps, rho): G_t = sum_q (-1)^{q eps} sum_p (-i)^{rho p} f[p, q, t]
+    // (rho = 3 reads the rho = 1 slab)
 #pragma unroll
     for (int eps = 0; eps < 2; ++eps)
 #pragma unroll
-      for (int rho = 0; rho < 4; ++rho) {
+      for (int rho = 0; rho < 3; ++rho) {
         unsigned hr[4], lr[4], hm[4], lm[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -389,7 +396,7 @@ __global__ __launch_bounds__(kGemmThreads) void fle_gemm_kernel(
   const int row0 = blockIdx.y * BM;
   int cls = 0;
   while (cls < kClasses - 1 && col0 >= plan_cols.col_off[cls + 1]) ++cls;
-  const float* A = X + (long long)cls * Mpad * Kpad;
+  const float* A = X + (long long)x_slab(cls) * Mpad * Kpad;
   const int kbeg = blockIdx.z * kchunk;
   int kend = kbeg + kchunk;
   if (kend > Kpad) kend = Kpad;
@@ -535,7 +542,7 @@ __global__ __launch_bounds__(64 * NW) void fle_gemm16_kernel(
 
   // this lane's DMA sources: A parts (row block rb, hi/lo) of the row-major X,
   // and this wave's WP of the 16 1-KB parts of the DMA-ordered W^T per stage
-  const long long xbase = ((long long)cls * Mpad + row0) * Kpad + (lane >> 4) * 8;
+  const long long xbase = ((long long)x_slab(cls) * Mpad + row0) * Kpad + (lane >> 4) * 8;
   const uint16_t* asrc[AP];
   int adst[AP];
 #pragma unroll
@@ -665,7 +672,7 @@ __global__ __launch_bounds__(512) void fle_gemm16w_kernel(
   const int kend = min(kbeg + kchunk, Kpad);
   const int nstage = kend > kbeg ? (kend - kbeg) / BK16 : 0;
 
-  const long long xbase = ((long long)cls * Mpad + row0) * Kpad + (lane >> 4) * 8;
+  const long long xbase = ((long long)x_slab(cls) * Mpad + row0) * Kpad + (lane >> 4) * 8;
   const uint16_t* asrc[AP];
   int adst[AP];
 #pragma unroll
@@ -751,7 +758,8 @@ __global__ void fle_xsplit_kernel(const float* __restrict__ X, const float* __re
        q += (long long)gridDim.x * blockDim.x) {
     const long long e = q * 4;
     const int row = (int)((e / Kpad) % Mpad);
-    if (row >= rows) continue;
+    const int cls = (int)(e / ((long long)Mpad * Kpad));
+    if (row >= rows || x_slab(cls) != cls) continue;
     const float rs = pow2_scale(rowmax[row]);
     const float4 v = reinterpret_cast<const float4*>(X)[q];
     uint16_t h[4], l[4];
@@ -836,7 +844,7 @@ __global__ void fle_finalize_kernel(const float* __restrict__ part, int splits, 
                                     const int* __restrict__ chan_minus,
                                     const int* __restrict__ chan_m, float2* __restrict__ a,
                                     long long lda, const float* __restrict__ colscale,
-                                    const float* __restrict__ rowmax) {
+                                    const float* __restrict__ rowmax, int4 flip) {
   const long long tot = (long long)B * Ntot;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < tot;
        t += (long long)gridDim.x * blockDim.x) {
@@ -854,6 +862,9 @@ __global__ void fle_finalize_kernel(const float* __restrict__ part, int splits, 
       re /= colscale[c] * pow2_scale(rowmax[b]);
       im /= colscale[c] * pow2_scale(rowmax[B + b]);
     }
+    // columns of the classes (eps, 3), [flip.x, flip.y) and [flip.z, flip.w):
+    // their operand is the (eps, 1) slab, whose imaginary output is negated
+    if ((c >= flip.x && c < flip.y) || (c >= flip.z && c < flip.w)) im = -im;
     a[(long long)b * lda + ip] = make_float2(re, im);
     const int imn = chan_minus[c];
     if (imn >= 0) {
@@ -1421,7 +1432,8 @@ int mch_fle_project_shifted(void* plan, const float* vols, long long ld_vol, int
   hipLaunchKernelGGL(fle_finalize_kernel, dim3(fb), dim3(256), 0, stream,
                      p->precision == kFp32 ? p->part : p->part16, splits, p->Mpad, p->Ntot, B,
                      p->chan_plus, p->chan_minus, p->chan_m, (float2*)coeffs, lda,
-                     fp16 ? p->colscale : nullptr, fp16 ? p->rowmax : nullptr);
+                     fp16 ? p->colscale : nullptr, fp16 ? p->rowmax : nullptr,
+                     make_int4(p->col_off[3], p->col_off[4], p->col_off[7], p->col_off[8]));
   return mch::hip_check(hipGetLastError(), "fle_finalize_kernel launch");
 }
 

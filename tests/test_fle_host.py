@@ -88,7 +88,12 @@ def emulate_projection(t, vols, ne):
     off = 0
     for c in range(8):
         n = t["class_colpad"][c]
-        out[:, off:off + n] = X[c] @ W[:, off:off + n]
+        # as the kernels: classes (eps, 3) read the (eps, 1) operand against their
+        # basis (sin rows negated) and negate the imaginary output
+        slab = c - 2 if c % 4 == 3 else c
+        out[:, off:off + n] = X[slab] @ W[:, off:off + n]
+        if c % 4 == 3:
+            out[B:, off:off + n] *= -1
         off += n
     a = np.zeros((B, ne), dtype=complex)
     for col in range(Ntot):
