@@ -582,8 +582,9 @@ def main():
         plain = run_path("plain", torch, dev, w, args, pool_vols, pool_R, pool_t, "fp32", Bp, [plain_plans])
 
     # the tuned path's projection GEMM: algorithmic bytes = W^T hi + lo (4 B per
-    # element) + the hi/lo X operand (4 B per element), each read once;
-    # algorithmic flops = one product 2 (2B) (2 V0) sum(columns); the GEMM issues
+    # element) + the nonzero part of the hi/lo X operand (4 B per element), each
+    # read once; algorithmic flops = one product over the nonzero K of each
+    # class (V0 for m = 0, 2 mod 4, 2 V0 for m = 1, 3 mod 4); the GEMM issues
     # three (hi hi + hi lo + lo hi), reported as mfma_issue_frac
     fle_ = ccm.fle
     gemm_t = time_gemm(torch, dev, ccm, pool_vols[:Bt], args.tuned_precision)
@@ -591,10 +592,10 @@ def main():
     march_fl = march_flops(matcher.levels, matcher.num_steps, matcher.candidates, Bt)
     gemm_plain = time_gemm(torch, dev, plain_plans[0], pool_vols[:Bp], "fp32") if plain else None
     w_bytes = 4.0 * fle_.Ntot * fle_.Kpad          # W^T hi + lo, 2 + 2 B per element
-    x_bytes = 4.0 * 8 * (2 * Bt) * fle_.Kpad       # X hi + lo of the 8 classes
+    x_bytes = Bt * fle_.operand_bytes_per_volume   # X hi + lo, nonzero halves of the 6 slabs
     alg_bytes = w_bytes + x_bytes
     achieved = alg_bytes / gemm_t / 1e9
-    flops_alg = 2.0 * (2 * Bt) * (2 * fle_.V0) * sum(fle_.class_cols)   # one product
+    flops_alg = Bt * fle_.flops_per_volume                               # one product
     flops16 = 3 * flops_alg                                              # the three split products issued
     fle_.set_compute_precision(args.tuned_precision)
     depth = int(_lib_depth(fle_))

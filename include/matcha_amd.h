@@ -103,10 +103,13 @@ int mch_search_orientations(void* plan, const void* sigma, long long ld_sigma, i
 
 /* Plan of the symmetry-adapted ball-harmonic projection: orbit table of the
  * fundamental voxels (V0 x 16 flat indices, image p + 4q + 8t of the voxel,
- * -1 where that image is outside the array), FP32 basis (Kpad x Ntot,
- * K = 2*V0; the classes m = 3 mod 4 hold their sin rows negated: they share
- * the operand of the classes m = 1 mod 4), per-column output maps.  Replaces FLEBasis3D.precomp (src/matcha/core/
- * FLEBasis3D.py:111-540: NUFFT plans, SHT, interpolation matrices). */
+ * -1 where that image is outside the array), FP32 basis (Kpad x Ntot: the
+ * cos-weighted row of voxel v at k = v, the sin-weighted one at
+ * k = Kpad/2 + v, Kpad a multiple of 128 with Kpad/2 >= V0; the classes
+ * m = 3 mod 4 hold their sin rows negated: they share the operand of the
+ * classes m = 1 mod 4), per-column output maps.  Replaces FLEBasis3D.precomp
+ * (src/matcha/core/FLEBasis3D.py:111-540: NUFFT plans, SHT, interpolation
+ * matrices). */
 int mch_fle_plan_create(int N, int V0, int Kpad, int Ntot, const int* class_colpad,
                         const int* orbit, const float* basis, const int* chan_plus,
                         const int* chan_minus, const int* chan_m, int ne, int max_batch,

@@ -213,8 +213,12 @@ def projection_tables(N: int, radius: float, spectrum, lmax: int, BK: int = 16):
         classes[cls].append((l, k, m, i, index.get((l, k, -m), -1)))
     for c in classes:
         c.sort()
-    K = 2 * V0
-    Kpad = ((K + BK - 1) // BK) * BK
+    # K order: the cos-weighted operand of fundamental voxel v at k = v, the
+    # sin-weighted one at k = Kpad/2 + v (classes m = 0, 2 mod 4 have a zero
+    # sin half in their real rows and a zero cos half in their imaginary rows,
+    # which the GEMM then skips)
+    KH = ((V0 + BK - 1) // BK) * BK
+    Kpad = 2 * KH
     cols = [len(c) for c in classes]
     colpad = [((n + COL_PAD - 1) // COL_PAD) * COL_PAD for n in cols]
     Ntot = int(sum(colpad))
@@ -242,8 +246,8 @@ def projection_tables(N: int, radius: float, spectrum, lmax: int, BK: int = 16):
             # negated: their sin rows are negated here and their imaginary
             # output in the kernel's finalize
             ssign = -1.0 if cls % 4 == 3 else 1.0
-            Wb[0:K:2, off:off + n] = rad * np.cos(ang)
-            Wb[1:K:2, off:off + n] = ssign * rad * np.sin(ang)
+            Wb[0:V0, off:off + n] = rad * np.cos(ang)
+            Wb[KH:KH + V0, off:off + n] = ssign * rad * np.sin(ang)
             for j, (l, k, m, ip, im) in enumerate(ch):
                 chan_plus[off + j] = ip
                 chan_minus[off + j] = im if m > 0 else -1
@@ -395,5 +399,18 @@ class FLEBasis3D:
 
     @property
     def flops_per_volume(self) -> float:
-        """Algorithmic FLOPs of the projection GEMM per volume."""
-        return 2.0 * 2 * self.Kpad * self.Ntot
+        """Algorithmic FLOPs of the projection GEMM per volume: two operand rows
+        (real and imaginary) against every basis column, over V0 voxels for the
+        classes m = 0, 2 mod 4 (one of the cos / sin halves of their rows is
+        zero) and 2 V0 for the classes m = 1, 3 mod 4."""
+        even = sum(c for i, c in enumerate(self.class_cols) if i % 2 == 0)
+        odd = sum(c for i, c in enumerate(self.class_cols) if i % 2 == 1)
+        return 2.0 * 2 * self.V0 * (even + 2 * odd)
+
+    @property
+    def operand_bytes_per_volume(self) -> float:
+        """Algorithmic bytes of the projection operand X per volume at 4 B per
+        element (FP32, or BF16/FP16 hi + lo): the six stored class slabs (the
+        classes m = 3 mod 4 read their m = 1 mod 4 slab), two rows each, the
+        zero half of the m = 0, 2 mod 4 slabs not read."""
+        return 4.0 * 2 * (4 * self.Kpad / 2 + 2 * self.Kpad)

@@ -8,7 +8,9 @@
 //   1. fle_symmetrize_kernel: each fundamental voxel's 16 orbit values ->
 //      4 combinations per class (8 classes), written as the GEMM A operand
 //      X[class][row][k] with rows (b: real part, B+b: imaginary part) and
-//      k = 2*v0 + {0: cos-weighted, 1: sin-weighted}.
+//      k = v0 (cos-weighted) or Kpad/2 + v0 (sin-weighted); for the classes
+//      m = 0, 2 mod 4 the real rows' sin half and the imaginary rows' cos
+//      half are zero, and GEMM row tiles wholly of one kind skip that half.
 //   2. the projection GEMM Out[row][col] = sum_k X[class(col)][row][k] W[k][col],
 //      split over K, in the plan's precision:
 //        FP32 (plain path): fle_gemm_kernel, v_mfma_f32_16x16x4f32;
@@ -83,6 +85,19 @@ struct CorrPlan {
 // slab against a basis whose sin rows are negated, and its imaginary output
 // is negated back in the finalize.  Slabs 3 and 7 are never written.
 __host__ __device__ __forceinline__ int x_slab(int cls) { return (cls & 3) == 3 ? cls - 2 : cls; }
+
+// K range a GEMM row tile [row0, row0 + rows) of class cls needs: the classes
+// m = 0, 2 mod 4 (cls even) have zero sin-weighted operands in their real
+// rows (< B) and zero cos-weighted ones in their imaginary rows, so a tile
+// wholly of one kind reads one half of K; other tiles read all of it.
+__device__ __forceinline__ void k_range(int cls, int row0, int rows, int B, int Kpad, int& klo, int& khi) {
+  klo = 0;
+  khi = Kpad;
+  if ((cls & 1) == 0) {
+    if (row0 + rows <= B) khi = Kpad / 2;
+    else if (row0 >= B) klo = Kpad / 2;
+  }
+}
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
@@ -207,10 +222,13 @@ __global__ void fle_symmetrize_kernel(const float* __restrict__ vols, long long 
           const float X2 = Bi[0][rho] - Bi[1][rho];
           const float X3 = Bi[0][rho] + Bi[1][rho];
           const float X4 = A[1][rho] - A[0][rho];
-          const long long orow = ((long long)cls * Mpad + b) * Kpad + 2 * v;
-          const long long oim = ((long long)cls * Mpad + B + b) * Kpad + 2 * v;
-          *reinterpret_cast<float2*>(X + orow) = make_float2(X1, X2);
-          *reinterpret_cast<float2*>(X + oim) = make_float2(X3, X4);
+          // cos-weighted operand at k = v, sin-weighted at k = Kpad / 2 + v
+          const long long orow = ((long long)cls * Mpad + b) * Kpad + v;
+          const long long oim = ((long long)cls * Mpad + B + b) * Kpad + v;
+          X[orow] = X1;
+          X[orow + Kpad / 2] = X2;
+          X[oim] = X3;
+          X[oim + Kpad / 2] = X4;
           mre = fmaxf(mre, fmaxf(fabsf(X1), fabsf(X2)));
           mim = fmaxf(mim, fmaxf(fabsf(X3), fabsf(X4)));
         }
@@ -249,9 +267,9 @@ __global__ void fle_symmetrize_kernel(const float* __restrict__ vols, long long 
 // BF16x3 path: the same combinations for 4 consecutive fundamental voxels per
 // thread.  Their 64 orbit values are requested at once (indices from the
 // packed coordinates: image p + 4q + 8t of (x, y, z) is R^p My^t Mz^q, R the
-// quarter turn (x, y) -> (-y, x)), and each row's 4 x (X1, X2) or (X3, X4)
-// hi/lo pairs leave as one 16-byte store.  Columns of a last partial group
-// (v >= V0) receive zeros, inside the row's K padding.
+// quarter turn (x, y) -> (-y, x)), and each row's 4 cos-weighted and 4
+// sin-weighted values leave as 8-byte stores of hi and of lo.  Columns of a
+// last partial group (v >= V0) receive zeros, inside the half's K padding.
 template <bool SHIFTED>
 __global__ __launch_bounds__(256) void fle_symmetrize_bf16_kernel(
     const float* __restrict__ vols, long long ld_vol, int B, const int* __restrict__ fcoord, int V0, int N,
@@ -337,7 +355,7 @@ __global__ __launch_bounds__(256) void fle_symmetrize_bf16_kernel(
     for (int eps = 0; eps < 2; ++eps)
 #pragma unroll
       for (int rho = 0; rho < 3; ++rho) {
-        unsigned hr[4], lr[4], hm[4], lm[4];
+        uint16_t hx[4][4], lx[4][4];  // [X1..X4][voxel u] hi and lo
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           float A[2], Bv[2];
@@ -360,23 +378,28 @@ __global__ __launch_bounds__(256) void fle_symmetrize_bf16_kernel(
             A[tt] = a;
             Bv[tt] = rho == 3 ? -bi : bi;  // rho = 3: conjugate of rho = 1
           }
-          uint16_t h1, l1, h2, l2, h3, l3, h4, l4;
-          Split16<false>::split(A[0] + A[1], h1, l1);
-          Split16<false>::split(Bv[0] - Bv[1], h2, l2);
-          Split16<false>::split(Bv[0] + Bv[1], h3, l3);
-          Split16<false>::split(A[1] - A[0], h4, l4);
-          hr[u] = (unsigned)h1 | ((unsigned)h2 << 16);
-          lr[u] = (unsigned)l1 | ((unsigned)l2 << 16);
-          hm[u] = (unsigned)h3 | ((unsigned)h4 << 16);
-          lm[u] = (unsigned)l3 | ((unsigned)l4 << 16);
+          Split16<false>::split(A[0] + A[1], hx[0][u], lx[0][u]);
+          Split16<false>::split(Bv[0] - Bv[1], hx[1][u], lx[1][u]);
+          Split16<false>::split(Bv[0] + Bv[1], hx[2][u], lx[2][u]);
+          Split16<false>::split(A[1] - A[0], hx[3][u], lx[3][u]);
         }
+        // the cos-weighted operands of the 4 voxels at k = 4 qd.., the sin-weighted
+        // ones at k = Kpad / 2 + 4 qd..: 8 bytes of hi and of lo per row and half
+        auto pk = [](const uint16_t (&h)[4]) {
+          return make_uint2((unsigned)h[0] | ((unsigned)h[1] << 16), (unsigned)h[2] | ((unsigned)h[3] << 16));
+        };
         const int cls = eps * 4 + rho;
-        const long long orow = ((long long)cls * Mpad + b) * Kpad + 8LL * qd;
-        const long long oim = ((long long)cls * Mpad + B + b) * Kpad + 8LL * qd;
-        *reinterpret_cast<uint4*>(Xhi + orow) = make_uint4(hr[0], hr[1], hr[2], hr[3]);
-        *reinterpret_cast<uint4*>(Xlo + orow) = make_uint4(lr[0], lr[1], lr[2], lr[3]);
-        *reinterpret_cast<uint4*>(Xhi + oim) = make_uint4(hm[0], hm[1], hm[2], hm[3]);
-        *reinterpret_cast<uint4*>(Xlo + oim) = make_uint4(lm[0], lm[1], lm[2], lm[3]);
+        const long long orow = ((long long)cls * Mpad + b) * Kpad + 4LL * qd;
+        const long long oim = ((long long)cls * Mpad + B + b) * Kpad + 4LL * qd;
+        const int kh = Kpad / 2;
+        *reinterpret_cast<uint2*>(Xhi + orow) = pk(hx[0]);
+        *reinterpret_cast<uint2*>(Xlo + orow) = pk(lx[0]);
+        *reinterpret_cast<uint2*>(Xhi + orow + kh) = pk(hx[1]);
+        *reinterpret_cast<uint2*>(Xlo + orow + kh) = pk(lx[1]);
+        *reinterpret_cast<uint2*>(Xhi + oim) = pk(hx[2]);
+        *reinterpret_cast<uint2*>(Xlo + oim) = pk(lx[2]);
+        *reinterpret_cast<uint2*>(Xhi + oim + kh) = pk(hx[3]);
+        *reinterpret_cast<uint2*>(Xlo + oim + kh) = pk(lx[3]);
       }
   }
 }
@@ -388,7 +411,7 @@ __global__ __launch_bounds__(256) void fle_symmetrize_bf16_kernel(
 // Each of the 4 waves computes 32 rows x 64 columns as 2 x 4 blocks of 16x16.
 __global__ __launch_bounds__(kGemmThreads) void fle_gemm_kernel(
     const float* __restrict__ X, const float* __restrict__ W, float* __restrict__ part,
-    int Mpad, int Kpad, int Ntot, int kchunk, FlePlan plan_cols) {
+    int Mpad, int Kpad, int Ntot, int kchunk, int B, FlePlan plan_cols) {
   __shared__ float As[2][BK][BM + 4];
   __shared__ float Bs[2][BK][BN + 4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -397,9 +420,10 @@ __global__ __launch_bounds__(kGemmThreads) void fle_gemm_kernel(
   int cls = 0;
   while (cls < kClasses - 1 && col0 >= plan_cols.col_off[cls + 1]) ++cls;
   const float* A = X + (long long)x_slab(cls) * Mpad * Kpad;
-  const int kbeg = blockIdx.z * kchunk;
-  int kend = kbeg + kchunk;
-  if (kend > Kpad) kend = Kpad;
+  int klo, khi;
+  k_range(cls, row0, BM, B, Kpad, klo, khi);
+  const int kbeg = max(blockIdx.z * kchunk, klo);
+  int kend = min(blockIdx.z * kchunk + kchunk, khi);
 
   floatx4 acc[2][4];
 #pragma unroll
@@ -513,7 +537,7 @@ __device__ __forceinline__ void lds_dma16(const void* g, u32x4* lds_wave_base) {
 template <bool FP16, int NST, int NW>
 __global__ __launch_bounds__(64 * NW) void fle_gemm16_kernel(
     const uint16_t* __restrict__ Xhi, const uint16_t* __restrict__ Xlo, const u32x4* __restrict__ Wt16,
-    float* __restrict__ part, int Mpad, int Kpad, int Ntot, int kchunk, int ntn, int nmt, FlePlan cols) {
+    float* __restrict__ part, int Mpad, int Kpad, int Ntot, int kchunk, int ntn, int nmt, int B, FlePlan cols) {
   static_assert(NST >= 2 && NST <= 3, "pipeline depth");
   static_assert(NW == 2 || NW == 4 || NW == 8, "waves per workgroup");
   constexpr int RT = gemm16_row_tiles(NW);  // 128-row tiles per workgroup (8 waves: 2)
@@ -536,8 +560,10 @@ __global__ __launch_bounds__(64 * NW) void fle_gemm16_kernel(
   const int col0 = tn * BN16, row0 = mt * BM16 * RT;
   int cls = 0;
   while (cls < kClasses - 1 && col0 >= cols.col_off[cls + 1]) ++cls;
-  const int kbeg = ks * kchunk;
-  const int kend = min(kbeg + kchunk, Kpad);
+  int klo, khi;
+  k_range(cls, row0, BM16 * RT, B, Kpad, klo, khi);
+  const int kbeg = max(ks * kchunk, klo);
+  const int kend = min(ks * kchunk + kchunk, khi);
   const int nstage = kend > kbeg ? (kend - kbeg) / BK16 : 0;
 
   // this lane's DMA sources: A parts (row block rb, hi/lo) of the row-major X,
@@ -648,7 +674,7 @@ constexpr int kWideStageU4 = 4096;  // A hi 1024, A lo 1024, W 2 tiles x 1024
 template <bool FP16>
 __global__ __launch_bounds__(512) void fle_gemm16w_kernel(
     const uint16_t* __restrict__ Xhi, const uint16_t* __restrict__ Xlo, const u32x4* __restrict__ Wt16,
-    float* __restrict__ part, int Mpad, int Kpad, int Ntot, int kchunk, int nunits, int nmt, FlePlan cols) {
+    float* __restrict__ part, int Mpad, int Kpad, int Ntot, int kchunk, int nunits, int nmt, int B, FlePlan cols) {
   constexpr int NST = 2, NW = 8, AREG = 1024, W0 = 2048, AP = 4, WP = 4, RB = 4, CB = 8;
   extern __shared__ u32x4 lds[];   // NST x kWideStageU4
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -668,8 +694,10 @@ __global__ __launch_bounds__(512) void fle_gemm16w_kernel(
   const int ntile = min(2, (cols.col_off[cls + 1] - col0) / BN16);
   const int tn0 = col0 / BN16, tn1 = ntile == 2 ? tn0 + 1 : tn0;
   const int row0 = mt * 2 * BM16;
-  const int kbeg = ks * kchunk;
-  const int kend = min(kbeg + kchunk, Kpad);
+  int klo, khi;
+  k_range(cls, row0, 2 * BM16, B, Kpad, klo, khi);
+  const int kbeg = max(ks * kchunk, klo);
+  const int kend = min(ks * kchunk + kchunk, khi);
   const int nstage = kend > kbeg ? (kend - kbeg) / BK16 : 0;
 
   const long long xbase = ((long long)x_slab(cls) * Mpad + row0) * Kpad + (lane >> 4) * 8;
@@ -1044,7 +1072,7 @@ int launch_gemm(FlePlan* p, int B, hipStream_t stream, int* splits_out) {
     const int kchunk = ((kblocks + p->splits - 1) / p->splits) * BK;
     const int splits = (p->Kpad + kchunk - 1) / kchunk;
     hipLaunchKernelGGL(fle_gemm_kernel, dim3(p->Ntot / BN, mt, splits), dim3(kGemmThreads), 0,
-                       stream, p->X, p->W, p->part, p->Mpad, p->Kpad, p->Ntot, kchunk, *p);
+                       stream, p->X, p->W, p->part, p->Mpad, p->Kpad, p->Ntot, kchunk, B, *p);
     *splits_out = splits;
     return mch::hip_check(hipGetLastError(), "fle_gemm_kernel launch");
   }
@@ -1061,10 +1089,10 @@ int launch_gemm(FlePlan* p, int B, hipStream_t stream, int* splits_out) {
     const size_t ldsw = gemm16_lds(2, 8, 2);
     if (fp16)
       hipLaunchKernelGGL(fle_gemm16w_kernel<true>, gridw, dim3(512), ldsw, stream, p->Xhi, p->Xlo,
-                         (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, kchunk, nunits, mt16, *p);
+                         (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, kchunk, nunits, mt16, B, *p);
     else
       hipLaunchKernelGGL(fle_gemm16w_kernel<false>, gridw, dim3(512), ldsw, stream, p->Xhi, p->Xlo,
-                         (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, kchunk, nunits, mt16, *p);
+                         (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, kchunk, nunits, mt16, B, *p);
     *splits_out = splits;
     return mch::hip_check(hipGetLastError(), "fle_gemm16w_kernel launch");
   }
@@ -1072,7 +1100,7 @@ int launch_gemm(FlePlan* p, int B, hipStream_t stream, int* splits_out) {
   const size_t lds = gemm16_lds(p->nst16, p->nw16);
 #define MCH_GEMM16(F, S, W)                                                                        \
   hipLaunchKernelGGL((fle_gemm16_kernel<F, S, W>), grid, dim3(64 * W), lds, stream, p->Xhi, p->Xlo, \
-                     (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, kchunk, ntn, mt16, *p)
+                     (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, kchunk, ntn, mt16, B, *p)
 #define MCH_GEMM16_NW(W)                               \
   do {                                                 \
     if (fp16 && p->nst16 == 3) MCH_GEMM16(true, 3, W); \
@@ -1113,7 +1141,8 @@ int mch_fle_plan_create(int N, int V0, int Kpad, int Ntot, const int* class_colp
     }
     sum += class_colpad[c];
   }
-  if (N < 2 || V0 < 1 || Kpad < 2 * V0 || Kpad % BK || sum != Ntot || ne < 1 || max_batch < 1) {
+  if (N < 2 || V0 < 1 || Kpad < 2 * V0 || Kpad % (2 * 64) || Kpad / 2 < V0 || sum != Ntot || ne < 1 ||
+      max_batch < 1) {
     mch::set_error("mch_fle_plan_create: inconsistent sizes (N=%d V0=%d Kpad=%d Ntot=%d ne=%d)", N,
                    V0, Kpad, Ntot, ne);
     return MCH_ERR_SIZE;

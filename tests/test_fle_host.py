@@ -77,12 +77,13 @@ def emulate_projection(t, vols, ne):
                 s1i = s1i + sg * (h[..., 3] - h[..., 1])
             A[tt, 0], A[tt, 1], A[tt, 2], A[tt, 3] = s0, s1r, s2, s1r
             Bi[tt, 1], Bi[tt, 3] = s1i, -s1i
+        KH = Kpad // 2  # cos-weighted operand at k = v, sin-weighted at KH + v
         for rho in range(4):
             c = eps * 4 + rho
-            X[c, :B, 0:2 * V0:2] = A[0, rho] + A[1, rho]
-            X[c, :B, 1:2 * V0:2] = Bi[0, rho] - Bi[1, rho]
-            X[c, B:, 0:2 * V0:2] = Bi[0, rho] + Bi[1, rho]
-            X[c, B:, 1:2 * V0:2] = A[1, rho] - A[0, rho]
+            X[c, :B, 0:V0] = A[0, rho] + A[1, rho]
+            X[c, :B, KH:KH + V0] = Bi[0, rho] - Bi[1, rho]
+            X[c, B:, 0:V0] = Bi[0, rho] + Bi[1, rho]
+            X[c, B:, KH:KH + V0] = A[1, rho] - A[0, rho]
     W = t["W"].astype(np.float64)
     out = np.zeros((2 * B, Ntot))
     off = 0
