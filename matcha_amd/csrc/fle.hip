@@ -99,6 +99,15 @@ __device__ __forceinline__ void k_range(int cls, int row0, int rows, int B, int 
   }
 }
 
+// Whether operand row `row` lies in the one GEMM row tile that holds both real
+// and imaginary rows (tiles are 128 or 256 rows, aligned to their height, so
+// it lies inside the aligned 256 rows around B).  Only there is the zero half
+// of a class m = 0, 2 mod 4 row read, so only there is it stored; elsewhere it
+// keeps whatever an earlier batch left.
+__device__ __forceinline__ bool straddle_row(int row, int B) {
+  return (B & 255) != 0 && row >= (B & ~255) && row < ((B + 255) & ~255);
+}
+
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -225,9 +234,11 @@ __global__ void fle_symmetrize_kernel(const float* __restrict__ vols, long long 
           // cos-weighted operand at k = v, sin-weighted at k = Kpad / 2 + v
           const long long orow = ((long long)cls * Mpad + b) * Kpad + v;
           const long long oim = ((long long)cls * Mpad + B + b) * Kpad + v;
+          // (X2 and X3 are zero for rho = 0, 2: stored only where a GEMM tile reads them)
+          const bool odd = rho & 1;
           X[orow] = X1;
-          X[orow + Kpad / 2] = X2;
-          X[oim] = X3;
+          if (odd || straddle_row(b, B)) X[orow + Kpad / 2] = X2;
+          if (odd || straddle_row(B + b, B)) X[oim] = X3;
           X[oim + Kpad / 2] = X4;
           mre = fmaxf(mre, fmaxf(fabsf(X1), fabsf(X2)));
           mim = fmaxf(mim, fmaxf(fabsf(X3), fabsf(X4)));
@@ -392,12 +403,18 @@ __global__ __launch_bounds__(256) void fle_symmetrize_bf16_kernel(
         const long long orow = ((long long)cls * Mpad + b) * Kpad + 4LL * qd;
         const long long oim = ((long long)cls * Mpad + B + b) * Kpad + 4LL * qd;
         const int kh = Kpad / 2;
+        // (X2 and X3 are zero for rho = 0, 2: stored only where a GEMM tile reads them)
+        const bool odd = rho & 1;
         *reinterpret_cast<uint2*>(Xhi + orow) = pk(hx[0]);
         *reinterpret_cast<uint2*>(Xlo + orow) = pk(lx[0]);
-        *reinterpret_cast<uint2*>(Xhi + orow + kh) = pk(hx[1]);
-        *reinterpret_cast<uint2*>(Xlo + orow + kh) = pk(lx[1]);
-        *reinterpret_cast<uint2*>(Xhi + oim) = pk(hx[2]);
-        *reinterpret_cast<uint2*>(Xlo + oim) = pk(lx[2]);
+        if (odd || straddle_row(b, B)) {
+          *reinterpret_cast<uint2*>(Xhi + orow + kh) = pk(hx[1]);
+          *reinterpret_cast<uint2*>(Xlo + orow + kh) = pk(lx[1]);
+        }
+        if (odd || straddle_row(B + b, B)) {
+          *reinterpret_cast<uint2*>(Xhi + oim) = pk(hx[2]);
+          *reinterpret_cast<uint2*>(Xlo + oim) = pk(lx[2]);
+        }
         *reinterpret_cast<uint2*>(Xhi + oim + kh) = pk(hx[3]);
         *reinterpret_cast<uint2*>(Xlo + oim + kh) = pk(lx[3]);
       }

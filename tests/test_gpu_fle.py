@@ -242,3 +242,32 @@ def test_every_gemm_tile_configuration_matches_fp32(gpu_device, mode):
     assert len(tested) == 14
     with pytest.raises(RuntimeError):
         f.set_gemm_tiles(3, 8, 2, 1)   # not a candidate (the wide tile has two stages)
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16x3", "fp16x3"])
+def test_batches_of_changing_size_reuse_the_operand(gpu_device, mode):
+    # The operand rows of the classes m = 0, 2 mod 4 have a zero half (the sin
+    # half of the real rows, the cos half of the imaginary rows) that only the
+    # GEMM row tile holding both kinds reads; the symmetrize pass stores it only
+    # there, so rows elsewhere keep an earlier batch's values.  A sequence of
+    # batch sizes that moves the real/imaginary boundary across row tiles
+    # (and leaves nonzero stale values where a later straddling tile reads)
+    # gives the oracle's coefficients at every call and in every tile shape.
+    from matcha_amd.core.FLEBasis3D import FLEBasis3D
+    N, bl, radius = 24, 8, 11
+    f = FLEBasis3D(N, bl, 1e-4, batchsize=256, radius=radius, device=gpu_device, compute_precision=mode)
+    rng = np.random.default_rng(41)
+    sizes = [200, 60, 256, 130, 200, 1, 255]
+    vols = [rng.standard_normal((B, N, N, N)).astype(np.float32) for B in sizes]
+    refs = [fle.evaluate_t(v.astype(np.float64), N, bl, radius) for v in vols]
+    tiles = [None]
+    if mode == "bf16x3":
+        tiles = [(2, 4, 1, 2), (3, 2, 1, 1), (2, 8, 1, 2), (2, 8, 2, 2)]
+    tol = SPLIT_TOL.get(mode, 2e-5)
+    for t in tiles:
+        if t is not None:
+            f.set_gemm_tiles(*t)
+        for B, v, ref in zip(sizes, vols, refs):
+            got = f.evaluate_t(torch.tensor(v, device=gpu_device)).cpu().numpy()
+            err = np.max(np.abs(got - ref), axis=1) / np.max(np.abs(ref), axis=1)
+            assert err.max() < tol, (t, B, int(np.argmax(err)), err.max())
