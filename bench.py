@@ -1,6 +1,6 @@
 """Benchmark: aligned volumes/s for Matcha's orientation-search hot path on one MI355X.
 
-Headline (the metric's own definition): the tuned path -- batched (512
+Headline (the metric's own definition): the tuned path -- batched (768
 particles per batch, consecutive batches alternating over two streams so that
 independent batches overlap on the device) with the projection GEMM on 16-bit
 hi/lo split operands (three MFMA products, FP32 accumulation) and every other
@@ -24,13 +24,15 @@ U(1,5) px, amplitude U(0.5,1), centres in a ball of 0.35*96), particles =
 Haar-random rotation + integer shift in [-6,6]^3 + white noise at 0 dB over
 the radius-45 mask; FLE bandlimit 32 (ne 8252, lmax 42), radius 45; march
 Li = 8, 16, 24 -> L_max 32; SOFFT grid 64^3 (L0 = 8, K = 4); 10 candidates;
-5 Newton steps; plain path batch 64, tuned path batch 512.
+5 Newton steps; plain path batch 64, tuned path batch 768 (1536 particles
+cycled).
 
-Three kernels take 18-21% of the step each: the tuned projection GEMM
-(``roofline_gemm16``: MFMA bound, ``frac`` on the algorithmic one-product
-FLOPs, the issue rate of the three split products in ``mfma_issue_frac``),
-the fused Newton march (``roofline_march``: FP32 vector bound) and the shift
-search's column pass (``roofline_shift_z``: HBM bound); each is timed live
+Three kernels take 17-24% of the step each: the tuned projection GEMM
+(``roofline_gemm16``: HBM bound by its algorithmic operands, ``frac`` on
+those bytes, ``mfma_frac`` on the one-product FLOPs over the nonzero operand,
+the issue rate of the three split products in ``mfma_issue_frac``), the fused
+Newton march (``roofline_march``: FP32 vector bound) and the shift search's
+column pass (``roofline_shift_z``: HBM bound); each is timed live
 with HIP events on its stream.  ``roofline`` repeats the one with the largest
 share of the step in this run.  ``cpu_baseline`` is a torch-CPU FP32
 restatement of the reference's algorithm (NUFFT projection, SHT, radial
@@ -57,19 +59,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 # 256-row tile).  Its writes are the K-split
 # partials, splits x rows x Ntot x 4 B (WRITE_SIZE reads exactly that), added
 # per run for the split the autotuner chose.
-GEMM16_FETCH_BYTES = {("box96_L32", 64, 4, 1): 587089 * 1024 * 2, ("box96_L32", 512, 8, 1): 1720199 * 1024 * 2,
-                      ("box96_L32", 512, 8, 2): 1434256 * 1024 * 2}
+GEMM16_FETCH_BYTES = {("box96_L32", 64, 4, 1): 587089 * 1024 * 2, ("box96_L32", 512, 8, 2): 1070211 * 1024 * 2,
+                      ("box96_L32", 768, 8, 2): 1501459 * 1024 * 2}
 # HBM bytes (FETCH_SIZE x 2.0 + WRITE_SIZE) per launch of the shift search's
 # column pass, keyed by (workload, batch, reference table precision); measured
 # like the GEMM's.
-SHIFT_Z_TRAFFIC_BYTES = {("box96_L32", 512, "fp32"): 2958538 * 1024 * 2 + 1904087 * 1024,
-                         ("box96_L32", 512, "reduced"): 2439318 * 1024 * 2 + 1012831 * 1024}
+SHIFT_Z_TRAFFIC_BYTES = {("box96_L32", 512, "fp32"): 2979462 * 1024 * 2 + 1953582 * 1024,
+                         ("box96_L32", 768, "fp32"): 4466747 * 1024 * 2 + 2928934 * 1024}
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 
 WORKLOADS = {
-    "box96_L32": dict(N=96, n_blobs=120, sigma=(1.0, 5.0), bandlimit=32, radius=45, L_max=32, tuned_batch=512,
+    "box96_L32": dict(N=96, n_blobs=120, sigma=(1.0, 5.0), bandlimit=32, radius=45, L_max=32, tuned_batch=768,
                       Li=[8, 16, 24], K=4, C=10, steps=5, batch=64, max_shift=6),
     "box64_L16": dict(N=64, n_blobs=40, sigma=(1.5, 4.0), bandlimit=16, radius=28, L_max=16,
                       Li=[8, 12], K=2, C=10, steps=5, batch=64, max_shift=4),
@@ -93,7 +95,7 @@ def parse():
                    help="streams the tuned path's batches alternate over (one plan set each)")
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP32 path")
     p.add_argument("--snr", type=float, default=0.0)
-    p.add_argument("--pool", type=int, default=512, help="distinct particles cycled through")
+    p.add_argument("--pool", type=int, default=1536, help="distinct particles cycled through")
     p.add_argument("--alternations", type=int, default=2)
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--no-cpu-baseline", action="store_true")
