@@ -24,6 +24,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <type_traits>
 #include <vector>
 
@@ -73,6 +74,7 @@ struct FlePlan {
   int splits16;               // K splits part16 holds (the largest autotune candidate)
   int ksplit16;               // K splits in use (autotuned, <= splits16)
   int cus;                    // compute units of the plan's device
+  int wide_slots;             // resident workgroups of the wide 16-bit GEMM on the device
 };
 
 struct CorrPlan {
@@ -90,7 +92,8 @@ __host__ __device__ __forceinline__ int x_slab(int cls) { return (cls & 3) == 3 
 // m = 0, 2 mod 4 (cls even) have zero sin-weighted operands in their real
 // rows (< B) and zero cos-weighted ones in their imaginary rows, so a tile
 // wholly of one kind reads one half of K; other tiles read all of it.
-__device__ __forceinline__ void k_range(int cls, int row0, int rows, int B, int Kpad, int& klo, int& khi) {
+__host__ __device__ __forceinline__ void k_range(int cls, int row0, int rows, int B, int Kpad, int& klo,
+                                                 int& khi) {
   klo = 0;
   khi = Kpad;
   if ((cls & 1) == 0) {
@@ -682,117 +685,146 @@ __global__ __launch_bounds__(64 * NW) void fle_gemm16_kernel(
 // workgroup of 8 waves, two stages of 64 KB.  X is re-read from L2 once per
 // 256 columns instead of once per 128 (the operand traffic into LDS, not the
 // MFMAs, bounds the 256 x 128 tile at large batches).  Waves: 4 row quarters
-// x 2 column halves, 64 x 128 per wave.  The grid enumerates column units
-// (pairs of tiles within a class, and the single last tile of a class with an
-// odd count), row tiles (fastest) and K splits; a single-tile unit reads its
-// tile twice and stores the first half only.
+// x 2 column halves, 64 x 128 per wave.  The work items are (K split, column
+// unit, row tile) triples whose K chunk meets the row tile's K range
+// (k_range: the real or imaginary row tiles of the classes m = 0, 2 mod 4
+// have empty chunks), numbered K split first, row tile fastest; a column unit
+// is a pair of tiles within a class, or the single last tile of a class with
+// an odd count, which reads its tile twice and stores the first half only.
+// The grid is persistent (at most one workgroup per resident slot) and
+// workgroup g takes items g, g + grid, ...: the hardware places workgroups on
+// shader engines without knowing which are empty, so a grid with empty items
+// can queue busy ones behind a full engine while other engines idle.  Empty
+// items store nothing; the finalize skips their partial slices.
 constexpr int kWideStageU4 = 4096;  // A hi 1024, A lo 1024, W 2 tiles x 1024
+constexpr int kWideMaxSplits = 16, kWideMaxRowTiles = 64;
+
+// The work items of one wide launch (host-built, passed by value).
+struct WideSched {
+  int nitems;
+  int splits;
+  int kchunk;
+  int nu[kClasses];                                  // column units per class
+  int per_split[kWideMaxSplits];                     // items of each K split
+  unsigned long long busy[2][kWideMaxSplits];        // row tiles with work, by class parity and split
+};
 
 template <bool FP16>
 __global__ __launch_bounds__(512) void fle_gemm16w_kernel(
     const uint16_t* __restrict__ Xhi, const uint16_t* __restrict__ Xlo, const u32x4* __restrict__ Wt16,
-    float* __restrict__ part, int Mpad, int Kpad, int Ntot, int kchunk, int nunits, int nmt, int B, FlePlan cols) {
+    float* __restrict__ part, int Mpad, int Kpad, int Ntot, int B, WideSched sched, FlePlan cols) {
   constexpr int NST = 2, NW = 8, AREG = 1024, W0 = 2048, AP = 4, WP = 4, RB = 4, CB = 8;
   extern __shared__ u32x4 lds[];   // NST x kWideStageU4
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int lb = mch::xcd_remap(blockIdx.x, gridDim.x);
-  const int mt = lb % nmt, rest = lb / nmt;
-  const int un = rest % nunits, ks = rest / nunits;
-  // column unit un -> class, first column and tile count
-  int cls = 0, ubase = 0;
-  for (;;) {
-    const int nt = (cols.col_off[cls + 1] - cols.col_off[cls]) / BN16;
-    const int nu = (nt + 1) >> 1;
-    if (un < ubase + nu || cls == kClasses - 1) break;
-    ubase += nu;
-    ++cls;
-  }
-  const int col0 = cols.col_off[cls] + (un - ubase) * 2 * BN16;
-  const int ntile = min(2, (cols.col_off[cls + 1] - col0) / BN16);
-  const int tn0 = col0 / BN16, tn1 = ntile == 2 ? tn0 + 1 : tn0;
-  const int row0 = mt * 2 * BM16;
-  int klo, khi;
-  k_range(cls, row0, 2 * BM16, B, Kpad, klo, khi);
-  const int kbeg = max(ks * kchunk, klo);
-  const int kend = min(ks * kchunk + kchunk, khi);
-  const int nstage = kend > kbeg ? (kend - kbeg) / BK16 : 0;
-
-  const long long xbase = ((long long)x_slab(cls) * Mpad + row0) * Kpad + (lane >> 4) * 8;
-  const uint16_t* asrc[AP];
-  int adst[AP];
-#pragma unroll
-  for (int t = 0; t < AP; ++t) {
-    const int part = wave * AP + t;            // row block part / 2, hi/lo part % 2
-    const int rb = part >> 1, hl = part & 1;
-    asrc[t] = (hl ? Xlo : Xhi) + xbase + (long long)(rb * 16 + (lane & 15)) * Kpad;
-    adst[t] = hl * AREG + rb * 64;
-  }
-  const int nK = Kpad / BK16;
-  // W^T parts P = 4 wave .. 4 wave + 3 of the unit's 32: tile P / 16, part P % 16
-  const u32x4* wsrc[WP];
-#pragma unroll
-  for (int q = 0; q < WP; ++q) {
-    const int P = wave * WP + q;
-    const int tn = (P >> 4) ? tn1 : tn0;
-    wsrc[q] = Wt16 + ((long long)tn * nK + kbeg / BK16) * 1024 + (P & 15) * 64 + lane;
-  }
-  auto issue = [&](int stage) {
-    u32x4* base = lds + (stage % NST) * kWideStageU4;
-    const int k0 = kbeg + stage * BK16;
-#pragma unroll
-    for (int t = 0; t < AP; ++t) lds_dma16(asrc[t] + k0, base + adst[t]);
-#pragma unroll
-    for (int q = 0; q < WP; ++q) lds_dma16<0>(wsrc[q] + (long long)stage * 1024, base + W0 + (wave * WP + q) * 64);
-  };
   const int rb0 = RB * (wave & 3);   // first row block of this wave
   const int chalf = wave >> 2;       // 128-column half of the unit
-  floatx4 acc[RB][CB];
-#pragma unroll
-  for (int i = 0; i < RB; ++i)
-#pragma unroll
-    for (int c = 0; c < CB; ++c) acc[i][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int nK = Kpad / BK16;
+  for (int it = mch::xcd_remap(blockIdx.x, gridDim.x); it < sched.nitems; it += gridDim.x) {
+    // item -> K split, class, column unit, row tile (the j-th busy one)
+    int r = it, ks = 0;
+    while (ks < sched.splits - 1 && r >= sched.per_split[ks]) r -= sched.per_split[ks++];
+    int cls = 0, ubase = 0, cnt = 0;
+    bool found = false;
+    for (; cls < kClasses; ++cls) {
+      cnt = __popcll(sched.busy[cls & 1][ks]);
+      if (r < sched.nu[cls] * cnt) { found = true; break; }
+      r -= sched.nu[cls] * cnt;
+      ubase += sched.nu[cls];
+    }
+    if (!found) continue;  // not reached: the host counts the same items (uniform per workgroup)
+    const int un = ubase + r / cnt;
+    int j = r - (r / cnt) * cnt;
+    unsigned long long m = sched.busy[cls & 1][ks];
+    for (; j > 0; --j) m &= m - 1;
+    const int mt = __ffsll((long long)m) - 1;
+    const int col0 = cols.col_off[cls] + (un - ubase) * 2 * BN16;
+    const int ntile = min(2, (cols.col_off[cls + 1] - col0) / BN16);
+    const int tn0 = col0 / BN16, tn1 = ntile == 2 ? tn0 + 1 : tn0;
+    const int row0 = mt * 2 * BM16;
+    int klo, khi;
+    k_range(cls, row0, 2 * BM16, B, Kpad, klo, khi);
+    const int kbeg = max(ks * sched.kchunk, klo);
+    const int kend = min(ks * sched.kchunk + sched.kchunk, khi);
+    const int nstage = kend > kbeg ? (kend - kbeg) / BK16 : 0;
 
-  if (nstage > 0) issue(0);
-  for (int st = 0; st < nstage; ++st) {
-    // this wave's DMAs of stage st have landed; after the barrier every
-    // wave's have, and every wave has finished stage st - 1, whose slot the
-    // next issue refills
-    __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
-    wg_barrier_keep_loads();
-    if (st + 1 < nstage) issue(st + 1);
-    const u32x4* base = lds + (st % NST) * kWideStageU4;
-    u32x4 w[2 * CB], ah[RB], al[RB];
+    const long long xbase = ((long long)x_slab(cls) * Mpad + row0) * Kpad + (lane >> 4) * 8;
+    const uint16_t* asrc[AP];
+    int adst[AP];
 #pragma unroll
-    for (int q = 0; q < 2 * CB; ++q) w[q] = base[W0 + chalf * 1024 + q * 64 + lane];
-#pragma unroll
-    for (int i = 0; i < RB; ++i) {
-      ah[i] = base[(rb0 + i) * 64 + lane];
-      al[i] = base[AREG + (rb0 + i) * 64 + lane];
+    for (int t = 0; t < AP; ++t) {
+      const int part = wave * AP + t;            // row block part / 2, hi/lo part % 2
+      const int rb = part >> 1, hl = part & 1;
+      asrc[t] = (hl ? Xlo : Xhi) + xbase + (long long)(rb * 16 + (lane & 15)) * Kpad;
+      adst[t] = hl * AREG + rb * 64;
     }
-    __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the products
+    // W^T parts P = 4 wave .. 4 wave + 3 of the unit's 32: tile P / 16, part P % 16
+    const u32x4* wsrc[WP];
 #pragma unroll
-    for (int i = 0; i < RB; ++i) {
+    for (int q = 0; q < WP; ++q) {
+      const int P = wave * WP + q;
+      const int tn = (P >> 4) ? tn1 : tn0;
+      wsrc[q] = Wt16 + ((long long)tn * nK + kbeg / BK16) * 1024 + (P & 15) * 64 + lane;
+    }
+    auto issue = [&](int stage) {
+      u32x4* base = lds + (stage % NST) * kWideStageU4;
+      const int k0 = kbeg + stage * BK16;
 #pragma unroll
-      for (int c = 0; c < CB; ++c) {
-        // column block c = group c / 2, block c % 2: parts 4 (c / 2) + 2 (c % 2) + {0, 1}
-        acc[i][c] = Split16<FP16>::mfma(ah[i], w[2 * c + 0], acc[i][c]);
-        acc[i][c] = Split16<FP16>::mfma(ah[i], w[2 * c + 1], acc[i][c]);
-        acc[i][c] = Split16<FP16>::mfma(al[i], w[2 * c + 0], acc[i][c]);
+      for (int t = 0; t < AP; ++t) lds_dma16(asrc[t] + k0, base + adst[t]);
+#pragma unroll
+      for (int q = 0; q < WP; ++q)
+        lds_dma16<0>(wsrc[q] + (long long)stage * 1024, base + W0 + (wave * WP + q) * 64);
+    };
+    floatx4 acc[RB][CB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+      for (int c = 0; c < CB; ++c) acc[i][c] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    if (nstage > 0) issue(0);
+    for (int st = 0; st < nstage; ++st) {
+      // this wave's DMAs of stage st have landed; after the barrier every
+      // wave's have, and every wave has finished stage st - 1, whose slot the
+      // next issue refills
+      __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+      wg_barrier_keep_loads();
+      if (st + 1 < nstage) issue(st + 1);
+      const u32x4* base = lds + (st % NST) * kWideStageU4;
+      u32x4 w[2 * CB], ah[RB], al[RB];
+#pragma unroll
+      for (int q = 0; q < 2 * CB; ++q) w[q] = base[W0 + chalf * 1024 + q * 64 + lane];
+#pragma unroll
+      for (int i = 0; i < RB; ++i) {
+        ah[i] = base[(rb0 + i) * 64 + lane];
+        al[i] = base[AREG + (rb0 + i) * 64 + lane];
+      }
+      __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the products
+#pragma unroll
+      for (int i = 0; i < RB; ++i) {
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+          // column block c = group c / 2, block c % 2: parts 4 (c / 2) + 2 (c % 2) + {0, 1}
+          acc[i][c] = Split16<FP16>::mfma(ah[i], w[2 * c + 0], acc[i][c]);
+          acc[i][c] = Split16<FP16>::mfma(ah[i], w[2 * c + 1], acc[i][c]);
+          acc[i][c] = Split16<FP16>::mfma(al[i], w[2 * c + 0], acc[i][c]);
+        }
       }
     }
+    if (chalf < ntile) {  // a single-tile unit: the second half is a copy
+      float* P = part + (long long)ks * Mpad * Ntot;
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int c = 0; c < CB; ++c)
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) {
+            const int row = row0 + (rb0 + i) * 16 + 4 * (lane >> 4) + rr;
+            const int col = col0 + chalf * BN16 + c * 16 + (lane & 15);
+            P[(long long)row * Ntot + col] = acc[i][c][rr];
+          }
+    }
+    // every wave has read its last stage before the next item refills the ring
+    __syncthreads();
   }
-  if (chalf >= ntile) return;  // a single-tile unit: the second half is a copy
-  float* P = part + (long long)ks * Mpad * Ntot;
-#pragma unroll
-  for (int i = 0; i < RB; ++i)
-#pragma unroll
-    for (int c = 0; c < CB; ++c)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = row0 + (rb0 + i) * 16 + 4 * (lane >> 4) + r;
-        const int col = col0 + chalf * BN16 + c * 16 + (lane & 15);
-        P[(long long)row * Ntot + col] = acc[i][c][r];
-      }
 }
 
 // FP16 mode: X (FP32) -> Xhi/Xlo of X * rowscale (rows of every class)
@@ -884,12 +916,16 @@ __global__ __launch_bounds__(256) void fle_wsplit_kernel(const float* __restrict
 // 3. split-K reduction and scatter to the reference coefficient order
 // ---------------------------------------------------------------------------
 // colscale / rowmax (NULL: 1) undo the power-of-2 scales of the FP16 split GEMM.
+// skip_kchunk > 0 (the wide GEMM, which stores no partial for an empty item):
+// split s of a row counts only if its K chunk [s, s + 1) * skip_kchunk meets
+// the K range of the row's 256-row tile.
 __global__ void fle_finalize_kernel(const float* __restrict__ part, int splits, int Mpad, int Ntot,
                                     int B, const int* __restrict__ chan_plus,
                                     const int* __restrict__ chan_minus,
                                     const int* __restrict__ chan_m, float2* __restrict__ a,
                                     long long lda, const float* __restrict__ colscale,
-                                    const float* __restrict__ rowmax, int4 flip) {
+                                    const float* __restrict__ rowmax, int4 flip, int skip_kchunk,
+                                    FlePlan cols) {
   const long long tot = (long long)B * Ntot;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < tot;
        t += (long long)gridDim.x * blockDim.x) {
@@ -898,10 +934,24 @@ __global__ void fle_finalize_kernel(const float* __restrict__ part, int splits, 
     const int ip = chan_plus[c];
     if (ip < 0) continue;
     float re = 0.f, im = 0.f;
-    for (int s = 0; s < splits; ++s) {
-      const float* P = part + (long long)s * Mpad * Ntot;
-      re += P[(long long)b * Ntot + c];
-      im += P[(long long)(B + b) * Ntot + c];
+    if (skip_kchunk > 0) {
+      int cls = 0;
+      while (cls < kClasses - 1 && c >= cols.col_off[cls + 1]) ++cls;
+      int klo_r, khi_r, klo_i, khi_i;
+      k_range(cls, (b / 256) * 256, 256, B, cols.Kpad, klo_r, khi_r);
+      k_range(cls, ((B + b) / 256) * 256, 256, B, cols.Kpad, klo_i, khi_i);
+      for (int s = 0; s < splits; ++s) {
+        const float* P = part + (long long)s * Mpad * Ntot;
+        const int k0 = s * skip_kchunk, k1 = k0 + skip_kchunk;
+        if (max(k0, klo_r) < min(k1, khi_r)) re += P[(long long)b * Ntot + c];
+        if (max(k0, klo_i) < min(k1, khi_i)) im += P[(long long)(B + b) * Ntot + c];
+      }
+    } else {
+      for (int s = 0; s < splits; ++s) {
+        const float* P = part + (long long)s * Mpad * Ntot;
+        re += P[(long long)b * Ntot + c];
+        im += P[(long long)(B + b) * Ntot + c];
+      }
     }
     if (colscale) {
       re /= colscale[c] * pow2_scale(rowmax[b]);
@@ -1029,13 +1079,17 @@ void free_fle_plan(FlePlan* p) {
 // Autotune candidates of the 16-bit GEMM: (pipeline depth, waves per
 // workgroup, rounds of resident workgroups the K split fills).  8 waves hold a
 // 256-row tile (candidates only when the plan's rows come in 256s).
+// rounds: the K split fills 1, 2 or 3 full rounds of resident workgroups, or
+// (0, wide tile only) the split of the fewest k-stages per workgroup under the
+// wide kernel's busy-item schedule (gemm16_splits)
 struct Gemm16Cand { int nst, nw, rounds, ct; };
 constexpr Gemm16Cand kGemm16Cands[] = {
     {2, 4, 1, 1}, {3, 4, 1, 1}, {2, 2, 1, 1}, {3, 2, 1, 1}, {2, 4, 2, 1}, {3, 4, 2, 1}, {2, 2, 2, 1},
     {3, 2, 2, 1}, {2, 4, 3, 1}, {3, 4, 3, 1}, {2, 2, 3, 1}, {3, 2, 3, 1}, {2, 8, 1, 1}, {3, 8, 1, 1},
-    {2, 8, 2, 1}, {3, 8, 2, 1}, {2, 8, 1, 2}, {2, 8, 2, 2}, {2, 8, 3, 2}};
+    {2, 8, 2, 1}, {3, 8, 2, 1}, {2, 8, 1, 2}, {2, 8, 2, 2}, {2, 8, 3, 2}, {2, 8, 0, 2}};
 constexpr int kGemm16Candidates = (int)(sizeof(kGemm16Cands) / sizeof(kGemm16Cands[0]));
 inline bool gemm16_usable(const FlePlan* p, int cand) {
+  if (kGemm16Cands[cand].ct == 2 && p->Mpad / (2 * BM16) > kWideMaxRowTiles) return false;
   return kGemm16Cands[cand].nw != 8 || p->Mpad % (2 * BM16) == 0;
 }
 
@@ -1069,6 +1123,29 @@ int gemm16_occupancy(int nst, int nw, int ct, int* occ) {
   return rc;
 }
 
+// The work items of a wide launch at batch B (see fle_gemm16w_kernel): for
+// every K split and class parity, the 256-row tiles whose K range meets the
+// split's chunk.  False if the launch does not fit the schedule's limits.
+bool wide_schedule(const FlePlan* p, int B, int splits, int kchunk, WideSched& ws) {
+  const int nmt = (2 * B + 2 * BM16 - 1) / (2 * BM16);
+  if (splits > kWideMaxSplits || nmt > kWideMaxRowTiles) return false;
+  memset(&ws, 0, sizeof(ws));
+  ws.splits = splits;
+  ws.kchunk = kchunk;
+  for (int c = 0; c < kClasses; ++c) ws.nu[c] = ((p->col_off[c + 1] - p->col_off[c]) / BN16 + 1) / 2;
+  for (int ks = 0; ks < splits; ++ks) {
+    for (int par = 0; par < 2; ++par)
+      for (int mt = 0; mt < nmt; ++mt) {
+        int klo, khi;
+        k_range(par, mt * 2 * BM16, 2 * BM16, B, p->Kpad, klo, khi);
+        if (std::max(ks * kchunk, klo) < std::min(ks * kchunk + kchunk, khi)) ws.busy[par][ks] |= 1ull << mt;
+      }
+    for (int c = 0; c < kClasses; ++c) ws.per_split[ks] += ws.nu[c] * __builtin_popcountll(ws.busy[c & 1][ks]);
+    ws.nitems += ws.per_split[ks];
+  }
+  return true;
+}
+
 // K split that fills `rounds` rounds of `slots` resident workgroups with
 // the (column tile x row tile) grid, rounded so that every split but the
 // last has the same number of K blocks.
@@ -1076,13 +1153,29 @@ int gemm16_splits(const FlePlan* p, int nw, int ct, int slots, int rounds) {
   const int ncol = ct == 2 ? gemm16_units(p) : p->Ntot / BN16;
   const int ntiles = ncol * (p->Mpad / (BM16 * gemm16_row_tiles(nw)));
   const int kblocks = p->Kpad / BK16;
+  const int smax = std::min(kblocks, ct == 2 ? kWideMaxSplits : kblocks);
+  if (rounds == 0 && ct == 2) {
+    // fewest k-stages per workgroup: rounds of busy items x chunk length (plus
+    // a few stages for each item's pipeline fill and partial-tile store)
+    int best = 1;
+    long long tbest = -1;
+    for (int s = 1; s <= smax; ++s) {
+      const int per = (kblocks + s - 1) / s, se = (kblocks + per - 1) / per;
+      WideSched ws;
+      if (!wide_schedule(p, p->max_batch, se, per * BK16, ws)) continue;
+      const long long t = (long long)((ws.nitems + slots - 1) / std::max(slots, 1)) * (per + 4);
+      if (tbest < 0 || t < tbest) { tbest = t; best = se; }
+    }
+    return best;
+  }
   int s = rounds * slots / ntiles;
-  s = std::min(std::max(s, 1), kblocks);
+  s = std::min(std::max(s, 1), smax);
   const int per = (kblocks + s - 1) / s;
   return (kblocks + per - 1) / per;
 }
 
-int launch_gemm(FlePlan* p, int B, hipStream_t stream, int* splits_out) {
+int launch_gemm(FlePlan* p, int B, hipStream_t stream, int* splits_out, int* skip_kchunk = nullptr) {
+  if (skip_kchunk) *skip_kchunk = 0;
   const int mt = (2 * B + BM - 1) / BM;
   if (p->precision == kFp32) {
     const int kblocks = p->Kpad / BK;
@@ -1101,16 +1194,21 @@ int launch_gemm(FlePlan* p, int B, hipStream_t stream, int* splits_out) {
   const int mt16 = (2 * B + BM16 * rt - 1) / (BM16 * rt);
   const bool fp16 = p->precision == kFp16x3;
   if (p->ct16 == 2) {
-    const int nunits = gemm16_units(p);
-    const dim3 gridw(nunits * mt16 * splits);
+    WideSched ws;
+    if (!wide_schedule(p, B, splits, kchunk, ws)) {
+      mch::set_error("wide GEMM tile: %d K splits or %d row tiles beyond its schedule", splits, mt16);
+      return MCH_ERR_UNSUPPORTED;
+    }
+    const dim3 gridw(std::max(1, std::min(ws.nitems, p->wide_slots)));
     const size_t ldsw = gemm16_lds(2, 8, 2);
     if (fp16)
       hipLaunchKernelGGL(fle_gemm16w_kernel<true>, gridw, dim3(512), ldsw, stream, p->Xhi, p->Xlo,
-                         (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, kchunk, nunits, mt16, B, *p);
+                         (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, B, ws, *p);
     else
       hipLaunchKernelGGL(fle_gemm16w_kernel<false>, gridw, dim3(512), ldsw, stream, p->Xhi, p->Xlo,
-                         (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, kchunk, nunits, mt16, B, *p);
+                         (const u32x4*)p->Wt16, p->part16, p->Mpad, p->Kpad, p->Ntot, B, ws, *p);
     *splits_out = splits;
+    if (skip_kchunk) *skip_kchunk = kchunk;
     return mch::hip_check(hipGetLastError(), "fle_gemm16w_kernel launch");
   }
   const dim3 grid(ntn * mt16 * splits);
@@ -1299,6 +1397,11 @@ int mch_fle_set_precision(void* plan, int precision) {
     MCH_TRY(hipGetDevice(&dev), "hipGetDevice");
     MCH_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "CU count");
     p->cus = cus > 0 ? cus : 256;
+    {
+      int occw = 0;
+      if ((rc = gemm16_occupancy(2, 8, 2, &occw))) return rc;
+      p->wide_slots = occw * p->cus;
+    }
     int smax = 1;
     for (int cand = 0; cand < kGemm16Candidates; ++cand) {
       if (!gemm16_usable(p, cand)) continue;
@@ -1400,7 +1503,8 @@ int mch_fle_set_gemm_tiles(void* plan, int stages, int waves, int column_tiles, 
   for (int cand = 0; cand < kGemm16Candidates && !found; ++cand)
     found = gemm16_usable(p, cand) && kGemm16Cands[cand].nst == stages && kGemm16Cands[cand].nw == waves &&
             kGemm16Cands[cand].ct == column_tiles;
-  if (!found || k_splits < 1 || k_splits > p->splits16 || k_splits > p->Kpad / BK16) {
+  if (!found || k_splits < 1 || k_splits > p->splits16 || k_splits > p->Kpad / BK16 ||
+      (column_tiles == 2 && k_splits > kWideMaxSplits)) {
     mch::set_error("mch_fle_set_gemm_tiles: (%d stages, %d waves, %d column tiles, %d splits) is not a usable "
                    "configuration (splits <= %d)", stages, waves, column_tiles, k_splits, p->splits16);
     return MCH_ERR_ARG;
@@ -1470,8 +1574,8 @@ int mch_fle_project_shifted(void* plan, const float* vols, long long ld_vol, int
                        p->Kpad, 2 * B, p->Xhi, p->Xlo);
     if ((rc = mch::hip_check(hipGetLastError(), "fle_xsplit_kernel launch"))) return rc;
   }
-  int splits = 0;
-  if ((rc = launch_gemm(p, B, stream, &splits))) return rc;
+  int splits = 0, skip_kchunk = 0;
+  if ((rc = launch_gemm(p, B, stream, &splits, &skip_kchunk))) return rc;
   const long long ftot = (long long)B * p->Ntot;
   int fb = (int)((ftot + 255) / 256);
   if (fb > 8192) fb = 8192;
@@ -1479,7 +1583,7 @@ int mch_fle_project_shifted(void* plan, const float* vols, long long ld_vol, int
                      p->precision == kFp32 ? p->part : p->part16, splits, p->Mpad, p->Ntot, B,
                      p->chan_plus, p->chan_minus, p->chan_m, (float2*)coeffs, lda,
                      fp16 ? p->colscale : nullptr, fp16 ? p->rowmax : nullptr,
-                     make_int4(p->col_off[3], p->col_off[4], p->col_off[7], p->col_off[8]));
+                     make_int4(p->col_off[3], p->col_off[4], p->col_off[7], p->col_off[8]), skip_kchunk, *p);
   return mch::hip_check(hipGetLastError(), "fle_finalize_kernel launch");
 }
 
