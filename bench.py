@@ -1,6 +1,6 @@
 """Benchmark: aligned volumes/s for Matcha's orientation-search hot path on one MI355X.
 
-Headline (the metric's own definition): the tuned path -- batched (768
+Headline (the metric's own definition): the tuned path -- batched (1024
 particles per batch, consecutive batches alternating over two streams so that
 independent batches overlap on the device) with the projection GEMM on 16-bit
 hi/lo split operands (three MFMA products, FP32 accumulation) and every other
@@ -24,7 +24,7 @@ U(1,5) px, amplitude U(0.5,1), centres in a ball of 0.35*96), particles =
 Haar-random rotation + integer shift in [-6,6]^3 + white noise at 0 dB over
 the radius-45 mask; FLE bandlimit 32 (ne 8252, lmax 42), radius 45; march
 Li = 8, 16, 24 -> L_max 32; SOFFT grid 64^3 (L0 = 8, K = 4); 10 candidates;
-5 Newton steps; plain path batch 64, tuned path batch 768 (1536 particles
+5 Newton steps; plain path batch 64, tuned path batch 1024 (2048 particles
 cycled).
 
 Three kernels take 17-24% of the step each: the tuned projection GEMM
@@ -52,26 +52,26 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 
-# HBM bytes the tuned projection GEMM fetches per launch, measured with
-# rocprofv3 --pmc FETCH_SIZE and scaled by 2.0 (FETCH_SIZE of a device copy of
-# 512 MiB reads 256 MiB); keyed by (workload, batch, waves per workgroup and
-# 128-column tiles per workgroup of the tile the autotuner chose: 8 waves is a
-# 256-row tile).  Its writes are the K-split
-# partials, splits x rows x Ntot x 4 B (WRITE_SIZE reads exactly that), added
-# per run for the split the autotuner chose.
-GEMM16_FETCH_BYTES = {("box96_L32", 64, 4, 1): 587089 * 1024 * 2, ("box96_L32", 512, 8, 2): 1070211 * 1024 * 2,
-                      ("box96_L32", 768, 8, 2): 1501459 * 1024 * 2}
+# HBM bytes the tuned projection GEMM moves per launch, measured with
+# rocprofv3 --pmc FETCH_SIZE (scaled by 2.0: FETCH_SIZE of a device copy of
+# 512 MiB reads 256 MiB) and WRITE_SIZE (the K-split partials of its busy work
+# items); keyed by (workload, batch, waves per workgroup, 128-column tiles per
+# workgroup, K splits) of the configuration the autotuner chose (8 waves is a
+# 256-row tile).
+GEMM16_TRAFFIC_BYTES = {("box96_L32", 64, 4, 1, 20): 587089 * 1024 * 2 + 47360 * 1024,
+                        ("box96_L32", 1024, 8, 2, 2): 1812678 * 1024 * 2 + 67235 * 1024}
 # HBM bytes (FETCH_SIZE x 2.0 + WRITE_SIZE) per launch of the shift search's
 # column pass, keyed by (workload, batch, reference table precision); measured
 # like the GEMM's.
 SHIFT_Z_TRAFFIC_BYTES = {("box96_L32", 512, "fp32"): 2979462 * 1024 * 2 + 1953582 * 1024,
-                         ("box96_L32", 768, "fp32"): 4466747 * 1024 * 2 + 2928934 * 1024}
+                         ("box96_L32", 768, "fp32"): 4466747 * 1024 * 2 + 2928934 * 1024,
+                         ("box96_L32", 1024, "fp32"): 5957119 * 1024 * 2 + 3903117 * 1024}
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 
 WORKLOADS = {
-    "box96_L32": dict(N=96, n_blobs=120, sigma=(1.0, 5.0), bandlimit=32, radius=45, L_max=32, tuned_batch=768,
+    "box96_L32": dict(N=96, n_blobs=120, sigma=(1.0, 5.0), bandlimit=32, radius=45, L_max=32, tuned_batch=1024,
                       Li=[8, 16, 24], K=4, C=10, steps=5, batch=64, max_shift=6),
     "box64_L16": dict(N=64, n_blobs=40, sigma=(1.5, 4.0), bandlimit=16, radius=28, L_max=16,
                       Li=[8, 12], K=2, C=10, steps=5, batch=64, max_shift=4),
@@ -95,7 +95,7 @@ def parse():
                    help="streams the tuned path's batches alternate over (one plan set each)")
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP32 path")
     p.add_argument("--snr", type=float, default=0.0)
-    p.add_argument("--pool", type=int, default=1536, help="distinct particles cycled through")
+    p.add_argument("--pool", type=int, default=2048, help="distinct particles cycled through")
     p.add_argument("--alternations", type=int, default=2)
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--no-cpu-baseline", action="store_true")
@@ -603,10 +603,8 @@ def main():
     depth = int(_lib_depth(fle_))
     tiles = {"k_splits": depth % 10000 // 100, "waves_per_workgroup": depth % 100 // 10,
              "lds_stages": depth % 10, "column_tiles_per_workgroup": depth // 10000 + 1}
-    fetch = GEMM16_FETCH_BYTES.get((args.workload, Bt, tiles["waves_per_workgroup"],
-                                    tiles["column_tiles_per_workgroup"]))
-    rows = -(-2 * Bt // 128) * 128
-    traffic = fetch + tiles["k_splits"] * rows * fle_.Ntot * 4.0 if fetch is not None else None
+    traffic = GEMM16_TRAFFIC_BYTES.get((args.workload, Bt, tiles["waves_per_workgroup"],
+                                        tiles["column_tiles_per_workgroup"], tiles["k_splits"]))
     gemm_hbm = achieved / 8000.0
     gemm_mfma = flops_alg / gemm_t / 1e12 / 2500.0
     gemm_issue = flops16 / gemm_t / 1e12 / 2500.0
