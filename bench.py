@@ -450,9 +450,10 @@ def precision_workload(torch, dev, args):
 def batching_workload(torch, dev, args):
     """Uneven batches: particles of boxes 64/96/128 (2:2:1, one blob template per
     box), missing wedges of +-U(45, 65) deg about a randomly offset y axis,
-    SNR U(-12, 0) dB, march ends L in {16, 24, 32}; bucketed per (box, L),
-    eager launches per batch (bf16x3 projection, FLE bandlimit 32); a HIP graph
-    replay of one full batch is timed beside its eager launch."""
+    SNR U(-12, 0) dB, march ends L in {16, 24, 32}; grouped per box, each
+    batch mixing the march ends in one search launch, eager launches per batch
+    (bf16x3 projection, FLE bandlimit 32); a HIP graph replay of one full batch
+    is timed beside its eager launch."""
     from matcha_amd.batching import UnevenAligner
     from matcha_amd.synthetic import UnevenWorkload, euler_zyz_to_matrix, geodesic_deg
     n = args.batching_particles
@@ -474,13 +475,13 @@ def batching_workload(torch, dev, args):
                        L_ends=wl.L_ends, precision=args.tuned_precision, graphs=False, max_shift=wl.max_shift)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
-    # warm every bucket: plan set-up, autotune and graph capture stay out of the timed pass
-    from matcha_amd.batching import plan_buckets
-    for (N, L), chunks in plan_buckets(specs, al.batch).items():
+    # warm every group: plan set-up and autotune stay out of the timed pass
+    from matcha_amd.batching import plan_groups
+    for (N, L0), chunks in plan_groups(specs, al.batch, al.cfg["Li"]).items():
         full = [c for c in chunks if len(c) == al.batch]
         if full:
-            al.bucket(N, L).run(torch.stack([particles[i] for i in full[0]]),
-                                torch.stack([ctfs[i] for i in full[0]]))
+            al.group(N, L0).run([particles[i] for i in full[0]], [ctfs[i] for i in full[0]],
+                                [specs[i][1] for i in full[0]])
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     res = al.align(specs, particles, ctfs)
@@ -495,30 +496,34 @@ def batching_workload(torch, dev, args):
         m = np.array([sp[0] == N for sp in specs])
         per_box[str(N)] = {"particles": int(m.sum()), "median_err_deg": float(np.median(err[m])),
                            "shift_exact_frac": float(ok[m].mean())}
-    # graph replay against eager launches of the same full batch (96^3, L = 32)
-    bk = al.bucket(96, 32)
-    sel = [i for i, sp in enumerate(specs) if sp == (96, 32)][:al.batch]
-    vb, cb = torch.stack([particles[i] for i in sel]), torch.stack([ctfs[i] for i in sel])
+    # graph replay against eager launches of the same full batch (box 96, mixed ends)
+    L0 = min(al.cfg["Li"])
+    bk = al.group(96, L0)
+    sel = [i for i, sp in enumerate(specs) if sp[0] == 96][:al.batch]
+    # both modes take the particles as a list, as align() passes them: eager
+    # launches stack them, the graph stacks them into its static buffer
+    vb, cb = [particles[i] for i in sel], [ctfs[i] for i in sel]
+    eb = [specs[i][1] for i in sel]
     reps = 10
     times = {}
     for mode in ("graph", "eager"):
         bk.graphs = mode == "graph"
-        bk.run(vb, cb)
+        bk.run(vb, cb, eb)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         for _ in range(reps):
-            bk.run(vb, cb)
+            bk.run(vb, cb, eb)
         torch.cuda.synchronize()
         times[mode] = (time.perf_counter() - t2) / reps * 1e3
     bk.graphs = False
     return {
         "workload": (f"{n} particles, boxes 64/96/128 at 2:2:1 (one template each), missing wedge "
                      f"+-U(45,65) deg, SNR U(-12,0) dB, march end L in {{16,24,32}}, 2 alternations, "
-                     f"batch {al.batch}, {args.tuned_precision}, eager launches; box 64: 40 blobs sigma "
+                     f"batch {al.batch} per box with mixed march ends, {args.tuned_precision}, eager launches; box 64: 40 blobs sigma "
                      f"U(1.5,4), box 96: 120 and box 128: 200 blobs sigma U(1,5)"),
-        "particles_per_s": n / el, "seconds": el, "buckets": len(al.buckets),
+        "particles_per_s": n / el, "seconds": el, "groups": len(al.groups),
         "median_err_deg": float(np.median(err)), "shift_exact_frac": float(ok.mean()), "per_box": per_box,
-        "graph_vs_eager_ms_per_batch_96_L32": times, "setup_s": t_data,
+        "graph_vs_eager_ms_per_batch_96_mixed_ends": times, "setup_s": t_data,
     }
 
 

@@ -99,6 +99,21 @@ int mch_search_orientations(void* plan, const void* sigma, long long ld_sigma, i
                             float base_lr, float* out_alpha, float* out_beta, float* out_gamma,
                             float* out_score, hipStream_t stream);
 
+/* mch_search_orientations with a march end per particle: march_ends is a
+ * device array of B ints; particle b marches the levels of `levels` (strictly
+ * increasing) below march_ends[b], then march_ends[b] (<= the plan's L_max) --
+ * the schedule of a Matcha built with L_max = march_ends[b]
+ * (src/matcha/core/Matcha.py:76,381-386), so one launch of the grid, the
+ * march and the reduction serves a batch of mixed bandlimit schedules that
+ * the reference runs as one Matcha per schedule
+ * (src/matcha/align_subtomograms.py:61-134).  A march end outside
+ * [1, L_max] gives that particle NaN outputs.  Not combined with stop_early. */
+int mch_search_orientations_ends(void* plan, const void* sigma, long long ld_sigma, int B, int C,
+                                 const int* levels, int n_levels, const int* march_ends,
+                                 int num_steps, int newton, float base_lr, float* out_alpha,
+                                 float* out_beta, float* out_gamma, float* out_score,
+                                 hipStream_t stream);
+
 /* ------------------------------------------------------- projection --- */
 
 /* Plan of the symmetry-adapted ball-harmonic projection: orbit table of the

@@ -37,6 +37,9 @@ _SIGNATURES = [
     ("mch_search_orientations", c_int, [c_void_p, c_void_p, c_ll, c_int, c_int, c_int_p, c_int, c_int,
                                         c_int, c_float, c_void_p, c_void_p, c_void_p, c_void_p,
                                         c_void_p]),
+    ("mch_search_orientations_ends", c_int, [c_void_p, c_void_p, c_ll, c_int, c_int, c_int_p, c_int,
+                                             c_void_p, c_int, c_int, c_float, c_void_p, c_void_p,
+                                             c_void_p, c_void_p, c_void_p]),
     ("mch_so3_evaluate", c_int, [c_void_p, c_void_p, c_ll, c_int, c_int, c_int, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),
     ("mch_fle_plan_create", c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,

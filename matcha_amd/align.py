@@ -103,9 +103,11 @@ class Alternation:
         self._shift0 = None
         self.history = []
         self.tracker = None
+        self._ends = None
 
     @torch.no_grad()
-    def run(self, subtomos: torch.Tensor, prior_tracker=None, ctfs: torch.Tensor | None = None):
+    def run(self, subtomos: torch.Tensor, prior_tracker=None, ctfs: torch.Tensor | None = None,
+            march_ends=None):
         """(alpha, beta, gamma, norm_score, shift (B,3), shift_score) of the last
         alternation.
 
@@ -115,12 +117,20 @@ class Alternation:
         by the previous tracker) and ``self.tracker.q`` holds the composed
         rotation trackers, the reference's ``rotation_tracker``; in global mode
         the angles are the full estimate.  ``ctfs`` (B, N, N, N//2+1): CTF or
-        missing-wedge weights of the shift search (``do_ctf_correction``)."""
+        missing-wedge weights of the shift search (``do_ctf_correction``).
+        ``march_ends``: per-particle march ends of the orientation searches
+        (``Matcha.search_orientations``)."""
+        self._ends = march_ends
         if self.mode == "reference":
             return self._run_reference(subtomos, prior_tracker, ctfs)
         if prior_tracker is not None:
             raise ValueError("a prior rotation needs mode='reference'")
         return self._run_global(subtomos, ctfs)
+
+    def _search(self, sigma):
+        if self._ends is None:
+            return self.rot.search_orientations(sigma)
+        return self.rot.search_orientations(sigma, march_ends=self._ends)
 
     def _buffers(self, subtomos, patches: bool):
         """Running-shift buffer, and (reference mode) the rotated-patch buffers."""
@@ -142,7 +152,7 @@ class Alternation:
             # particles (the values extract_shifted_patches would copy out)
             sh = shift if it > 0 else None
             sigma, norms = self.ccm.get_sigma(subtomos, max_degree=self.rot.Lmax, shift_zyx=sh)
-            a, b, g, s = self.rot.search_orientations(sigma[:B])
+            a, b, g, s = self._search(sigma[:B])
             nscore = s / (norms[:B] * self.ccm.bh_norm_template).clamp_min(1e-12)
             last = it == self.num_alternations - 1
             sscore = None
@@ -168,7 +178,7 @@ class Alternation:
             vols = extract_rotated_batch(subtomos, shift, self.tracker.rot_particle, radius,
                                          out=self._patches, work=self._work)
             sigma, norms = self.ccm.get_sigma(vols, max_degree=self.rot.Lmax)
-            a, b, g, s = self.rot.search_orientations(sigma[:B])
+            a, b, g, s = self._search(sigma[:B])
             nscore = s / (norms[:B] * self.ccm.bh_norm_template).clamp_min(1e-12)
             self.tracker.update(a, b, g)
             last = it == self.num_alternations - 1

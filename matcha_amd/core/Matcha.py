@@ -87,6 +87,8 @@ class Matcha:
                 raise ValueError("stop_early supports at most 63 num_steps")
             _lib.check(_lib.lib().mch_so3_set_stop_early(self.plan.handle, 1), "mch_so3_set_stop_early")
         self._levels_c = (ctypes.c_int * len(self.levels))(*self.levels)
+        self._li_sorted = sorted(int(x) for x in cfg_get(cfg, "Li"))
+        self._li_c = (ctypes.c_int * len(self._li_sorted))(*self._li_sorted)
         self._truncate_dsize_cache = {int(l): Dsize(int(l)) for l in set(self.Li)}
 
     def _assert_config(self, matcha_config):
@@ -125,11 +127,49 @@ class Matcha:
         _lib.check(rc, "mch_newton_march")
         return a, b, g, s
 
+    def march_ends_tensor(self, march_ends) -> torch.Tensor:
+        """Per-particle march ends as the (B,) int32 device tensor that
+        ``search_orientations(sigma, march_ends=...)`` reads, checked on the host:
+        each end in [min(Li), L_max], so that every particle shares this
+        matcher's SOFFT grid (L0 = min(Li)) and marches the schedule of a
+        ``Matcha`` built with ``L_max`` = its end."""
+        ends = torch.as_tensor(march_ends).reshape(-1).to(torch.int64)
+        if ends.is_cuda:
+            raise ValueError("march_ends_tensor checks host values; pass a list, array or CPU tensor")
+        lo = self._li_sorted[0] if self._li_sorted else 0
+        if ends.numel() and (int(ends.min()) < max(lo, 1) or int(ends.max()) > self.Lmax):
+            raise ValueError(f"march ends must lie in [{max(lo, 1)}, {self.Lmax}], got "
+                             f"[{int(ends.min())}, {int(ends.max())}]")
+        return ends.to(torch.int32).to(self.device)
+
     @torch.no_grad()
-    def search_orientations(self, sigma: torch.Tensor):
+    def search_orientations(self, sigma: torch.Tensor, march_ends=None):
+        """``Matcha.search_orientations`` (``Matcha.py:369-403``).
+
+        ``march_ends`` (optional, beyond the reference): per-particle march
+        ends -- a (B,) int32 device tensor from ``march_ends_tensor`` (read on
+        the device, so a graph can capture the call), or host values checked
+        here.  Particle b then gets the result a ``Matcha`` of ``L_max`` =
+        its end would give, in one launch for the whole batch."""
         sig = _check_sigma(sigma, self.device, Dsize(self.Lmax), self.batchsize)
         B = sig.shape[0]
         outs = [torch.empty(B, dtype=torch.float32, device=self.device) for _ in range(4)]
+        if march_ends is not None:
+            if self.stop_early:
+                raise ValueError("per-particle march ends cannot be combined with stop_early")
+            ends = march_ends
+            if not (isinstance(ends, torch.Tensor) and ends.is_cuda):
+                ends = self.march_ends_tensor(ends)
+            if ends.dtype != torch.int32 or ends.device != self.device or ends.numel() < B:
+                raise ValueError("march_ends must be a device int32 tensor with one end per particle")
+            ends = ends.contiguous()
+            rc = _lib.lib().mch_search_orientations_ends(
+                self.plan.handle, sig.data_ptr(), sig.stride(0), B, self.candidates, self._li_c,
+                len(self._li_sorted), ends.data_ptr(), self.num_steps,
+                1 if self.step_type == "newton" else 0, float(self.base_lr),
+                *[o.data_ptr() for o in outs], _lib.stream_handle(self.device))
+            _lib.check(rc, "mch_search_orientations_ends")
+            return tuple(outs)
         rc = _lib.lib().mch_search_orientations(
             self.plan.handle, sig.data_ptr(), sig.stride(0), B, self.candidates, self._levels_c,
             len(self.levels), self.num_steps, 1 if self.step_type == "newton" else 0,

@@ -488,10 +488,17 @@ struct MarchArgs {
   int li0, st0;
   double* state;
   int* accept;
+  // per-particle march ends (B ints on the device, NULL: every particle
+  // marches `levels`): particle b marches the levels below lend[b], then
+  // lend[b] itself -- the schedule that sorted(Li) + [L_end] gives a Matcha of
+  // L_max = lend[b] (Matcha.py:76,381-386); a value outside [1, Lmax] makes the
+  // particle's candidates NaN
+  const int* lend;
 };
 
 constexpr int kMarchThreads = 256;
 constexpr int kMarchWaves = kMarchThreads / 64;
+constexpr int kMaxLevelsMarch = 16;
 constexpr int kMaxL = 96;
 
 struct EvalOut {
@@ -815,9 +822,36 @@ __global__ __launch_bounds__(kMarchThreads, 5) void so3_march_kernel(MarchArgs a
     }
   }
   float ev[10];
-  const int li_begin = STEPPED ? args.li0 : 0, li_end = STEPPED ? args.li0 + 1 : args.n_levels;
+  // the workgroup's schedule: the shared one, or the levels below the
+  // particle's own march end followed by that end
+  __shared__ int slev[kMaxLevelsMarch + 1];
+  __shared__ int snlev;
+  if (tid == 0) {
+    int n = 0;
+    if (!STEPPED && args.lend) {
+      const int le = args.lend[b];
+      if (le >= 1 && le <= args.Lmax) {
+        for (int i = 0; i < args.n_levels; ++i)
+          if (args.levels[i] < le) slev[n++] = args.levels[i];
+        slev[n++] = le;
+      }
+    } else {
+      for (int i = 0; i < args.n_levels; ++i) slev[n++] = args.levels[i];
+    }
+    snlev = n;
+  }
+  __syncthreads();
+  const int nlev = snlev;
+  if (nlev == 0) {  // invalid march end: no level to march
+    if (tid == 0) {
+      const float nan = __builtin_nanf("");
+      args.alpha[q] = nan; args.beta[q] = nan; args.gamma[q] = nan; args.score[q] = nan;
+    }
+    return;
+  }
+  const int li_begin = STEPPED ? args.li0 : 0, li_end = STEPPED ? args.li0 + 1 : nlev;
   for (int li = li_begin; li < li_end; ++li) {
-    const int Li = args.levels[li];
+    const int Li = slev[li];
     const int st_begin = STEPPED ? args.st0 : 0, st_end = STEPPED ? args.st0 : args.num_steps;
     for (int st = st_begin; st <= st_end; ++st) {
       // thread 0 proposes the point to evaluate; everybody reads it from LDS
@@ -1083,14 +1117,26 @@ int mch_sofft_eval(void* plan, const void* sigma, long long ld_sigma, int B, flo
   return launch_sofft_slice(p, sigma, ld_sigma, B, 0, grid, stream);
 }
 
-// Newton march over the level schedule for (B, C) candidates (in/out).
-int mch_newton_march(void* plan, const void* sigma, long long ld_sigma, int B, int C,
-                     const int* levels, int n_levels, int num_steps, int newton, float base_lr,
-                     float* alpha, float* beta, float* gamma, float* score, hipStream_t stream) {
-  So3Plan* p = (So3Plan*)plan;
+// Newton march over the level schedule for (B, C) candidates (in/out);
+// lend (B device ints or NULL): per-particle march ends.
+static int newton_march(So3Plan* p, const void* sigma, long long ld_sigma, int B, int C,
+                        const int* levels, int n_levels, const int* lend, int num_steps, int newton,
+                        float base_lr, float* alpha, float* beta, float* gamma, float* score,
+                        hipStream_t stream) {
   if (!p || !sigma || !levels || !alpha || !beta || !gamma || !score) {
     mch::set_error("mch_newton_march: NULL argument");
     return MCH_ERR_ARG;
+  }
+  if (lend && p->stop_early) {
+    mch::set_error("mch_newton_march: per-particle march ends cannot be combined with stop_early "
+                   "(its exit is batch-global per level)");
+    return MCH_ERR_UNSUPPORTED;
+  }
+  for (int i = 1; lend && i < n_levels; ++i) {
+    if (levels[i] <= levels[i - 1]) {
+      mch::set_error("mch_newton_march: levels must increase with per-particle march ends");
+      return MCH_ERR_ARG;
+    }
   }
   if (B < 1 || B > p->max_batch || C < 1 || n_levels < 1 || n_levels > 16 || num_steps < 0 ||
       ld_sigma < mch::dsize(p->Lmax)) {
@@ -1124,6 +1170,7 @@ int mch_newton_march(void* plan, const void* sigma, long long ld_sigma, int B, i
   a.newton = newton;
   a.base_lr = base_lr;
   a.Lmax = p->Lmax;
+  a.lend = lend;
   if (!p->stop_early) {
     hipLaunchKernelGGL(so3_march_kernel<false>, dim3(B * C), dim3(kMarchThreads), 0, stream, a, p->lgf);
     return mch::hip_check(hipGetLastError(), "so3_march_kernel launch");
@@ -1147,6 +1194,13 @@ int mch_newton_march(void* plan, const void* sigma, long long ld_sigma, int B, i
       if ((rc = mch::hip_check(hipGetLastError(), "so3_march_kernel launch"))) return rc;
     }
   return MCH_OK;
+}
+
+int mch_newton_march(void* plan, const void* sigma, long long ld_sigma, int B, int C,
+                     const int* levels, int n_levels, int num_steps, int newton, float base_lr,
+                     float* alpha, float* beta, float* gamma, float* score, hipStream_t stream) {
+  return newton_march((So3Plan*)plan, sigma, ld_sigma, B, C, levels, n_levels, nullptr, num_steps,
+                      newton, base_lr, alpha, beta, gamma, score, stream);
 }
 
 int mch_so3_set_stop_early(void* plan, int stop_early) {
@@ -1191,10 +1245,10 @@ int mch_reduce_candidates(const float* alpha, const float* beta, const float* ga
 }
 
 // Full Matcha.search_orientations: grid top-C -> march -> best candidate.
-int mch_search_orientations(void* plan, const void* sigma, long long ld_sigma, int B, int C,
-                            const int* levels, int n_levels, int num_steps, int newton,
-                            float base_lr, float* out_alpha, float* out_beta, float* out_gamma,
-                            float* out_score, hipStream_t stream) {
+static int search_orientations(void* plan, const void* sigma, long long ld_sigma, int B, int C,
+                               const int* levels, int n_levels, const int* lend, int num_steps,
+                               int newton, float base_lr, float* out_alpha, float* out_beta,
+                               float* out_gamma, float* out_score, hipStream_t stream) {
   So3Plan* p = (So3Plan*)plan;
   if (!p) { mch::set_error("NULL plan"); return MCH_ERR_ARG; }
   if (B > p->max_batch || C > p->max_cand) {
@@ -1208,11 +1262,32 @@ int mch_search_orientations(void* plan, const void* sigma, long long ld_sigma, i
   float* cs = cg + (size_t)p->max_batch * p->max_cand;
   int rc = mch_sofft_topk(plan, sigma, ld_sigma, B, C, ca, cb, cg, cs, stream);
   if (rc) return rc;
-  rc = mch_newton_march(plan, sigma, ld_sigma, B, C, levels, n_levels, num_steps, newton, base_lr,
-                        ca, cb, cg, cs, stream);
+  rc = newton_march(p, sigma, ld_sigma, B, C, levels, n_levels, lend, num_steps, newton, base_lr, ca,
+                    cb, cg, cs, stream);
   if (rc) return rc;
   return mch_reduce_candidates(ca, cb, cg, cs, B, C, out_alpha, out_beta, out_gamma, out_score,
                                stream);
+}
+
+int mch_search_orientations(void* plan, const void* sigma, long long ld_sigma, int B, int C,
+                            const int* levels, int n_levels, int num_steps, int newton,
+                            float base_lr, float* out_alpha, float* out_beta, float* out_gamma,
+                            float* out_score, hipStream_t stream) {
+  return search_orientations(plan, sigma, ld_sigma, B, C, levels, n_levels, nullptr, num_steps,
+                             newton, base_lr, out_alpha, out_beta, out_gamma, out_score, stream);
+}
+
+int mch_search_orientations_ends(void* plan, const void* sigma, long long ld_sigma, int B, int C,
+                                 const int* levels, int n_levels, const int* march_ends,
+                                 int num_steps, int newton, float base_lr, float* out_alpha,
+                                 float* out_beta, float* out_gamma, float* out_score,
+                                 hipStream_t stream) {
+  if (!march_ends) {
+    mch::set_error("mch_search_orientations_ends: NULL march_ends");
+    return MCH_ERR_ARG;
+  }
+  return search_orientations(plan, sigma, ld_sigma, B, C, levels, n_levels, march_ends, num_steps,
+                             newton, base_lr, out_alpha, out_beta, out_gamma, out_score, stream);
 }
 
 // Host FP64 Wigner d table (textbook convention, D layout) -- exported so the

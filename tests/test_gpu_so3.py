@@ -290,3 +290,54 @@ def test_production_search_config_matches_oracle(gpu_device):
     assert geodesic_deg(A, Bt, G, ra, rb, rg).max() < 0.1
     assert np.max(np.abs(S - rs) / np.abs(rs)) < 1e-4
     assert geodesic_deg(A, Bt, G, *[np.array(x) for x in zip(*truth)]).max() < 0.5
+
+
+def test_mixed_march_ends_match_one_matcher_per_end(gpu_device):
+    # one batch, march ends 16 / 32 / 24 / 16 / 32 through one search launch:
+    # each particle's result equals bit for bit that of a Matcha built with
+    # L_max = its end on the same sigma, and the FP64 oracle's search at that end
+    from matcha_amd.core.Matcha import Matcha
+    rng = np.random.default_rng(77)
+    Li = [8, 16, 24]
+    ends = [16, 32, 24, 16, 32]
+    truth = [(1.0, 0.8, 2.0), (4.0, 2.2, 5.5), (0.3, 1.6, 3.9), (2.5, 0.4, 1.2), (5.0, 2.9, 0.7)]
+    sig = np.stack([peaked_sigma(rng, 32, *t, noise=0.05) for t in truth])
+    cfg = {"Li": Li, "candidates": 10, "num_steps": 5, "oversampling_factor_K": 4}
+    m = Matcha(len(ends), gpu_device, 32, dict(cfg))
+    got = [x.cpu().numpy() for x in m.search_orientations(_sig_t(sig, gpu_device), march_ends=ends)]
+    # the device table form gives the same
+    dev_ends = m.march_ends_tensor(ends)
+    got2 = [x.cpu().numpy() for x in m.search_orientations(_sig_t(sig, gpu_device), march_ends=dev_ends)]
+    for g1, g2 in zip(got, got2):
+        np.testing.assert_array_equal(g1, g2)
+    for L in sorted(set(ends)):
+        sel = [i for i, e in enumerate(ends) if e == L]
+        one = Matcha(len(sel), gpu_device, L, dict(cfg))
+        ref = [x.cpu().numpy() for x in one.search_orientations(_sig_t(sig[sel][:, :W.dsize(L)], gpu_device))]
+        for k, (g, r) in enumerate(zip(got, ref)):
+            np.testing.assert_array_equal(g[sel], r, err_msg=f"output {k}, end {L}")
+        ra, rb, rg, rs = so3.search_orientations(sig[sel][:, :W.dsize(L)], L, Li, candidates=10, num_steps=5,
+                                                 K=4)
+        geo = geodesic_deg(got[0][sel].astype(np.float64), got[1][sel].astype(np.float64),
+                           got[2][sel].astype(np.float64), ra, rb, rg)
+        assert geo.max() < 0.1, (L, geo)
+        assert np.max(np.abs(got[3][sel] - rs) / np.abs(rs)) < 1e-4
+
+
+def test_mixed_march_ends_reject_bad_ends(gpu_device):
+    from matcha_amd.core.Matcha import Matcha
+    m = Matcha(2, gpu_device, 24, {"Li": [8, 16], "candidates": 4, "num_steps": 2})
+    sig = torch.zeros((2, W.dsize(24)), dtype=torch.complex64, device=gpu_device)
+    with pytest.raises(ValueError):
+        m.search_orientations(sig, march_ends=[16, 25])
+    with pytest.raises(ValueError):
+        m.search_orientations(sig, march_ends=[7, 16])
+    with pytest.raises(ValueError):
+        m.search_orientations(sig, march_ends=torch.tensor([16, 24], dtype=torch.int64, device=gpu_device))
+    # a device table is read as given: an end outside [1, L_max] yields NaN for that particle only
+    bad = torch.tensor([16, 99], dtype=torch.int32, device=gpu_device)
+    a, b, g, s = (x.cpu().numpy() for x in m.search_orientations(sig, march_ends=bad))
+    assert np.isfinite(s[0]) and np.isnan(s[1])
+    m2 = Matcha(2, gpu_device, 24, {"Li": [8, 16], "candidates": 4, "num_steps": 2, "stop_early": True})
+    with pytest.raises(ValueError):
+        m2.search_orientations(sig, march_ends=[16, 24])
