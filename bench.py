@@ -94,6 +94,9 @@ def parse():
     p.add_argument("--tuned-streams", type=int, default=2,
                    help="streams the tuned path's batches alternate over (one plan set each)")
     p.add_argument("--no-plain", action="store_true", help="skip the plain FP32 path")
+    p.add_argument("--second-snr", type=float, default=-10.0,
+                   help="noise level of an untimed accuracy check of both paths (the workload's "
+                        "second SNR); pass nan to skip")
     p.add_argument("--snr", type=float, default=0.0)
     p.add_argument("--pool", type=int, default=2048, help="distinct particles cycled through")
     p.add_argument("--alternations", type=int, default=2)
@@ -229,6 +232,26 @@ def run_path(name, torch, dev, w, args, pool_vols, pool_R, pool_t, precision, B,
                                "search_orientations": ev[1].elapsed_time(ev[2]),
                                "search_shifts": ev[2].elapsed_time(ev[3])},
     }
+
+
+def accuracy_at(torch, plan_sets, precision, args, vols, R, t):
+    """Median / p90 geodesic error and exact-shift fraction of one path on a
+    pool of particles (untimed): the same alternations as the timed run."""
+    from matcha_amd.align import Alternation
+    from matcha_amd.synthetic import euler_zyz_to_matrix, geodesic_deg
+    c, m, s = plan_sets[0]
+    c.fle.set_compute_precision(precision)
+    pipe = Alternation(c, m, s, num_alternations=args.alternations, mode="global")
+    B = m.batchsize
+    errs, ok = [], []
+    for i in range(0, vols.shape[0], B):
+        a, b, g, ns, sh, ss = pipe.run(vols[i:i + B])
+        a, b, g = (x.cpu().numpy().astype(np.float64) for x in (a, b, g))
+        errs.append(geodesic_deg(R[i:i + B], euler_zyz_to_matrix(a, b, g)))
+        ok.append(np.all(sh.cpu().numpy() == t[i:i + B][:, ::-1], axis=1))
+    errs, ok = np.concatenate(errs), np.concatenate(ok)
+    return {"median_err_deg": float(np.median(errs)), "p90_err_deg": float(np.percentile(errs, 90)),
+            "shift_exact_frac": float(ok.mean()), "particles_checked": int(errs.size)}
 
 
 def time_gemm(torch, dev, ccm, vols, precision, reps=20):
@@ -587,6 +610,19 @@ def main():
     plain = None
     if not args.no_plain:
         plain = run_path("plain", torch, dev, w, args, pool_vols, pool_R, pool_t, "fp32", Bp, [plain_plans])
+    # accuracy of both paths at the workload's second noise level (untimed)
+    snr2 = None
+    if args.second_snr is not None and not math.isnan(args.second_snr):
+        n2 = max(Bt, Bp)
+        ch2 = [wl.particles(min(64, n2 - i), w["max_shift"], args.second_snr) for i in range(0, n2, 64)]
+        v2 = torch.cat([c[0] for c in ch2])
+        R2 = np.concatenate([c[1] for c in ch2])
+        t2 = np.concatenate([c[2] for c in ch2])
+        snr2 = {"snr_db": args.second_snr,
+                "tuned": accuracy_at(torch, tuned_sets, args.tuned_precision, args, v2, R2, t2)}
+        if plain_plans is not None:
+            snr2["plain"] = accuracy_at(torch, [plain_plans], "fp32", args, v2, R2, t2)
+        del v2
 
     # the tuned path's projection GEMM: algorithmic bytes = W^T hi + lo (4 B per
     # element) + the nonzero part of the hi/lo X operand (4 B per element), each
@@ -669,6 +705,7 @@ def main():
             "plain": plain,
             "tuned_over_plain": (tuned["volumes_per_s"] / plain["volumes_per_s"]) if plain else None,
             "meets_accuracy": tuned["median_err_deg"] <= 1.0 and (plain is None or plain["median_err_deg"] <= 1.0),
+            "accuracy_second_snr": snr2,
             "gemm_ms": {"tuned": gemm_t * 1e3, "plain_fp32": gemm_plain * 1e3 if gemm_plain else None},
             "gemm_tiles": tiles,
             "setup_s": t_setup,

@@ -375,3 +375,61 @@ def test_reduced_precision_upsampled(gpu_device):
 def test_shift_precision_rejects_unknown_mode(gpu_device):
     with pytest.raises(ValueError):
         _matcher(gpu_device, 32, 1, shift_precision="fp16")
+
+
+def test_reference_mode_alternations_match_oracle_chain(gpu_device):
+    # the reference's alternation loop (align_subtomograms.py:316-361) against its
+    # FP64 restatement built from the pinned oracle pieces: per alternation the
+    # rotated, normalised patches (oracle.resample), the FLE projection and sigma
+    # (oracle.fle), the search (oracle.so3), the tracker update (the host rule of
+    # rotation_ops.py:23-34) and the shift search on the original particles at the
+    # accumulated shift (oracle.shift); every alternation's angles within 0.1
+    # degree, shifts equal, trackers within 1e-4
+    from scipy.spatial.transform import Rotation
+    from matcha_amd import relion
+    from matcha_amd.align import Alternation
+    from matcha_amd.core.CrossCorrelationMatcher import CrossCorrelationMatcher
+    from matcha_amd.core.Matcha import Matcha
+    from matcha_amd.core.ShiftMatcher import ShiftMatcher
+    from oracle import resample as RS, so3
+    from tests.so3_helpers import geodesic_deg
+    N, B, bl, radius, L, Li, K, C, n_alt = 32, 4, 12, 14, 10, [4, 6, 8], 2, 10, 2
+    rng = np.random.default_rng(77)
+    tmpl = fle.blob_volume(rng, N, 10, sigma_range=(1.5, 3.0))
+    vols = []
+    for b in range(B):
+        M = Rotation.random(random_state=int(rng.integers(1 << 30))).as_matrix()
+        v = np.roll(RS.rotate_volume(tmpl, M), tuple(rng.integers(-2, 3, 3)), axis=(0, 1, 2))
+        vols.append(v + 0.3 * rng.standard_normal(v.shape))
+    vols = np.stack(vols).astype(np.float32)
+    ccm = CrossCorrelationMatcher(N, gpu_device, batchsize=B, bandlimit=bl, radius=radius)
+    ccm.set_template(tmpl.astype(np.float32))
+    m = Matcha(B, gpu_device, L, {"Li": Li, "candidates": C, "num_steps": 5, "oversampling_factor_K": K})
+    sm = ShiftMatcher({"N": N, "num_subtomograms_per_batch": B, "shift_upsample_factor": 1,
+                       "shift_search_radius": 2, "voxel_size": 1.0}, gpu_device)
+    sm.set_reference(tmpl.astype(np.float32))
+    pipe = Alternation(ccm, m, sm, num_alternations=n_alt, mode="reference")
+    pipe.run(torch.tensor(vols, device=gpu_device))
+    # the FP64 chain on the same particles
+    aT = fle.evaluate_t(tmpl.astype(np.float32).astype(np.float64)[None], N, bl, radius)[0]
+    refF = S.load_reference_full(tmpl.astype(np.float32), N, 1.0).astype(np.complex64).astype(np.complex128)
+    v64 = vols.astype(np.float64)
+    tracker = relion.identity_tracker(B)
+    shift = np.zeros((B, 3))
+    for it in range(n_alt):
+        Mp = relion.quat_to_matrix(tracker)
+        rot_vols = np.stack([RS.extract_rotated(v64[b], shift[b], Mp[b], radius) for b in range(B)])
+        aV = fle.evaluate_t(rot_vols, N, bl, radius)
+        sig = fle.compute_sigma(aT, aV, ccm.fle.ls, ccm.fle.mds, ccm.lmax)
+        ra, rb, rg, rs = so3.search_orientations(sig, L, Li, candidates=C, num_steps=5, K=K)
+        ga, gb, gg, gns, gsh, gss, gq = (x.cpu().numpy().astype(np.float64) for x in pipe.history[it])
+        geo = geodesic_deg(ga, gb, gg, ra, rb, rg)
+        assert geo.max() < 0.1, (it, geo)
+        # the tracker composes the angles the device search returned (FP32)
+        tracker, _ = relion.update_rotation_estimate(ga.astype(np.float32), gb.astype(np.float32),
+                                                     gg.astype(np.float32), tracker)
+        np.testing.assert_allclose(gq, tracker, atol=1e-4, err_msg=f"tracker, alternation {it}")
+        Mr = np.swapaxes(relion.quat_to_matrix(tracker), 1, 2)
+        ds, _ = S.estimate_shift(refF, v64, shift[:, None, :], Mr, N=N, radius=2)
+        shift = shift + ds
+        assert np.array_equal(gsh, shift), (it, gsh, shift)
