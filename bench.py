@@ -291,10 +291,13 @@ def time_march(torch, dev, ccm, matcher, vols, reps=5):
 def march_flops(levels, num_steps, candidates, batch):
     """FP32 operations of the fused march's shell work: per fundamental pair
     (m' >= |m|) and shell l >= m', the three-term recurrences of d, d', d''
-    (29) and the twelve accumulating products with sigma (24), for every
-    evaluation (num_steps + 1 per level) of every candidate."""
+    (29) and the twelve accumulating products with sigma (24) for the
+    evaluations whose derivatives are used (num_steps per level), and the d
+    recurrence (14) with its four products (8) for the last trial of a level,
+    whose score alone is used, for every candidate."""
     pair_shells = lambda L: sum((2 * m + 1) * (L - m + 1) for m in range(L + 1))
-    return 53.0 * (num_steps + 1) * sum(pair_shells(L) for L in levels) * candidates * batch
+    per = 53.0 * num_steps + (22.0 if num_steps > 0 else 53.0)
+    return per * sum(pair_shells(L) for L in levels) * candidates * batch
 
 
 def time_shift_kernels(torch, dev, sm, vols, reps=10):

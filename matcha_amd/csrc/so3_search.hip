@@ -538,6 +538,10 @@ __device__ __forceinline__ void beta_scalars(float beta, float* bs) {
   bs[7] = hs / hc;
 }
 
+// SCORE_ONLY: the score alone (out[0]; out[1..9] are zero): only the d
+// recurrence and its sums, in the same order of operations as the full
+// evaluation, so the score is bit for bit the same.
+template <bool SCORE_ONLY = false>
 __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int Li, const float* bs,
                            const float2* __restrict__ za, const float2* __restrict__ zg,
                            const float* __restrict__ lgf, const float4* __restrict__ lrec, float* red,
@@ -588,8 +592,9 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
     // test) by ratios, so that a task costs one exponential; every product then
     // stays below 2^40 (|term(a, bb)| <= 1), also where its coefficient is zero
     const float t0 = term(a, bb);
-    float tp1, tm1, tp2, tm2;  // term(a + 1, bb - 1), term(a - 1, bb + 1), term(a + 2, bb - 2), term(a - 2, bb + 2)
-    if (ratios) {
+    float tp1 = 0.f, tm1 = 0.f, tp2 = 0.f, tm2 = 0.f;  // term(a +- 1, bb -+ 1), term(a +- 2, bb -+ 2)
+    if (SCORE_ONLY) {
+    } else if (ratios) {
       tp1 = t0 * rcs;
       tm1 = t0 * rsc;
       tp2 = tp1 * rcs;
@@ -630,10 +635,14 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
         const float A = c.y * r;
         const float xb = x - mmp * c.z;
         const float nd = A * (xb * d - Cprev * pd);
-        const float ndp = A * (xb * dp - sx * d - Cprev * pdp);
-        const float ndpp = A * (xb * dpp - 2.f * sx * dp - x * d - Cprev * pdpp);
-        pd = d; pdp = dp; pdpp = dpp;
-        d = nd; dp = ndp; dpp = ndpp;
+        if (!SCORE_ONLY) {
+          const float ndp = A * (xb * dp - sx * d - Cprev * pdp);
+          const float ndpp = A * (xb * dpp - 2.f * sx * dp - x * d - Cprev * pdpp);
+          pdp = dp; pdpp = dpp;
+          dp = ndp; dpp = ndpp;
+        }
+        pd = d;
+        d = nd;
         Cprev = Pl * r * c.w;
       }
       if (!guarded || (valid && l >= mp)) {
@@ -642,8 +651,10 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
           U[i].x += sv[i].x * d;   U[i].y += sv[i].y * d;
-          U1[i].x += sv[i].x * dp; U1[i].y += sv[i].y * dp;
-          U2[i].x += sv[i].x * dpp; U2[i].y += sv[i].y * dpp;
+          if (!SCORE_ONLY) {
+            U1[i].x += sv[i].x * dp; U1[i].y += sv[i].y * dp;
+            U2[i].x += sv[i].x * dpp; U2[i].y += sv[i].y * dpp;
+          }
         }
       }
     };
@@ -695,6 +706,10 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
         if (v < 0) pg.y = -pg.y;
         const float2 ph = make_float2(pa.x * pg.x - pa.y * pg.y, pa.x * pg.y + pa.y * pg.x);
         const float q0r = ph.x * U[i].x - ph.y * U[i].y;
+        if (SCORE_ONLY) {
+          acc[0] += q0r;
+          continue;
+        }
         const float q0i = ph.x * U[i].y + ph.y * U[i].x;
         const float q1r = ph.x * U1[i].x - ph.y * U1[i].y;
         const float q1i = ph.x * U1[i].y + ph.y * U1[i].x;
@@ -880,7 +895,13 @@ __global__ __launch_bounds__(kMarchThreads, 5) void so3_march_kernel(MarchArgs a
       phase_table((double)ta, Li, za);
       phase_table((double)tg, Li, zg);
       __syncthreads();
-      march_eval(sp, sp_bytes, Li, bsc, za, zg, lgf, lrec, red, ev);
+      // the last trial of a level needs its score only (its gradient and
+      // Hessian would seed a step that is not taken: the next level starts by
+      // evaluating its own)
+      if (st == args.num_steps && st > 0)
+        march_eval<true>(sp, sp_bytes, Li, bsc, za, zg, lgf, lrec, red, ev);
+      else
+        march_eval<false>(sp, sp_bytes, Li, bsc, za, zg, lgf, lrec, red, ev);
       if (tid == 0) {
         const bool first = (st == 0);
         const bool acc = first || (ev[0] > opt[3]);
