@@ -789,45 +789,25 @@ __global__ __launch_bounds__(512) void fle_gemm16w_kernel(
       wg_barrier_keep_loads();
       if (st + 1 < nstage) issue(st + 1);
       const u32x4* base = lds + (st % NST) * kWideStageU4;
-      // the A fragments of the stage, then the W^T fragments one column block
-      // ahead of its products: 48 live fragment registers instead of 96 with
-      // the whole stage read first (which spilled registers, and a spill's
-      // reload waits on every load in flight, the next stage's DMA included)
-      const u32x4* wb = base + W0 + chalf * 1024 + lane;
-      u32x4 ah[RB], al[RB];
+      u32x4 w[2 * CB], ah[RB], al[RB];
+#pragma unroll
+      for (int q = 0; q < 2 * CB; ++q) w[q] = base[W0 + chalf * 1024 + q * 64 + lane];
 #pragma unroll
       for (int i = 0; i < RB; ++i) {
         ah[i] = base[(rb0 + i) * 64 + lane];
         al[i] = base[AREG + (rb0 + i) * 64 + lane];
       }
-      u32x4 w0 = wb[0], w1 = wb[64];
+      __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the products
 #pragma unroll
-      for (int c = 0; c < CB; ++c) {
-        // column block c = group c / 2, block c % 2: parts 4 (c / 2) + 2 (c % 2) + {0, 1}
-        u32x4 n0 = w0, n1 = w1;
-        if (c + 1 < CB) {
-          n0 = wb[(2 * c + 2) * 64];
-          n1 = wb[(2 * c + 3) * 64];
+      for (int i = 0; i < RB; ++i) {
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+          // column block c = group c / 2, block c % 2: parts 4 (c / 2) + 2 (c % 2) + {0, 1}
+          acc[i][c] = Split16<FP16>::mfma(ah[i], w[2 * c + 0], acc[i][c]);
+          acc[i][c] = Split16<FP16>::mfma(ah[i], w[2 * c + 1], acc[i][c]);
+          acc[i][c] = Split16<FP16>::mfma(al[i], w[2 * c + 0], acc[i][c]);
         }
-#pragma unroll
-        for (int i = 0; i < RB; ++i) acc[i][c] = Split16<FP16>::mfma(ah[i], w0, acc[i][c]);
-#pragma unroll
-        for (int i = 0; i < RB; ++i) acc[i][c] = Split16<FP16>::mfma(ah[i], w1, acc[i][c]);
-#pragma unroll
-        for (int i = 0; i < RB; ++i) acc[i][c] = Split16<FP16>::mfma(al[i], w0, acc[i][c]);
-        w0 = n0;
-        w1 = n1;
       }
-      // issue order: the A fragments and the first W^T pair, then each next
-      // pair before the 12 products of the current one (the scheduler would
-      // otherwise reuse the pair's registers and wait on every read)
-      __builtin_amdgcn_sched_group_barrier(0x100, 2 * RB + 2, 0);
-#pragma unroll
-      for (int c = 0; c + 1 < CB; ++c) {
-        __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
-        __builtin_amdgcn_sched_group_barrier(0x008, 3 * RB, 0);
-      }
-      __builtin_amdgcn_sched_group_barrier(0x008, 3 * RB, 0);
     }
     if (chalf < ntile) {  // a single-tile unit: the second half is a copy
       float* P = part + (long long)ks * Mpad * Ntot;
