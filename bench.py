@@ -334,7 +334,8 @@ def diff_workload(torch, dev, args):
     """Differentiable score at box 128, bandlimit 48, L = 48 with FP32
     coefficients: time of score + gradients in (omega, t, aT) for the batch,
     and a check of the rotation/shift gradient against FP32 central
-    differences (1e-3 rad / 1e-2 px) on a few particles."""
+    differences (1e-3 rad / 1e-2 px) on a few particles; the same with the
+    projection in BF16x3 (and its agreement with the FP32 run)."""
     from matcha_amd.diff import ScoreEngine
     from matcha_amd.synthetic import Workload
     from scipy.spatial.transform import Rotation
@@ -364,34 +365,52 @@ def diff_workload(torch, dev, args):
         s.sum().backward()
         return s
 
-    step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(args.diff_steps):
-        step()
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t1) / args.diff_steps * 1e3
-    # FP32 central differences on the first few particles
+    def fd_check(nchk):
+        """Worst relative error of the (omega, t) gradient against FP32 central
+        differences of the score (1e-3 rad / 1e-2 px) on the first particles."""
+        with torch.no_grad():
+            base, _ = eng.score_and_gradients(omega[:nchk], shift[:nchk], aT, vols[:nchk])
+            fd = torch.zeros((nchk, 6), device=dev, dtype=torch.float64)
+            for j, h in ((0, 1e-3), (1, 1e-2)):
+                for i in range(3):
+                    e = torch.zeros(3, device=dev)
+                    e[i] = h
+                    args_p = [omega[:nchk] + (e if j == 0 else 0), shift[:nchk] + (e if j == 1 else 0)]
+                    args_m = [omega[:nchk] - (e if j == 0 else 0), shift[:nchk] - (e if j == 1 else 0)]
+                    sp = eng.score_and_gradients(args_p[0], args_p[1], aT, vols[:nchk])[0][:, 0].double()
+                    sm = eng.score_and_gradients(args_m[0], args_m[1], aT, vols[:nchk])[0][:, 0].double()
+                    fd[:, 3 * j + i] = (sp - sm) / (2 * h)
+            an = base[:, 1:].double()
+            return ((an - fd).abs().amax(dim=1) / fd.abs().amax(dim=1)).max().item(), base
+
+    # the FP32 projection (the configuration's FP32 coefficients), then the
+    # BF16x3 split GEMM for the projection of the 4B patch and tangent volumes
     nchk = min(4, B)
-    with torch.no_grad():
-        base, _ = eng.score_and_gradients(omega[:nchk], shift[:nchk], aT, vols[:nchk])
-        fd = torch.zeros((nchk, 6), device=dev, dtype=torch.float64)
-        for j, h in ((0, 1e-3), (1, 1e-2)):
-            for i in range(3):
-                e = torch.zeros(3, device=dev)
-                e[i] = h
-                args_p = [omega[:nchk] + (e if j == 0 else 0), shift[:nchk] + (e if j == 1 else 0)]
-                args_m = [omega[:nchk] - (e if j == 0 else 0), shift[:nchk] - (e if j == 1 else 0)]
-                sp = eng.score_and_gradients(args_p[0], args_p[1], aT, vols[:nchk])[0][:, 0].double()
-                sm = eng.score_and_gradients(args_m[0], args_m[1], aT, vols[:nchk])[0][:, 0].double()
-                fd[:, 3 * j + i] = (sp - sm) / (2 * h)
-        an = base[:, 1:].double()
-        rel = ((an - fd).abs().amax(dim=1) / fd.abs().amax(dim=1)).max().item()
+    runs = {}
+    for prec in ("fp32", "bf16x3"):
+        eng.ccm.fle.set_compute_precision(prec)
+        step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.diff_steps):
+            step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t1) / args.diff_steps * 1e3
+        rel, base = fd_check(nchk)
+        runs[prec] = {"ms_per_step": ms, "particles_per_s": B / ms * 1e3,
+                      "fd_check": {"particles": nchk, "steps": "1e-3 rad / 1e-2 px", "max_rel_err": rel},
+                      "_base": base}
+    b32, b16 = runs["fp32"].pop("_base").double(), runs["bf16x3"].pop("_base").double()
+    runs["bf16x3"]["vs_fp32"] = {
+        "score_max_rel": ((b16[:, 0] - b32[:, 0]).abs() / b32[:, 0].abs()).max().item(),
+        "gradient_max_rel": ((b16[:, 1:] - b32[:, 1:]).abs().amax(dim=1) / b32[:, 1:].abs().amax(dim=1)).max().item()}
+    eng.ccm.fle.set_compute_precision("fp32")
     return {
         "workload": f"{B} particles, box {N}, FLE bandlimit {bl} radius {radius}, L = {L}, -6 dB, FP32 coefficients",
-        "ms_per_step": ms, "particles_per_s": B / ms * 1e3,
+        "ms_per_step": runs["fp32"]["ms_per_step"], "particles_per_s": runs["fp32"]["particles_per_s"],
         "step": "score + d/d(axis-angle, shift, template coefficients), forward and backward",
-        "fd_check": {"particles": nchk, "steps": "1e-3 rad / 1e-2 px", "max_rel_err": rel},
+        "fd_check": runs["fp32"]["fd_check"],
+        "precisions": runs,
         "setup_s": setup,
     }
 

@@ -168,3 +168,26 @@ def test_gradient_at_box128_L48_matches_fp32_central_differences(gpu_device):
     for b in range(B):
         err = np.abs(out[b, 1:] - fd[b]).max() / np.abs(fd[b]).max()
         assert err < 1e-3, (b, out[b, 1:], fd[b])
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
+def test_split_precision_projection_keeps_the_bars(gpu_device, precision):
+    # the projection of the patch and its tangent volumes in the 16-bit hi/lo
+    # split GEMM: score and (omega, t) gradient against the FP64 oracle to the
+    # FP32 path's bars (measured here: score 2e-7, gradient 2e-6 relative)
+    from matcha_amd.diff import ScoreEngine
+    rng = np.random.default_rng(42)
+    tmpl = fle.blob_volume(rng, N, 8, sigma_range=(1.0, 2.5))
+    parts = np.stack([fle.blob_volume(rng, N, 8, sigma_range=(1.0, 2.5)) for _ in range(3)]).astype(np.float32)
+    omega = np.array([[0.4, -1.1, 0.7], [2.0, 0.3, -0.9], [1e-5, -2e-5, 1e-5]])
+    t = np.array([[0.3, -0.45, 0.2], [-1.35, 0.6, 0.15], [0.25, 0.25, -0.7]])
+    eng = ScoreEngine(N, BL, RADIUS, L, 4, gpu_device, compute_precision=precision)
+    aT = eng.template_coefficients(torch.tensor(tmpl, dtype=torch.float32, device=gpu_device))
+    out, _ = _gpu(eng, omega, t, aT, parts, gpu_device)
+    aT64 = aT.cpu().numpy().astype(np.complex128)
+    for b in range(len(parts)):
+        pr = DF.Problem(parts[b], aT64, N, BL, RADIUS, L)
+        s = pr.score(omega[b], t[b])
+        assert abs(out[b, 0] - s) < 1e-4 * abs(s), (b, out[b, 0], s)
+        ref = pr.fd_gradient(omega[b], t[b])
+        assert np.abs(out[b, 1:] - ref).max() < 1e-3 * np.abs(ref).max(), (b, out[b, 1:], ref)
