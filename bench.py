@@ -108,7 +108,7 @@ def parse():
     p.add_argument("--batching", action="store_true",
                    help="also measure uneven batches (4096 particles, boxes 64/96/128, wedges, L 16/24/32)")
     p.add_argument("--batching-particles", type=int, default=4096)
-    p.add_argument("--batching-batch", type=int, default=64)
+    p.add_argument("--batching-batch", type=int, default=512)
     p.add_argument("--diff-steps", type=int, default=3)
     p.add_argument("--precision-study", action="store_true",
                    help="also run the precision study (box 128, L = 48: FP32 / BF16x3 / FP16x3 on the same particles)")
