@@ -122,14 +122,29 @@ int mch_search_orientations_ends(void* plan, const void* sigma, long long ld_sig
  * cos-weighted row of voxel v at k = v, the sin-weighted one at
  * k = Kpad/2 + v, Kpad a multiple of 128 with Kpad/2 >= V0; the classes
  * m = 3 mod 4 hold their sin rows negated: they share the operand of the
- * classes m = 1 mod 4), per-column output maps.  Replaces FLEBasis3D.precomp
- * (src/matcha/core/FLEBasis3D.py:111-540: NUFFT plans, SHT, interpolation
- * matrices). */
+ * classes m = 1 mod 4; NULL: built later by mch_fle_build_basis), per-column
+ * output maps.  Replaces FLEBasis3D.precomp (src/matcha/core/FLEBasis3D.py:
+ * 111-540: NUFFT plans, SHT, interpolation matrices). */
 int mch_fle_plan_create(int N, int V0, int Kpad, int Ntot, const int* class_colpad,
                         const int* orbit, const float* basis, const int* chan_plus,
                         const int* chan_minus, const int* chan_m, int ne, int max_batch,
                         void** out_plan);
 int mch_fle_plan_destroy(void* plan);
+
+/* The plan's projection table built on the device in FP64 (rounded to FP32)
+ * instead of uploaded: pass basis = NULL to mch_fle_plan_create, then this.
+ * vox (V0 x 4 doubles, host): (r, cos theta, phi, h w) per fundamental voxel;
+ * chan_lm (Ntot x 2 ints, host): (l, m) per column, l < 0 for padding;
+ * chan_val (Ntot x 3 doubles, host): (lambda_lk, c_lk, sign of the sin half).
+ * Entry (v, col) = h w j_l(lambda r / radius) P_l^m(cos theta) c cos(m phi),
+ * and the sign times the same with sin(m phi) at row Kpad/2 + v.  Replaces the
+ * basis evaluation of FLEBasis3D.precomp (src/matcha/core/FLEBasis3D.py:
+ * 111-540, there a radial table sampled and interpolated).  Synchronous. */
+int mch_fle_build_basis(void* plan, const double* vox, const int* chan_lm, const double* chan_val,
+                        double radius);
+/* Copy of the plan's table W (Kpad x Ntot floats, device) into out (device);
+ * no reference counterpart (tests). */
+int mch_fle_copy_basis(void* plan, float* out, hipStream_t stream);
 
 /* a = B^H f: volumes (B, N^3) float32 (row stride ld_vol) -> coefficients
  * (B, ne) complex64 (row stride lda), reference order.  Replaces

@@ -45,6 +45,8 @@ _SIGNATURES = [
     ("mch_fle_plan_create", c_int, [c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p, c_int, c_int, ctypes.POINTER(c_void_p)]),
     ("mch_fle_plan_destroy", c_int, [c_void_p]),
+    ("mch_fle_build_basis", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_double]),
+    ("mch_fle_copy_basis", c_int, [c_void_p, c_void_p, c_void_p]),
     ("mch_fle_project", c_int, [c_void_p, c_void_p, c_ll, c_int, c_void_p, c_ll, c_void_p]),
     ("mch_fle_project_shifted", c_int, [c_void_p, c_void_p, c_ll, c_int, c_void_p, c_void_p, c_ll, c_void_p]),
     ("mch_fle_gemm_only", c_int, [c_void_p, c_int, c_void_p]),

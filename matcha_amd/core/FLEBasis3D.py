@@ -142,34 +142,36 @@ def fundamental_voxels(N: int, radius: float):
     ball and their +N/2 mirror images are not in the array.  Such images get
     the index -1 (read as 0); orbits with no image in the array are dropped.
 
-    Returns coords (V0, 3) int, orbit flat indices (V0, 16) int32 (axis 0 = x,
-    -1 outside the array), weights 1/|stabiliser| (V0,)."""
+    Returns coords (V0, 3) int (x >= y >= 0, z >= 0, ordered by x, then y,
+    then z), orbit flat indices (V0, 16) int32 (image p + 4 q + 8 t of
+    _orbit, axis 0 = x, -1 outside the array), weights 1/|stabiliser| (V0,)."""
     if not radius > 0:
         raise ValueError(f"radius must be positive, got {radius}")
     R = int(math.floor(radius * (1 + 1e-13)))
     c = N // 2
-    coords, orbits, wts = [], [], []
     lim = radius * (1 + 1e-13)
-
-    def flat(g):
-        if min(g) < -c or max(g) >= N - c:
-            return -1
-        return ((g[0] + c) * N + (g[1] + c)) * N + (g[2] + c)
-
-    for x in range(0, R + 1):
-        for y in range(0, x + 1):
-            for z in range(0, R + 1):
-                if math.sqrt(x * x + y * y + z * z) > lim:
-                    continue
-                imgs = _orbit((x, y, z))
-                ids = [flat(g) for g in imgs]
-                if max(ids) < 0:
-                    continue
-                stab = sum(1 for g in imgs if g == (x, y, z))
-                coords.append((x, y, z))
-                orbits.append(ids)
-                wts.append(1.0 / stab)
-    return np.asarray(coords, dtype=np.int64), np.asarray(orbits, dtype=np.int32), np.asarray(wts)
+    x, y, z = np.meshgrid(np.arange(R + 1), np.arange(R + 1), np.arange(R + 1), indexing="ij")
+    keep = (y <= x) & (np.sqrt((x * x + y * y + z * z).astype(np.float64)) <= lim)
+    x, y, z = x[keep], y[keep], z[keep]
+    # the 16 images R^p My^t Mz^q v, in the order p + 4 q + 8 t
+    imgs = np.empty((x.size, 16, 3), dtype=np.int64)
+    for t in (0, 1):
+        for q in (0, 1):
+            yt = -y if t else y
+            zq = -z if q else z
+            ax, ay = x, yt
+            for p in range(4):
+                imgs[:, p + 4 * q + 8 * t] = np.stack([ax, ay, zq], axis=1)
+                ax, ay = -ay, ax
+    inside = np.all((imgs >= -c) & (imgs < N - c), axis=2)
+    g = imgs + c
+    ids = np.where(inside, (g[..., 0] * N + g[..., 1]) * N + g[..., 2], -1)
+    any_in = ids.max(axis=1) >= 0
+    base = np.stack([x, y, z], axis=1)[:, None, :]
+    stab = np.all(imgs == base, axis=2).sum(axis=1)
+    coords = np.stack([x, y, z], axis=1)[any_in]
+    return (coords.astype(np.int64), ids[any_in].astype(np.int32),
+            1.0 / stab[any_in].astype(np.float64))
 
 
 def _legendre_orthonormal(lmax: int, x: np.ndarray):
@@ -191,11 +193,15 @@ def _legendre_orthonormal(lmax: int, x: np.ndarray):
     return P
 
 
-def projection_tables(N: int, radius: float, spectrum, lmax: int, BK: int = 16):
-    """Host FP64 -> FP32 tables of the symmetry-adapted projection GEMM.
+def projection_layout(N: int, radius: float, spectrum, lmax: int, BK: int = 16):
+    """Layout of the symmetry-adapted projection GEMM and the inputs of its
+    table, which the library builds on the device (``mch_fle_build_basis``).
 
     Returns dict with V0, Kpad, Ntot, class_cols, class_colpad, orbits (V0, 16)
-    int32, W (Kpad, Ntot) float32, chan_plus/chan_minus/chan_m (Ntot,) int32."""
+    int32, chan_plus/chan_minus/chan_m (Ntot,) int32, and the table's inputs:
+    vox (V0, 4) FP64 = (r, cos theta, phi, h w) per fundamental voxel,
+    chan_lm (Ntot, 2) int32 = (l, m) per column (-1 for padding), chan_val
+    (Ntot, 3) FP64 = (lambda_lk, c_lk, sign of the sin half) per column."""
     ls, ks, ms, lmds, cs = spectrum
     coords, orbits, w = fundamental_voxels(N, radius)
     V0 = len(coords)
@@ -203,7 +209,6 @@ def projection_tables(N: int, radius: float, spectrum, lmax: int, BK: int = 16):
     r = np.sqrt(x * x + y * y + z * z)
     x0 = np.where(r > 0, z / np.where(r > 0, r, 1), 1.0)
     phi0 = np.arctan2(y, x)
-    P = _legendre_orthonormal(lmax, x0)
     index = {(int(l), int(k), int(m)): i for i, (l, k, m) in enumerate(zip(ls, ks, ms))}
     classes = [[] for _ in range(N_CLASSES)]
     for (l, k, m), i in index.items():
@@ -222,40 +227,61 @@ def projection_tables(N: int, radius: float, spectrum, lmax: int, BK: int = 16):
     cols = [len(c) for c in classes]
     colpad = [((n + COL_PAD - 1) // COL_PAD) * COL_PAD for n in cols]
     Ntot = int(sum(colpad))
-    Wb = np.zeros((Kpad, Ntot), dtype=np.float32)
     chan_plus = np.full(Ntot, -1, dtype=np.int32)
     chan_minus = np.full(Ntot, -1, dtype=np.int32)
     chan_m = np.zeros(Ntot, dtype=np.int32)
+    chan_lm = np.full((Ntot, 2), -1, dtype=np.int32)
+    chan_val = np.zeros((Ntot, 3), dtype=np.float64)
     h = radius ** -1.5
-    jl_cache = {}
     off = 0
     for cls in range(N_CLASSES):
-        ch = classes[cls]
-        n = len(ch)
-        if n:
-            for (l, k, m, ip, im) in ch:
-                if (l, k) not in jl_cache:
-                    jl_cache[(l, k)] = special.spherical_jn(l, lmds[ip] * r / radius)
-            J = np.stack([jl_cache[(l, k)] for (l, k, m, ip, im) in ch], axis=1)
-            Pm = np.stack([P[l][m] for (l, k, m, ip, im) in ch], axis=1)
-            mm = np.array([m for (l, k, m, ip, im) in ch], dtype=np.float64)
-            cc = np.array([cs[ip] for (l, k, m, ip, im) in ch])
-            rad = (h * w)[:, None] * J * Pm * cc[None, :]
-            ang = phi0[:, None] * mm[None, :]
-            # classes (eps, 3) read the (eps, 1) operand, which has X2 and X3
-            # negated: their sin rows are negated here and their imaginary
-            # output in the kernel's finalize
-            ssign = -1.0 if cls % 4 == 3 else 1.0
-            Wb[0:V0, off:off + n] = rad * np.cos(ang)
-            Wb[KH:KH + V0, off:off + n] = ssign * rad * np.sin(ang)
-            for j, (l, k, m, ip, im) in enumerate(ch):
-                chan_plus[off + j] = ip
-                chan_minus[off + j] = im if m > 0 else -1
-                chan_m[off + j] = m
+        # classes (eps, 3) read the (eps, 1) operand, which has X2 and X3
+        # negated: their sin rows are negated here and their imaginary
+        # output in the kernel's finalize
+        ssign = -1.0 if cls % 4 == 3 else 1.0
+        for j, (l, k, m, ip, im) in enumerate(classes[cls]):
+            chan_plus[off + j] = ip
+            chan_minus[off + j] = im if m > 0 else -1
+            chan_m[off + j] = m
+            chan_lm[off + j] = (l, m)
+            chan_val[off + j] = (lmds[ip], cs[ip], ssign)
         off += colpad[cls]
+    vox = np.stack([r, x0, phi0, h * w], axis=1)
     return dict(V0=V0, Kpad=Kpad, Ntot=Ntot, class_cols=cols, class_colpad=colpad,
-                orbits=np.ascontiguousarray(orbits), W=Wb, chan_plus=chan_plus,
-                chan_minus=chan_minus, chan_m=chan_m)
+                orbits=np.ascontiguousarray(orbits), chan_plus=chan_plus, chan_minus=chan_minus,
+                chan_m=chan_m, vox=np.ascontiguousarray(vox), chan_lm=chan_lm, chan_val=chan_val,
+                lmax=int(lmax), radius=float(radius))
+
+
+def host_basis(layout) -> np.ndarray:
+    """The table W (Kpad, Ntot) float32 of a layout on the host in FP64 (SciPy's
+    spherical Bessel functions, the host Legendre recurrence): what the
+    library's device build computes, for tests and small sizes."""
+    V0, Kpad, Ntot = layout["V0"], layout["Kpad"], layout["Ntot"]
+    KH = Kpad // 2
+    r, x0, phi0, hw = (layout["vox"][:, i] for i in range(4))
+    P = _legendre_orthonormal(layout["lmax"], x0)
+    Wb = np.zeros((Kpad, Ntot), dtype=np.float32)
+    jl_cache = {}
+    for col in range(Ntot):
+        l, m = (int(v) for v in layout["chan_lm"][col])
+        if l < 0:
+            continue
+        lam, c, ssign = layout["chan_val"][col]
+        if (l, lam) not in jl_cache:
+            jl_cache[(l, lam)] = special.spherical_jn(l, lam * r / layout["radius"])
+        rad = hw * jl_cache[(l, lam)] * P[l][m] * c
+        ang = phi0 * m
+        Wb[0:V0, col] = rad * np.cos(ang)
+        Wb[KH:KH + V0, col] = ssign * rad * np.sin(ang)
+    return Wb
+
+
+def projection_tables(N: int, radius: float, spectrum, lmax: int, BK: int = 16):
+    """projection_layout with the host-built table as ``W`` (tests)."""
+    t = projection_layout(N, radius, spectrum, lmax, BK)
+    t["W"] = host_basis(t)
+    return t
 
 
 # compute precision of the projection GEMM (mch_fle_set_precision)
@@ -330,11 +356,12 @@ class FLEBasis3D:
     def _build_tables(self):
         if self.N != self.N1:
             raise ValueError("odd box sizes are not supported")
-        t = projection_tables(self.N, self.radius, (self.ls, self.ks, self.ms, self.lmds, self.cs),
+        t = projection_layout(self.N, self.radius, (self.ls, self.ks, self.ms, self.lmds, self.cs),
                               self.lmax, self.BK)
+        self._layout = t
         self.V0, self.Kpad, self.Ntot = t["V0"], t["Kpad"], t["Ntot"]
         self.class_cols, self.class_colpad = t["class_cols"], t["class_colpad"]
-        self._orbits, self._basis = t["orbits"], t["W"]
+        self._orbits = t["orbits"]
         self._chan_plus, self._chan_minus, self._chan_m = t["chan_plus"], t["chan_minus"], t["chan_m"]
 
     def _make_plan(self):
@@ -343,15 +370,28 @@ class FLEBasis3D:
         with torch.cuda.device(self.device):
             rc = _lib.lib().mch_fle_plan_create(
                 self.N, self.V0, self.Kpad, self.Ntot, cc,
-                self._orbits.ctypes.data_as(ctypes.c_void_p),
-                self._basis.ctypes.data_as(ctypes.c_void_p),
+                self._orbits.ctypes.data_as(ctypes.c_void_p), None,
                 self._chan_plus.ctypes.data_as(ctypes.c_void_p),
                 self._chan_minus.ctypes.data_as(ctypes.c_void_p),
                 self._chan_m.ctypes.data_as(ctypes.c_void_p),
                 self.ne, self.batchsize, ctypes.byref(handle))
-        _lib.check(rc, "mch_fle_plan_create")
-        self._plan = handle
-        self._basis = None  # the device copy is the one used
+            _lib.check(rc, "mch_fle_plan_create")
+            self._plan = handle
+            # the table itself is built on the device, in FP64, from the layout
+            t = self._layout
+            rc = _lib.lib().mch_fle_build_basis(
+                handle, t["vox"].ctypes.data_as(ctypes.c_void_p), t["chan_lm"].ctypes.data_as(ctypes.c_void_p),
+                t["chan_val"].ctypes.data_as(ctypes.c_void_p), float(self.radius))
+            _lib.check(rc, "mch_fle_build_basis")
+
+    def basis_table(self) -> torch.Tensor:
+        """The projection table W (Kpad, Ntot) float32 as the library holds it
+        (a device copy; tests)."""
+        W = torch.empty((self.Kpad, self.Ntot), dtype=torch.float32, device=self.device)
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().mch_fle_copy_basis(self._plan, W.data_ptr(), _lib.stream_handle(self.device)),
+                       "mch_fle_copy_basis")
+        return W
 
     def __del__(self):
         h = getattr(self, "_plan", None)

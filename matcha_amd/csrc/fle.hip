@@ -22,6 +22,7 @@
 //      a_{lkm} = Re + i Im and a_{lk,-m} = (-1)^m conj(a_{lkm}), scattered
 //      to the reference order.
 #include "common.hpp"
+#include "special.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -1234,6 +1235,34 @@ int launch_gemm(FlePlan* p, int B, hipStream_t stream, int* splits_out, int* ski
 
 }  // namespace
 
+// The projection table W (Kpad, Ntot) in FP64, rounded to FP32: entry
+// (v, col) of column (l, k, m) is h w_v j_l(lam_lk r_v / R) P_l^m(cos theta_v)
+// c_lk cos(m phi_v) in the cos half (row v) and ssign times the same with
+// sin(m phi_v) in the sin half (row KH + v); vox = (r, cos theta, phi, h w)
+// per fundamental voxel (FP64, from the host), chan_lm = (l, m) per column
+// (l < 0: padding, left zero), chan_val = (lam_lk, c_lk, ssign) per column.
+// Rows beyond V0 in either half stay zero (the caller clears W first).
+__global__ __launch_bounds__(256) void fle_basis_kernel(float* __restrict__ W, int V0, int KH, int Ntot,
+                                                        const double4* __restrict__ vox,
+                                                        const int2* __restrict__ chan_lm,
+                                                        const double4* __restrict__ chan_val, double radius) {
+  const long long total = (long long)V0 * Ntot;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int v = (int)(e / Ntot), col = (int)(e - (long long)v * Ntot);
+    const int2 lm = chan_lm[col];
+    if (lm.x < 0) continue;
+    const double4 q = vox[v], cv = chan_val[col];
+    const double J = mch::special::sph_jn(lm.x, cv.x * q.x / radius);
+    const double P = mch::special::legendre_on(lm.x, lm.y, q.y);
+    const double rad = q.w * J * P * cv.y;
+    double sn, cs;
+    sincos(q.z * (double)lm.y, &sn, &cs);
+    W[(long long)v * Ntot + col] = (float)(rad * cs);
+    W[(long long)(KH + v) * Ntot + col] = (float)(cv.z * rad * sn);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -1243,7 +1272,7 @@ int mch_fle_plan_create(int N, int V0, int Kpad, int Ntot, const int* class_colp
                         const int* orbit, const float* basis, const int* chan_plus,
                         const int* chan_minus, const int* chan_m, int ne, int max_batch,
                         void** out_plan) {
-  if (!out_plan || !class_colpad || !orbit || !basis || !chan_plus || !chan_minus || !chan_m) {
+  if (!out_plan || !class_colpad || !orbit || !chan_plus || !chan_minus || !chan_m) {
     mch::set_error("mch_fle_plan_create: NULL argument");
     return MCH_ERR_ARG;
   }
@@ -1327,7 +1356,10 @@ int mch_fle_plan_create(int N, int V0, int Kpad, int Ntot, const int* class_colp
     MCH_TRY(hipMemcpy(p->fcoord, fc.data(), (size_t)V0 * sizeof(int), hipMemcpyHostToDevice), "copy fcoord");
   }
   MCH_TRY(hipMalloc(&p->W, (size_t)Kpad * Ntot * sizeof(float)), "hipMalloc(W)");
-  MCH_TRY(hipMemcpy(p->W, basis, (size_t)Kpad * Ntot * sizeof(float), hipMemcpyHostToDevice), "copy W");
+  if (basis)
+    MCH_TRY(hipMemcpy(p->W, basis, (size_t)Kpad * Ntot * sizeof(float), hipMemcpyHostToDevice), "copy W");
+  else  // built on the device by mch_fle_build_basis
+    MCH_TRY(hipMemset(p->W, 0, (size_t)Kpad * Ntot * sizeof(float)), "memset(W)");
   MCH_TRY(hipMalloc(&p->chan_plus, (size_t)Ntot * sizeof(int)), "hipMalloc(chan)");
   MCH_TRY(hipMemcpy(p->chan_plus, chan_plus, (size_t)Ntot * sizeof(int), hipMemcpyHostToDevice), "copy chan");
   MCH_TRY(hipMalloc(&p->chan_minus, (size_t)Ntot * sizeof(int)), "hipMalloc(chan)");
@@ -1343,6 +1375,63 @@ int mch_fle_plan_create(int N, int V0, int Kpad, int Ntot, const int* class_colp
 fail:
   free_fle_plan(p);
   return rc;
+}
+
+int mch_fle_build_basis(void* plan, const double* vox, const int* chan_lm, const double* chan_val,
+                        double radius) {
+  FlePlan* p = (FlePlan*)plan;
+  if (!p || !vox || !chan_lm || !chan_val) {
+    mch::set_error("mch_fle_build_basis: NULL argument");
+    return MCH_ERR_ARG;
+  }
+  if (!(radius > 0.0)) {
+    mch::set_error("mch_fle_build_basis: radius must be positive");
+    return MCH_ERR_ARG;
+  }
+  for (int c = 0; c < p->Ntot; ++c) {
+    const int l = chan_lm[2 * c], m = chan_lm[2 * c + 1];
+    if (l >= 0 && (m < 0 || m > l || l > 1000)) {
+      mch::set_error("mch_fle_build_basis: column %d has (l, m) = (%d, %d)", c, l, m);
+      return MCH_ERR_ARG;
+    }
+  }
+  int rc = MCH_OK;
+  void *dv = nullptr, *dl = nullptr, *dc = nullptr;
+#define MCH_TRY(expr, what) do { if ((rc = mch::hip_check((expr), what)) != MCH_OK) goto done; } while (0)
+  {
+    MCH_TRY(hipMalloc(&dv, (size_t)p->V0 * 4 * sizeof(double)), "hipMalloc(vox)");
+    MCH_TRY(hipMalloc(&dl, (size_t)p->Ntot * 2 * sizeof(int)), "hipMalloc(chan_lm)");
+    MCH_TRY(hipMalloc(&dc, (size_t)p->Ntot * 4 * sizeof(double)), "hipMalloc(chan_val)");
+    MCH_TRY(hipMemcpy(dv, vox, (size_t)p->V0 * 4 * sizeof(double), hipMemcpyHostToDevice), "copy vox");
+    MCH_TRY(hipMemcpy(dl, chan_lm, (size_t)p->Ntot * 2 * sizeof(int), hipMemcpyHostToDevice), "copy chan_lm");
+    // (lam, c, ssign) padded to four doubles per column
+    std::vector<double> cv((size_t)p->Ntot * 4, 0.0);
+    for (int c = 0; c < p->Ntot; ++c)
+      for (int i = 0; i < 3; ++i) cv[(size_t)c * 4 + i] = chan_val[(size_t)c * 3 + i];
+    MCH_TRY(hipMemcpy(dc, cv.data(), cv.size() * sizeof(double), hipMemcpyHostToDevice), "copy chan_val");
+    MCH_TRY(hipMemset(p->W, 0, (size_t)p->Kpad * p->Ntot * sizeof(float)), "memset(W)");
+    const long long total = (long long)p->V0 * p->Ntot;
+    const int grid = (int)std::min<long long>((total + 255) / 256, 65536LL);
+    hipLaunchKernelGGL(fle_basis_kernel, dim3(grid), dim3(256), 0, 0, p->W, p->V0, p->Kpad / 2, p->Ntot,
+                       (const double4*)dv, (const int2*)dl, (const double4*)dc, radius);
+    MCH_TRY(hipGetLastError(), "fle_basis_kernel launch");
+    MCH_TRY(hipDeviceSynchronize(), "fle_basis_kernel");
+  }
+#undef MCH_TRY
+done:
+  (void)hipFree(dv); (void)hipFree(dl); (void)hipFree(dc);
+  return rc;
+}
+
+int mch_fle_copy_basis(void* plan, float* out, hipStream_t stream) {
+  FlePlan* p = (FlePlan*)plan;
+  if (!p || !out) {
+    mch::set_error("mch_fle_copy_basis: NULL argument");
+    return MCH_ERR_ARG;
+  }
+  return mch::hip_check(hipMemcpyAsync(out, p->W, (size_t)p->Kpad * p->Ntot * sizeof(float),
+                                       hipMemcpyDeviceToDevice, stream),
+                        "mch_fle_copy_basis");
 }
 
 int mch_fle_plan_destroy(void* plan) {

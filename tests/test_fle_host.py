@@ -57,6 +57,42 @@ def test_fundamental_voxels_partition_the_ball(N, radius):
     assert (orbits.min() >= 0) == (radius < N // 2)
 
 
+def _fundamental_voxels_loops(N, radius):
+    # the definition, voxel by voxel: x >= y >= 0, z >= 0 inside the ball, the
+    # 16 images R^p My^t Mz^q v in the order p + 4 q + 8 t, -1 outside the array
+    c, lim = N // 2, radius * (1 + 1e-13)
+    R = int(np.floor(lim))
+    coords, orbits, wts = [], [], []
+    for x in range(R + 1):
+        for y in range(x + 1):
+            for z in range(R + 1):
+                if np.sqrt(x * x + y * y + z * z) > lim:
+                    continue
+                imgs = []
+                for t in (0, 1):
+                    for q in (0, 1):
+                        for p in range(4):
+                            a, b, cz = x, (-y if t else y), (-z if q else z)
+                            for _ in range(p):
+                                a, b = -b, a
+                            imgs.append((a, b, cz))
+                ids = [((g[0] + c) * N + g[1] + c) * N + g[2] + c
+                       if min(g) >= -c and max(g) < N - c else -1 for g in imgs]
+                if max(ids) < 0:
+                    continue
+                coords.append((x, y, z))
+                orbits.append(ids)
+                wts.append(1.0 / sum(1 for g in imgs if g == (x, y, z)))
+    return np.array(coords), np.array(orbits), np.array(wts)
+
+
+@pytest.mark.parametrize("N,radius", [(32, 13), (32, 16), (20, 10), (16, 9.5), (24, 11.5)])
+def test_fundamental_voxels_match_their_definition(N, radius):
+    coords, orbits, w = P.fundamental_voxels(N, radius)
+    rc, ro, rw = _fundamental_voxels_loops(N, radius)
+    assert np.array_equal(coords, rc) and np.array_equal(orbits, ro) and np.array_equal(w, rw)
+
+
 def emulate_projection(t, vols, ne):
     B = vols.shape[0]
     f = np.concatenate([vols.reshape(B, -1), np.zeros((B, 1))], axis=1)

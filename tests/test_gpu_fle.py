@@ -271,3 +271,19 @@ def test_batches_of_changing_size_reuse_the_operand(gpu_device, mode):
             got = f.evaluate_t(torch.tensor(v, device=gpu_device)).cpu().numpy()
             err = np.max(np.abs(got - ref), axis=1) / np.max(np.abs(ref), axis=1)
             assert err.max() < tol, (t, B, int(np.argmax(err)), err.max())
+
+
+@pytest.mark.parametrize("N,bl,radius", [(24, 8, 11), (32, 12, 15), (32, 12, 16), (48, 16, 23)])
+def test_device_built_table_matches_host_fp64(gpu_device, N, bl, radius):
+    # the projection table the library builds on the device (FP64 special
+    # functions, rounded to FP32) against the host FP64 build (SciPy's j_l, the
+    # host Legendre recurrence) of the same layout: equal to FP32 rounding
+    from matcha_amd.core.FLEBasis3D import FLEBasis3D, host_basis
+    f = FLEBasis3D(N, bl, 1e-4, radius=radius, device=gpu_device)
+    got = f.basis_table().cpu().numpy()
+    ref = host_basis(f._layout)
+    scale = np.abs(ref).max(axis=0, keepdims=True) + 1e-30
+    assert np.max(np.abs(got - ref) / scale) < 2e-7
+    # padding rows and columns are zero
+    assert np.all(got[f.V0:f.Kpad // 2] == 0) and np.all(got[f.Kpad // 2 + f.V0:] == 0)
+    assert np.all(got[:, f._layout["chan_lm"][:, 0] < 0] == 0)
