@@ -64,9 +64,12 @@ def test_search_end_to_end_vs_fp64_oracle(gpu_device, small_problem, mode):
     assert np.max(geo_R(p["R"], euler_zyz_to_matrix(ra, rb, rg))) < 1.0
 
 
-def test_projection_box128_bl48_vs_fp64_subset(gpu_device):
+@pytest.mark.parametrize("bl", [48, 65])
+def test_projection_box128_vs_fp64_subset(gpu_device, bl):
+    # bandlimit 48 (the precision workload) and 65 (the reference's production
+    # correlator bandlimit, max(Li) of its config; lmax 92, a 17.9 GB table)
     from matcha_amd.core.FLEBasis3D import FLEBasis3D
-    N, bl, radius = 128, 48, 60
+    N, radius = 128, 60
     rng = np.random.default_rng(128)
     vol = fle.blob_volume(rng, N, 60, sigma_range=(1.0, 5.0)) + 0.5 * rng.standard_normal((N, N, N))
     vols = vol[None].astype(np.float32)
