@@ -302,27 +302,31 @@ __global__ __launch_bounds__(256) void sofft_slice_kernel(
               part_val, part_idx);
 }
 
-// Same slice as sofft_slice_kernel for n = 16 R: the phase table
+// Same slice as sofft_slice_kernel for n <= 16 R: the phase table
 // E[u][x] = e^{2 pi i u x / n} (|u| < L0) is built once in LDS and serves both
-// DFT stages, and each thread owns an R x R block of (alpha, gamma) points of
-// stage B, so every pair of E / G loads feeds R outputs.
+// DFT stages, and each thread owns R x R (alpha, gamma) points of stage B, so
+// every pair of E / G loads feeds R outputs.  The 16 x 16 threads cover
+// NP = 16 R >= n points per axis: for n < NP the tables carry NP columns (the
+// surplus ones repeat phases mod n) and the points at alpha or gamma >= n
+// are computed but neither stored nor ranked.
 template <int R>
 __global__ __launch_bounds__(256) void sofft_slice_blk_kernel(
     const float2* __restrict__ sig, long long ldsig, const float* __restrict__ dgrid,
-    const float2* __restrict__ twiddle, int L0, int nsz, int C, float* __restrict__ part_val,
+    const float2* __restrict__ twiddle, int n, int L0, int nsz, int C, float* __restrict__ part_val,
     int* __restrict__ part_idx, float* __restrict__ grid_out) {
-  constexpr int n = 16 * R, nn = n * n;
+  constexpr int NP = 16 * R;
+  const int nn = n * n;
   extern __shared__ float4 smem4[];
   float2* smem = reinterpret_cast<float2*>(smem4);
   const int j = blockIdx.x;
   const int b = blockIdx.y;
   const int W = 2 * L0 - 1;
   float2* F = smem;            // W*W
-  float2* G = F + W * W;       // W*n   G[ui][g]
-  float2* E = G + W * n;       // W*n   E[ui][x]
+  float2* G = F + W * W;       // W*NP  G[ui][g]
+  float2* E = G + W * NP;      // W*NP  E[ui][x]
   const int tid = threadIdx.x;
-  for (int q = tid; q < W * n; q += 256) {
-    const int ui = q / n, xx = q - (q / n) * n;
+  for (int q = tid; q < W * NP; q += 256) {
+    const int ui = q / NP, xx = q - (q / NP) * NP;
     int k = ((ui - (L0 - 1)) * xx) % n;
     if (k < 0) k += n;
     E[q] = twiddle[k];
@@ -332,17 +336,17 @@ __global__ __launch_bounds__(256) void sofft_slice_blk_kernel(
   for (int q = tid; q < W * W; q += 256) F[q] = sofft_f_entry(s, dj, L0, q / W - (L0 - 1), q % W - (L0 - 1));
   __syncthreads();
   // stage A: G(u, g) = sum_v F(u, v) e^{2 pi i v g / n}
-  constexpr int SA = 256 % n == 0 ? 256 / n : 1, KR = 8;
-  if (256 % n == 0 && W <= KR * SA) {
-    // a thread owns column g of rows ui0, ui0 + S, ... (S = 256 / n, at most
+  constexpr int SA = 256 % NP == 0 ? 256 / NP : 1, KR = 8;
+  if (256 % NP == 0 && W <= KR * SA) {
+    // a thread owns column g of rows ui0, ui0 + S, ... (S = 256 / NP, at most
     // KR rows): one E read per term serves all its rows
     constexpr int S = SA;
-    const int g = tid % n, ui0 = tid / n;
+    const int g = tid % NP, ui0 = tid / NP;
     float2 acc[KR];
 #pragma unroll
     for (int k = 0; k < KR; ++k) acc[k] = make_float2(0.f, 0.f);
     for (int vi = 0; vi < W; ++vi) {
-      const float2 t = E[vi * n + g];
+      const float2 t = E[vi * NP + g];
 #pragma unroll
       for (int k = 0; k < KR; ++k) {
         const int ui = ui0 + k * S;
@@ -355,22 +359,47 @@ __global__ __launch_bounds__(256) void sofft_slice_blk_kernel(
     }
 #pragma unroll
     for (int k = 0; k < KR; ++k)
-      if (ui0 + k * S < W) G[(ui0 + k * S) * n + g] = acc[k];
+      if (ui0 + k * S < W) G[(ui0 + k * S) * NP + g] = acc[k];
   } else {
-    for (int q = tid; q < W * n; q += 256) {
-      const int ui = q / n, g = q - (q / n) * n;
-      float ax = 0.f, ay = 0.f;
+    // register blocks: thread (ty, tx) owns rows ty + 16 k (four at a time)
+    // of columns tx + 16 c, c < R; per term 4 F and R E reads feed 4 R
+    // complex products
+    const int ty = tid >> 4, tx = tid & 15;
+    for (int kb = 0; kb < W; kb += 64) {
+      float2 acc[4][R];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int c = 0; c < R; ++c) acc[k][c] = make_float2(0.f, 0.f);
+      int urow[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) urow[k] = min(kb + ty + 16 * k, W - 1) * W;
       for (int vi = 0; vi < W; ++vi) {
-        const float2 f = F[ui * W + vi], t = E[vi * n + g];
-        ax += f.x * t.x - f.y * t.y;
-        ay += f.x * t.y + f.y * t.x;
+        float2 f[4], t[R];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) f[k] = F[urow[k] + vi];
+#pragma unroll
+        for (int c = 0; c < R; ++c) t[c] = E[vi * NP + tx + 16 * c];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+          for (int c = 0; c < R; ++c) {
+            acc[k][c].x += f[k].x * t[c].x - f[k].y * t[c].y;
+            acc[k][c].y += f[k].x * t[c].y + f[k].y * t[c].x;
+          }
       }
-      G[q] = make_float2(ax, ay);
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (kb + ty + 16 * k < W)
+#pragma unroll
+          for (int c = 0; c < R; ++c) G[(kb + ty + 16 * k) * NP + tx + 16 * c] = acc[k][c];
     }
   }
   __syncthreads();
-  // stage B: f(a, g) = Re sum_u e^{2 pi i u a / n} G(u, g) / n^2 on the thread's block
-  const int a0 = (tid >> 4) * R, g0 = (tid & 15) * R;
+  // stage B: f(a, g) = Re sum_u e^{2 pi i u a / n} G(u, g) / n^2 on the
+  // thread's points a = ty + 16 r, g = tx + 16 c (r, c < R): the G reads of a
+  // wave are consecutive, its E reads four broadcasts
+  const int ty = tid >> 4, tx = tid & 15;
   float vals[R * R];
 #pragma unroll
   for (int i = 0; i < R * R; ++i) vals[i] = 0.f;
@@ -378,8 +407,8 @@ __global__ __launch_bounds__(256) void sofft_slice_blk_kernel(
     float2 e[R], gv[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      e[r] = E[ui * n + a0 + r];
-      gv[r] = G[ui * n + g0 + r];
+      e[r] = E[ui * NP + ty + 16 * r];
+      gv[r] = G[ui * NP + tx + 16 * r];
     }
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -393,10 +422,14 @@ __global__ __launch_bounds__(256) void sofft_slice_blk_kernel(
 #pragma unroll
     for (int c = 0; c < R; ++c) {
       const float v = vals[r * R + c] * inv;
-      keys[r * R + c] = float_key(v);
-      if (grid_out != nullptr) grid_out[((long long)b * n + j) * nn + (a0 + r) * n + g0 + c] = v;
+      const int a = ty + 16 * r, g = tx + 16 * c;
+      const bool in = NP == n || (a < n && g < n);
+      keys[r * R + c] = in ? float_key(v) : 0u;
+      if (grid_out != nullptr && in) grid_out[((long long)b * n + j) * nn + a * n + g] = v;
     }
-  block_top_c(keys, [&](int i) { return (a0 + i / R) * n + g0 + i % R; }, C,
+  // the thread's flat ids a n + g rise with r R + c, so its first maximum is
+  // the smallest id among equal values
+  block_top_c(keys, [&](int i) { return (ty + 16 * (i / R)) * n + tx + 16 * (i % R); }, C,
               ((long long)b * n + j) * C, j * nn, part_val, part_idx);
 }
 
@@ -991,6 +1024,30 @@ extern "C" {
 
 const char* mch_last_error(void) { return mch::get_error(); }
 
+// The blocked slice kernel's R for an n-point grid (0: the grid is left to
+// sofft_slice_kernel), and its LDS bytes: F, then G and E of 16 R columns.
+static int sofft_blk_r(int L0, int n) {
+  const int R = (n + 15) / 16;
+  const size_t W = 2 * L0 - 1;
+  return R <= 8 && (W * W + 2 * W * 16 * R) * sizeof(float2) <= 160 * 1024 ? R : 0;
+}
+static size_t sofft_blk_lds(int L0, int R) {
+  const size_t W = 2 * L0 - 1;
+  return (W * W + 2 * W * 16 * R) * sizeof(float2);
+}
+static const void* sofft_blk_fn(int R) {
+  switch (R) {
+    case 1: return (const void*)sofft_slice_blk_kernel<1>;
+    case 2: return (const void*)sofft_slice_blk_kernel<2>;
+    case 3: return (const void*)sofft_slice_blk_kernel<3>;
+    case 4: return (const void*)sofft_slice_blk_kernel<4>;
+    case 5: return (const void*)sofft_slice_blk_kernel<5>;
+    case 6: return (const void*)sofft_slice_blk_kernel<6>;
+    case 7: return (const void*)sofft_slice_blk_kernel<7>;
+    default: return (const void*)sofft_slice_blk_kernel<8>;
+  }
+}
+
 int mch_so3_plan_create(int L_max, int L0, int K, int max_batch, int max_candidates,
                         void** out_plan) {
   if (!out_plan) { mch::set_error("out_plan is NULL"); return MCH_ERR_ARG; }
@@ -1029,6 +1086,10 @@ int mch_so3_plan_create(int L_max, int L0, int K, int max_batch, int max_candida
   for (int k = 0; k < (int)lg.size(); ++k) lg[k] = (float)(lgamma((double)k + 1.0) / M_LN2);
   int rc = MCH_OK;
 #define MCH_TRY(expr, what) do { if ((rc = mch::hip_check((expr), what)) != MCH_OK) goto fail; } while (0)
+  if (sofft_blk_r(L0, n) > 0 && sofft_blk_lds(L0, sofft_blk_r(L0, n)) > 64 * 1024)
+    MCH_TRY(hipFuncSetAttribute(sofft_blk_fn(sofft_blk_r(L0, n)), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)sofft_blk_lds(L0, sofft_blk_r(L0, n))),
+            "hipFuncSetAttribute(sofft_slice_blk_kernel)");
   MCH_TRY(hipMalloc(&p->d_grid, dg.size() * sizeof(float)), "hipMalloc(d_grid)");
   MCH_TRY(hipMemcpy(p->d_grid, dg.data(), dg.size() * sizeof(float), hipMemcpyHostToDevice), "copy d_grid");
   MCH_TRY(hipMalloc(&p->twiddle, n * sizeof(float2)), "hipMalloc(twiddle)");
@@ -1077,18 +1138,23 @@ static int launch_sofft_slice(So3Plan* p, const void* sigma, long long ld_sigma,
                               float* grid_out, hipStream_t stream) {
   const int W = 2 * p->L0 - 1;
   const int n = p->n;
-  const size_t lds_blk = (size_t)(W * W + 2 * W * n) * sizeof(float2);
-  if (n % 16 == 0 && n <= 64 && lds_blk <= 64 * 1024) {
+  const int R = sofft_blk_r(p->L0, n);
+  if (R > 0) {
     const dim3 grid(n, B);
+    const size_t lds_blk = sofft_blk_lds(p->L0, R);
 #define MCH_BLK(R)                                                                               \
   hipLaunchKernelGGL(sofft_slice_blk_kernel<R>, grid, dim3(256), lds_blk, stream,                \
-                     (const float2*)sigma, ld_sigma, p->d_grid, p->twiddle, p->L0, p->nsz, C,     \
+                     (const float2*)sigma, ld_sigma, p->d_grid, p->twiddle, n, p->L0, p->nsz, C,  \
                      p->part_val, p->part_idx, grid_out)
-    switch (n / 16) {
+    switch (R) {
       case 1: MCH_BLK(1); break;
       case 2: MCH_BLK(2); break;
       case 3: MCH_BLK(3); break;
-      default: MCH_BLK(4); break;
+      case 4: MCH_BLK(4); break;
+      case 5: MCH_BLK(5); break;
+      case 6: MCH_BLK(6); break;
+      case 7: MCH_BLK(7); break;
+      default: MCH_BLK(8); break;
     }
 #undef MCH_BLK
     return mch::hip_check(hipGetLastError(), "sofft_slice_blk_kernel launch");

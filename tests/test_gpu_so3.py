@@ -65,10 +65,14 @@ def test_sofft_grid_matches_oracle(gpu_device, L0, K):
     assert np.max(np.abs(f - ref)) < 1e-5 * rng_
 
 
-def test_sofft_topk_values_and_angles(gpu_device):
+# n = 64 fills the blocked slice kernel's 16 x 16 blocks of 4 x 4 points;
+# n = 40 and 120 leave masked points (alpha or gamma >= n) in the last blocks,
+# which must never be ranked
+@pytest.mark.parametrize("L0,K", [(8, 4), (10, 2), (30, 2)])
+def test_sofft_topk_values_and_angles(gpu_device, L0, K):
     from matcha_amd.core.SOFFT import SOFFT
-    rng = np.random.default_rng(21)
-    B, L0, K, C = 3, 8, 4, 10
+    rng = np.random.default_rng(21 + L0)
+    B, C = 3, 10
     sig = random_sigma(rng, B, L0)
     sof = SOFFT(L0, gpu_device, B, oversampling_factor=K, candidates=C)
     a, b, g, s = (x.cpu().numpy() for x in sof.topk(_sig_t(sig, gpu_device), C))
@@ -244,8 +248,12 @@ def test_evaluate_score_grad_hessian_high_degree(gpu_device, L):
     assert err.max() < 2e-5, (err.max(axis=(0, 1)))
 
 
-@pytest.mark.parametrize("L0,K", [(40, 1), (30, 2)])
-def test_sofft_grid_generic_kernel_matches_oracle(gpu_device, L0, K):
+# n = 2 L0 K points per axis: n = 80 (a multiple of 16) and the padded
+# n = 120, 40 and 12 run the blocked slice kernel (16 x 16 blocks of R x R
+# points, the points beyond n masked); n = 96 with 95 x 95 coefficients
+# exceeds its LDS and runs the one-point-per-thread kernel
+@pytest.mark.parametrize("L0,K", [(40, 1), (30, 2), (10, 2), (6, 1), (48, 1)])
+def test_sofft_grid_large_and_padded_matches_oracle(gpu_device, L0, K):
     from matcha_amd.core.SOFFT import SOFFT
     rng = np.random.default_rng(L0 + K)
     B = 1
