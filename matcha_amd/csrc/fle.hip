@@ -279,23 +279,29 @@ __global__ void fle_symmetrize_kernel(const float* __restrict__ vols, long long 
   }
 }
 
-// BF16x3 path: the same combinations for 4 consecutive fundamental voxels per
-// thread.  Their 64 orbit values are requested at once (indices from the
+// BF16x3 path: the same combinations for 4 fundamental voxels per thread, 256
+// apart.  Their 64 orbit values are requested at once (indices from the
 // packed coordinates: image p + 4q + 8t of (x, y, z) is R^p My^t Mz^q, R the
-// quarter turn (x, y) -> (-y, x)), and each row's 4 cos-weighted and 4
-// sin-weighted values leave as 8-byte stores of hi and of lo.  Columns of a
-// last partial group (v >= V0) receive zeros, inside the half's K padding.
+// quarter turn (x, y) -> (-y, x)), and each row's values leave as 2-byte
+// stores of hi and of lo, 128 contiguous bytes per wave.  Columns v >= V0 of
+// the half's K padding receive zeros.
 template <bool SHIFTED>
 __global__ __launch_bounds__(256) void fle_symmetrize_bf16_kernel(
     const float* __restrict__ vols, long long ld_vol, int B, const int* __restrict__ fcoord, int V0, int N,
     int Kpad, int Mpad, const float* __restrict__ shift_zyx, uint16_t* __restrict__ Xhi,
     uint16_t* __restrict__ Xlo) {
-  const int nq = (V0 + 3) >> 2;
+  // a workgroup takes 1024 consecutive voxels of one particle, its thread l
+  // voxels l, l + 256, l + 512, l + 768: each load instruction of a wave reads
+  // 64 consecutive voxels (the z runs of two or three columns), whole cache
+  // lines of each image
+  const int nq = ((V0 + 1023) >> 10) << 8;  // threads per particle
   const long long tot = (long long)B * nq;
   const int c = N >> 1;
+  const int KH = Kpad >> 1;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < tot;
        t += (long long)gridDim.x * blockDim.x) {
     const int b = (int)(t / nq), qd = (int)(t - (long long)b * nq);
+    const int v0 = ((qd >> 8) << 10) + (qd & 255);  // voxel of u = 0; u adds 256 u
     const float* f = vols + (long long)b * ld_vol;
     // the patch vol[a + s] at the particle's shift (SHIFTED; values as
     // extract_shifted_patches copies them out)
@@ -304,7 +310,7 @@ __global__ __launch_bounds__(256) void fle_symmetrize_bf16_kernel(
     int pcs[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int v = 4 * qd + u;
+      const int v = v0 + 256 * u;
       pcs[u] = fcoord[v < V0 ? v : 0];
     }
     // an integral shift under which every image of the thread's voxels stays in
@@ -331,7 +337,7 @@ __global__ __launch_bounds__(256) void fle_symmetrize_bf16_kernel(
     auto fill = [&](auto&& value_at) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const bool ok = 4 * qd + u < V0;
+        const bool ok = v0 + 256 * u < V0;
         const int x = pcs[u] & 1023, y = (pcs[u] >> 10) & 1023, z = pcs[u] >> 20;
 #pragma unroll
         for (int tt = 0; tt < 2; ++tt)
@@ -398,29 +404,31 @@ __global__ __launch_bounds__(256) void fle_symmetrize_bf16_kernel(
           Split16<false>::split(Bv[0] + Bv[1], hx[2][u], lx[2][u]);
           Split16<false>::split(A[1] - A[0], hx[3][u], lx[3][u]);
         }
-        // the cos-weighted operands of the 4 voxels at k = 4 qd.., the sin-weighted
-        // ones at k = Kpad / 2 + 4 qd..: 8 bytes of hi and of lo per row and half
-        auto pk = [](const uint16_t (&h)[4]) {
-          return make_uint2((unsigned)h[0] | ((unsigned)h[1] << 16), (unsigned)h[2] | ((unsigned)h[3] << 16));
-        };
+        // the cos-weighted operand of voxel v at k = v, the sin-weighted one at
+        // k = Kpad / 2 + v; voxels V0 <= v < Kpad / 2 get zeros
         const int cls = eps * 4 + rho;
-        const long long orow = ((long long)cls * Mpad + b) * Kpad + 4LL * qd;
-        const long long oim = ((long long)cls * Mpad + B + b) * Kpad + 4LL * qd;
-        const int kh = Kpad / 2;
+        const long long orow = ((long long)cls * Mpad + b) * Kpad + v0;
+        const long long oim = ((long long)cls * Mpad + B + b) * Kpad + v0;
         // (X2 and X3 are zero for rho = 0, 2: stored only where a GEMM tile reads them)
         const bool odd = rho & 1;
-        *reinterpret_cast<uint2*>(Xhi + orow) = pk(hx[0]);
-        *reinterpret_cast<uint2*>(Xlo + orow) = pk(lx[0]);
-        if (odd || straddle_row(b, B)) {
-          *reinterpret_cast<uint2*>(Xhi + orow + kh) = pk(hx[1]);
-          *reinterpret_cast<uint2*>(Xlo + orow + kh) = pk(lx[1]);
+        const bool st1 = odd || straddle_row(b, B), st2 = odd || straddle_row(B + b, B);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (v0 + 256 * u >= KH) break;
+          const int du = 256 * u;
+          Xhi[orow + du] = hx[0][u];
+          Xlo[orow + du] = lx[0][u];
+          if (st1) {
+            Xhi[orow + KH + du] = hx[1][u];
+            Xlo[orow + KH + du] = lx[1][u];
+          }
+          if (st2) {
+            Xhi[oim + du] = hx[2][u];
+            Xlo[oim + du] = lx[2][u];
+          }
+          Xhi[oim + KH + du] = hx[3][u];
+          Xlo[oim + KH + du] = lx[3][u];
         }
-        if (odd || straddle_row(B + b, B)) {
-          *reinterpret_cast<uint2*>(Xhi + oim) = pk(hx[2]);
-          *reinterpret_cast<uint2*>(Xlo + oim) = pk(lx[2]);
-        }
-        *reinterpret_cast<uint2*>(Xhi + oim + kh) = pk(hx[3]);
-        *reinterpret_cast<uint2*>(Xlo + oim + kh) = pk(lx[3]);
       }
   }
 }
@@ -1644,7 +1652,7 @@ int mch_fle_project_shifted(void* plan, const float* vols, long long ld_vol, int
     if ((rc = mch::hip_check(hipGetLastError(), "fle_fill_kernel launch"))) return rc;
   }
   if (p->precision == kBf16x3) {
-    const long long nq = (long long)B * ((p->V0 + 3) / 4);
+    const long long nq = (long long)B * (((p->V0 + 1023) / 1024) * 256);
     const int nb4 = (int)std::min<long long>((nq + 255) / 256, 16384);
     if (shift_zyx)
       hipLaunchKernelGGL(fle_symmetrize_bf16_kernel<true>, dim3(nb4), dim3(256), 0, stream, vols, ld_vol, B,
