@@ -33,6 +33,35 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
+// Maximum of w over the wave, returned to every lane.  Within each row of 16
+// lanes four DPP exchanges (xor 1, xor 2, the half-row and the row mirror:
+// after them every lane holds its row's maximum), then the four row maxima
+// read as scalars: no LDS round trips, unlike a shuffle through ds_bpermute.
+// Every lane of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_max_u64(unsigned long long w) {
+  const int lo = (int)(unsigned)w, hi = (int)(unsigned)(w >> 32);
+  const unsigned olo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+  const unsigned ohi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+  const unsigned long long o = ((unsigned long long)ohi << 32) | olo;
+  return o > w ? o : w;
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long w) {
+  w = dpp_max_u64<0xB1>(w);   // quad_perm [1, 0, 3, 2]
+  w = dpp_max_u64<0x4E>(w);   // quad_perm [2, 3, 0, 1]
+  w = dpp_max_u64<0x141>(w);  // row_half_mirror
+  w = dpp_max_u64<0x140>(w);  // row_mirror
+  const int lo = (int)(unsigned)w, hi = (int)(unsigned)(w >> 32);
+  unsigned long long r = 0ull;
+#pragma unroll
+  for (int row = 0; row < 4; ++row) {
+    const unsigned long long x = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane(hi, 16 * row) << 32) |
+                                 (unsigned)__builtin_amdgcn_readlane(lo, 16 * row);
+    r = x > r ? x : r;
+  }
+  return r;
+}
+
 // D layout size for 0 <= l <= L: sum (2l+1)^2.
 __host__ __device__ __forceinline__ int dsize(int L) {
   return L < 0 ? 0 : (L + 1) * (2 * L + 1) * (2 * L + 3) / 3;

@@ -162,32 +162,49 @@ __device__ __forceinline__ float key_float(unsigned k) {
 // index qof(i) of the slice, increasing with i; round c writes
 // part_val/part_idx[orow + c] (index qbase + q, or -1 once the slice is
 // exhausted).  A thread's best is one 64-bit word (key, ~q), so a round is one
-// wave max-reduction and one four-way merge; each thread keeps its best word
-// and only the owner of the round's winner rescans its values.
+// wave max-reduction and one four-way merge; each thread keeps its best two
+// words, and the owner of the round's winner rescans its values only when it
+// has no second left.
 template <int NV, class QOf>
 __device__ __forceinline__ void block_top_c(unsigned (&hk)[NV], QOf qof, int C, long long orow,
                                             int qbase, float* __restrict__ part_val,
                                             int* __restrict__ part_idx) {
   __shared__ unsigned long long red[2][4];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int ib = 0;
-  // first maximum over i (= smallest q among equal keys); empty slots (0) never win
-  auto local_best = [&]() {
-    unsigned bk = 0u;
-    ib = 0;
-#pragma unroll
-    for (int i = 0; i < NV; ++i)
-      if (hk[i] > bk) { bk = hk[i]; ib = i; }
-    return bk == 0u ? 0ull : ((unsigned long long)bk << 32) | (unsigned)~qof(ib);
+  // the thread's best word and the best of the rest (first maxima over i =
+  // smallest q among equal keys; empty slots (0) never win): a win promotes
+  // the second, and only a win without a known second rescans the values
+  int ib = 0, ib2 = 0;
+  unsigned long long best = 0ull, second = 0ull;
+  bool has2 = false;
+  auto word = [&](unsigned k, int i) {
+    return k == 0u ? 0ull : ((unsigned long long)k << 32) | (unsigned)~qof(i);
   };
-  unsigned long long best = local_best();
-  for (int c = 0; c < C; ++c) {
-    unsigned long long w = best;
+  auto local_best2 = [&]() {
+    unsigned b1 = 0u, b2 = 0u;
+    int i1 = 0, i2 = 0;
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      const unsigned long long o = __shfl_xor(w, off, 64);
-      w = o > w ? o : w;
+    for (int i = 0; i < NV; ++i) {
+      const unsigned k = hk[i];
+      if (k > b1) {
+        b2 = b1;
+        i2 = i1;
+        b1 = k;
+        i1 = i;
+      } else if (k > b2) {
+        b2 = k;
+        i2 = i;
+      }
     }
+    ib = i1;
+    ib2 = i2;
+    best = word(b1, i1);
+    second = word(b2, i2);
+    has2 = true;
+  };
+  local_best2();
+  for (int c = 0; c < C; ++c) {
+    unsigned long long w = mch::wave_max_u64(best);
     // two buffers: round c + 1 writes the other one, and round c + 2 starts
     // after every thread has passed round c + 1's barrier, i.e. read round c's
     if (lane == 0) red[c & 1][wave] = w;
@@ -202,7 +219,13 @@ __device__ __forceinline__ void block_top_c(unsigned (&hk)[NV], QOf qof, int C, 
 #pragma unroll
       for (int i = 0; i < NV; ++i)
         if (i == ib) hk[i] = 0u;
-      best = local_best();
+      if (has2) {
+        best = second;
+        ib = ib2;
+        has2 = false;
+      } else {
+        local_best2();
+      }
     }
     if (threadIdx.x == 0) {
       part_val[orow + c] = w != 0ull ? key_float((unsigned)(w >> 32)) : -INFINITY;

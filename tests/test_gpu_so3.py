@@ -91,6 +91,32 @@ def test_sofft_topk_values_and_angles(gpu_device, L0, K):
                 assert abs(g[bi, c] - rg[bi, c]) < 1e-6
 
 
+# a sigma with only the l = 0 coefficient makes every grid value the same
+# float (d^0 = 1, the DFT phases of frequency 0 are exactly 1): the top C are
+# then the first C flat indices, (beta, alpha, gamma) = (0, 0, 0..C-1), the
+# tie order of the oracle's top-k.  Covers the blocked slice kernel (n = 64),
+# its padded form (n = 40) and the one-point-per-thread kernel (n = 96).
+@pytest.mark.parametrize("L0,K", [(8, 4), (10, 2), (48, 1)])
+def test_sofft_topk_ties_take_the_lowest_flat_indices(gpu_device, L0, K):
+    from matcha_amd.core.SOFFT import SOFFT
+    B, C = 2, 10
+    sig = np.zeros((B, W.dsize(L0)), dtype=np.complex128)
+    sig[0, 0] = 0.75 - 0.25j
+    sof = SOFFT(L0, gpu_device, B, oversampling_factor=K, candidates=C)
+    a, b, g, s = (x.cpu().numpy().astype(np.float64) for x in sof.topk(_sig_t(sig, gpu_device), C))
+    f = so3.sofft_grid(sig, L0, K)
+    assert np.all(f[0] == f[0].flat[0]) and np.all(f[1] == 0.0)
+    ids, vals = so3.topk_grid(f, C)
+    ra, rb, rg = so3.ids_to_angles(ids, f.shape[1])
+    n = 2 * L0 * K
+    np.testing.assert_array_equal(ids[:, :, 2], np.tile(np.arange(C), (B, 1)))
+    np.testing.assert_allclose(s, vals, atol=1e-7)
+    np.testing.assert_allclose(g, rg, atol=1e-6)
+    np.testing.assert_allclose(a, ra, atol=0)
+    np.testing.assert_allclose(b, rb, atol=0)
+    assert np.allclose(g[0], 2 * np.pi * np.arange(C) / n, atol=1e-6)
+
+
 def test_newton_march_matches_oracle(gpu_device):
     rng = np.random.default_rng(31)
     B, C, L = 2, 4, 16
