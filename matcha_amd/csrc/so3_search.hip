@@ -87,6 +87,7 @@ struct So3Plan {
   int nsz;               // compute_size(L0) = dsize(L0-1): entries used by the grid
   float* d_grid;         // (n, nsz)
   float2* twiddle;       // (n) e^{2 pi i k / n}
+  float2* etab;          // (2 L0 - 1, 16 R) e^{2 pi i u x / n} of the blocked slice kernel (R > 0)
   float2* sig_pk;        // (max_batch, pk_size(Lmax))
   float* part_val;       // (max_batch, n, max_cand)
   int* part_idx;         // (max_batch, n, max_cand)
@@ -235,21 +236,30 @@ __device__ __forceinline__ void block_top_c(unsigned (&hk)[NV], QOf qof, int C, 
 }
 
 // F(u, v) = sum over max(|u|, |v|) <= l < L0 of d^l_{uv}(beta_j) sigma^l_{uv},
-// the terms' loads issued eight at a time (indices clamped to the last shell,
-// surplus terms masked), so that a thread waits for one round trip per eight
-// terms.
+// the terms' loads issued eight at a time (surplus terms read the last
+// shell's entry and are masked), so that a thread waits for one round trip
+// per eight terms.  The indices: each from dindex with the shell clamped, or
+// (STEP) stepped from shell l to l + 1 by dindex(l + 1, u, v) -
+// dindex(l, u, v) = (2l + 1)^2 + 4l + 2u + 4.  Measured in the blocked slice
+// kernel (batch 1024 at n = 64, L0 = 8; 64 at n = 120, L0 = 30): stepping
+// 0.75-0.77 against 0.81 ms where an entry has one batch of terms, 2.28
+// against 1.93 ms where it has up to four.
+template <bool STEP = false>
 __device__ __forceinline__ float2 sofft_f_entry(const float2* __restrict__ s, const float* __restrict__ dj,
                                                 int L0, int u, int v) {
   const int l0 = abs(u) > abs(v) ? abs(u) : abs(v);
   float ax = 0.f, ay = 0.f;
+  int id = mch::dindex(l0, u, v);
   for (int lb = l0; lb < L0; lb += 8) {
     float dv[8];
     float2 sv[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const int id = mch::dindex(min(lb + k, L0 - 1), u, v);
+      if (!STEP) id = mch::dindex(min(lb + k, L0 - 1), u, v);
       dv[k] = dj[id];
       sv[k] = s[id];
+      const int l = lb + k;
+      if (STEP && l + 1 < L0) id += (2 * l + 1) * (2 * l + 1) + 4 * l + 2 * u + 4;
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -335,7 +345,7 @@ __global__ __launch_bounds__(256) void sofft_slice_kernel(
 template <int R>
 __global__ __launch_bounds__(256) void sofft_slice_blk_kernel(
     const float2* __restrict__ sig, long long ldsig, const float* __restrict__ dgrid,
-    const float2* __restrict__ twiddle, int n, int L0, int nsz, int C, float* __restrict__ part_val,
+    const float2* __restrict__ etab, int n, int L0, int nsz, int C, float* __restrict__ part_val,
     int* __restrict__ part_idx, float* __restrict__ grid_out) {
   constexpr int NP = 16 * R;
   const int nn = n * n;
@@ -348,16 +358,106 @@ __global__ __launch_bounds__(256) void sofft_slice_blk_kernel(
   float2* G = F + W * W;       // W*NP  G[ui][g]
   float2* E = G + W * NP;      // W*NP  E[ui][x]
   const int tid = threadIdx.x;
-  for (int q = tid; q < W * NP; q += 256) {
-    const int ui = q / NP, xx = q - (q / NP) * NP;
-    int k = ((ui - (L0 - 1)) * xx) % n;
-    if (k < 0) k += n;
-    E[q] = twiddle[k];
+  // the plan's phase table, eight loads in flight per thread
+  for (int q0 = tid; q0 < W * NP; q0 += 8 * 256) {
+    float2 t[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t[k] = etab[min(q0 + 256 * k, W * NP - 1)];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (q0 + 256 * k < W * NP) E[q0 + 256 * k] = t[k];
   }
   const float2* s = sig + (long long)b * ldsig;
   const float* dj = dgrid + (long long)j * nsz;
-  for (int q = tid; q < W * W; q += 256) F[q] = sofft_f_entry(s, dj, L0, q / W - (L0 - 1), q % W - (L0 - 1));
+  // stepped indices where every entry has at most one batch of terms
+  if (L0 <= 8) {
+    for (int q = tid; q < W * W; q += 256) F[q] = sofft_f_entry<true>(s, dj, L0, q / W - (L0 - 1), q % W - (L0 - 1));
+  } else {
+    for (int q = tid; q < W * W; q += 256) F[q] = sofft_f_entry<false>(s, dj, L0, q / W - (L0 - 1), q % W - (L0 - 1));
+  }
   __syncthreads();
+  if constexpr (R % 4 == 0) {
+    // both DFT stages on the FP32 matrix cores (v_mfma_f32_16x16x4_f32: lane
+    // l holds A[l % 16][l / 16] and B[l / 16][l % 16] of a 16 x 4 by 4 x 16
+    // product, and rows 4 (l / 16) + j, column l % 16 of the 16 x 16 sum);
+    // a complex product is four real ones, K runs over 4 terms per step
+    // (rows beyond W read the last row, their terms zeroed)
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const int lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
+    const int KS = (W + 3) >> 2, RT = (W + 15) >> 4;
+    // stage A: G(u, g) over (16-row tile rt of u, 16-column tile tc of g) pairs
+    for (int pr = wave; pr < RT * R; pr += 4) {
+      const int rt = pr / R, tc = pr - (pr / R) * R;
+      const int uu = min(16 * rt + lr, W - 1);
+      v4f gr = {0.f, 0.f, 0.f, 0.f}, gi = {0.f, 0.f, 0.f, 0.f};
+      for (int ks = 0; ks < KS; ++ks) {
+        const int v = 4 * ks + lk, vv = min(v, W - 1);
+        float2 f = F[uu * W + vv];
+        if (v >= W) f = make_float2(0.f, 0.f);
+        const float2 e = E[vv * NP + 16 * tc + lr];
+        gr = __builtin_amdgcn_mfma_f32_16x16x4f32(f.x, e.x, gr, 0, 0, 0);
+        gr = __builtin_amdgcn_mfma_f32_16x16x4f32(-f.y, e.y, gr, 0, 0, 0);
+        gi = __builtin_amdgcn_mfma_f32_16x16x4f32(f.x, e.y, gi, 0, 0, 0);
+        gi = __builtin_amdgcn_mfma_f32_16x16x4f32(f.y, e.x, gi, 0, 0, 0);
+      }
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj) {
+        const int u = 16 * rt + 4 * lk + jj;
+        if (u < W) G[u * NP + 16 * tc + lr] = make_float2(gr[jj], gi[jj]);
+      }
+    }
+    __syncthreads();
+    // stage B: f(a, g) = sum_u Er(u, a) Gr(u, g) - Ei(u, a) Gi(u, g); the wave
+    // owns alpha tiles wave + 4 t (t < R / 4) and every gamma tile
+    constexpr int TR = R / 4;
+    v4f acc[TR][R];
+#pragma unroll
+    for (int t = 0; t < TR; ++t)
+#pragma unroll
+      for (int c = 0; c < R; ++c) acc[t][c] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < KS; ++ks) {
+      const int u = 4 * ks + lk, uu = min(u, W - 1);
+      const bool ok = u < W;
+      float ar[TR], ai[TR];
+#pragma unroll
+      for (int t = 0; t < TR; ++t) {
+        const float2 e = E[uu * NP + 16 * (wave + 4 * t) + lr];
+        ar[t] = ok ? e.x : 0.f;
+        ai[t] = ok ? -e.y : 0.f;
+      }
+#pragma unroll
+      for (int c = 0; c < R; ++c) {
+        const float2 gv = G[uu * NP + 16 * c + lr];
+#pragma unroll
+        for (int t = 0; t < TR; ++t) {
+          acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[t], gv.x, acc[t][c], 0, 0, 0);
+          acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[t], gv.y, acc[t][c], 0, 0, 0);
+        }
+      }
+    }
+    const float inv = 1.0f / ((float)n * (float)n);
+    unsigned keys[R * R];  // float_key of the values, i = (4 t + jj) R + c
+#pragma unroll
+    for (int t = 0; t < TR; ++t)
+#pragma unroll
+      for (int jj = 0; jj < 4; ++jj)
+#pragma unroll
+        for (int c = 0; c < R; ++c) {
+          const float v = acc[t][c][jj] * inv;
+          const int a = 16 * (wave + 4 * t) + 4 * lk + jj, g = 16 * c + lr;
+          const bool in = NP == n || (a < n && g < n);
+          keys[(4 * t + jj) * R + c] = in ? float_key(v) : 0u;
+          if (grid_out != nullptr && in) grid_out[((long long)b * n + j) * nn + a * n + g] = v;
+        }
+    // flat ids a n + g rise with i: alpha with (t, jj), gamma with c
+    block_top_c(keys,
+                [&](int i) {
+                  const int tj = i / R;
+                  return (16 * (wave + 4 * (tj >> 2)) + 4 * lk + (tj & 3)) * n + 16 * (i % R) + lr;
+                },
+                C, ((long long)b * n + j) * C, j * nn, part_val, part_idx);
+    return;
+  } else {
   // stage A: G(u, g) = sum_v F(u, v) e^{2 pi i v g / n}
   constexpr int SA = 256 % NP == 0 ? 256 / NP : 1, KR = 8;
   if (256 % NP == 0 && W <= KR * SA) {
@@ -454,6 +554,7 @@ __global__ __launch_bounds__(256) void sofft_slice_blk_kernel(
   // the smallest id among equal values
   block_top_c(keys, [&](int i) { return (ty + 16 * (i / R)) * n + tx + 16 * (i % R); }, C,
               ((long long)b * n + j) * C, j * nn, part_val, part_idx);
+  }
 }
 
 // Merge the per-row candidates into the global top-C per particle and convert
@@ -1117,6 +1218,20 @@ int mch_so3_plan_create(int L_max, int L0, int K, int max_batch, int max_candida
   MCH_TRY(hipMemcpy(p->d_grid, dg.data(), dg.size() * sizeof(float), hipMemcpyHostToDevice), "copy d_grid");
   MCH_TRY(hipMalloc(&p->twiddle, n * sizeof(float2)), "hipMalloc(twiddle)");
   MCH_TRY(hipMemcpy(p->twiddle, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice), "copy twiddle");
+  if (sofft_blk_r(L0, n) > 0) {
+    // E[u][x] = e^{2 pi i (u - (L0 - 1)) x / n}, x < 16 R (the columns past n
+    // repeat phases mod n), from the same twiddles
+    const int NP = 16 * sofft_blk_r(L0, n);
+    std::vector<float2> et((size_t)W * NP);
+    for (int ui = 0; ui < W; ++ui)
+      for (int x = 0; x < NP; ++x) {
+        int k = ((ui - (L0 - 1)) * x) % n;
+        if (k < 0) k += n;
+        et[(size_t)ui * NP + x] = tw[k];
+      }
+    MCH_TRY(hipMalloc(&p->etab, et.size() * sizeof(float2)), "hipMalloc(etab)");
+    MCH_TRY(hipMemcpy(p->etab, et.data(), et.size() * sizeof(float2), hipMemcpyHostToDevice), "copy etab");
+  }
   MCH_TRY(hipMalloc(&p->lgf, lg.size() * sizeof(float)), "hipMalloc(lgf)");
   MCH_TRY(hipMemcpy(p->lgf, lg.data(), lg.size() * sizeof(float), hipMemcpyHostToDevice), "copy lgf");
   MCH_TRY(hipMalloc(&p->sig_pk, (size_t)max_batch * mch::pk_size(L_max) * sizeof(float2)), "hipMalloc(sig_pk)");
@@ -1138,7 +1253,8 @@ int mch_so3_plan_create(int L_max, int L0, int K, int max_batch, int max_candida
   *out_plan = p;
   return MCH_OK;
 fail:
-  (void)hipFree(p->d_grid); (void)hipFree(p->twiddle); (void)hipFree(p->lgf); (void)hipFree(p->sig_pk);
+  (void)hipFree(p->d_grid); (void)hipFree(p->twiddle); (void)hipFree(p->etab); (void)hipFree(p->lgf);
+  (void)hipFree(p->sig_pk);
   (void)hipFree(p->part_val); (void)hipFree(p->part_idx); (void)hipFree(p->cand);
   (void)hipFree(p->state); (void)hipFree(p->accept);
   delete p;
@@ -1148,7 +1264,8 @@ fail:
 int mch_so3_plan_destroy(void* plan) {
   So3Plan* p = (So3Plan*)plan;
   if (!p) return MCH_OK;
-  (void)hipFree(p->d_grid); (void)hipFree(p->twiddle); (void)hipFree(p->lgf); (void)hipFree(p->sig_pk);
+  (void)hipFree(p->d_grid); (void)hipFree(p->twiddle); (void)hipFree(p->etab); (void)hipFree(p->lgf);
+  (void)hipFree(p->sig_pk);
   (void)hipFree(p->part_val); (void)hipFree(p->part_idx); (void)hipFree(p->cand);
   (void)hipFree(p->state); (void)hipFree(p->accept);
   delete p;
@@ -1167,7 +1284,7 @@ static int launch_sofft_slice(So3Plan* p, const void* sigma, long long ld_sigma,
     const size_t lds_blk = sofft_blk_lds(p->L0, R);
 #define MCH_BLK(R)                                                                               \
   hipLaunchKernelGGL(sofft_slice_blk_kernel<R>, grid, dim3(256), lds_blk, stream,                \
-                     (const float2*)sigma, ld_sigma, p->d_grid, p->twiddle, n, p->L0, p->nsz, C,  \
+                     (const float2*)sigma, ld_sigma, p->d_grid, p->etab, n, p->L0, p->nsz, C,     \
                      p->part_val, p->part_idx, grid_out)
     switch (R) {
       case 1: MCH_BLK(1); break;
