@@ -888,17 +888,24 @@ __global__ __launch_bounds__(slice_threads<N>()) void shift_inv_xy_kernel(
   }
   __syncthreads();
   mch::fft::forward<N, NL, NT, 1>(buf, S, 1, tw);
-  float bv = -INFINITY;
-  int bi = 0x7fffffff;
-  for (int t = threadIdx.x; t < NL * N; t += NT) {
-    const int j = t / N, s = t - (t / N) * N;
-    const float2 zv = buf[j * S + s];
-    const int xx = mch::fft::freq_of_slot<N>(s);
-    const int f1 = (z * N + j) * N + xx, f2 = f1 + NL * N;
+  // NT is a multiple of N: a thread keeps slot s (x = freq_of_slot(s)) and
+  // visits rows j = j0, j0 + NT / N, ...; the flat ids of each of its two
+  // sequences (rows j and rows j + N/2) rise, so a strict comparison keeps
+  // the first maximum of each, and the two merge by (value, id)
+  const int sx = threadIdx.x % N, j0 = threadIdx.x / N;
+  const int xx = mch::fft::freq_of_slot<N>(sx);
+  float b1 = -INFINITY, b2 = -INFINITY;
+  int i1 = 0x7fffffff, i2 = 0x7fffffff;
+  for (int j = j0; j < NL; j += NT / N) {
+    const float2 zv = buf[j * S + sx];
     const float v1 = 0.5f * zv.x, v2 = 0.5f * zv.y;
-    if (v1 > bv || (v1 == bv && f1 < bi)) { bv = v1; bi = f1; }
-    if (v2 > bv || (v2 == bv && f2 < bi)) { bv = v2; bi = f2; }
+    const int f1 = (z * N + j) * N + xx;
+    if (v1 > b1) { b1 = v1; i1 = f1; }
+    if (v2 > b2) { b2 = v2; i2 = f1 + NL * N; }
   }
+  float bv = b1;
+  int bi = i1;
+  if (b2 > bv || (b2 == bv && i2 < bi)) { bv = b2; bi = i2; }
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     const float ov = __shfl_xor(bv, off, 64);
