@@ -33,6 +33,37 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
+// Sum of v over the wave, returned to every lane (as a wave-uniform value):
+// within each row of 16 lanes four DPP additions (xor 1, xor 2, the half-row
+// and the row mirror: after them every lane holds its row's sum), then the
+// four row sums read as scalars and added in row order.  One VALU instruction
+// per exchange and no LDS round trips (a shuffle goes through ds_bpermute and
+// its address arithmetic).  Every lane of the wave must be active.
+template <int CTRL>
+__device__ __forceinline__ float dpp_add(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF,
+                                                        false));
+}
+__device__ __forceinline__ float wave_sum_dpp(float v) {
+  v = dpp_add<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  v = dpp_add<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v = dpp_add<0x141>(v);  // row_half_mirror
+  v = dpp_add<0x140>(v);  // row_mirror
+  const int b = __float_as_int(v);
+  return ((__int_as_float(__builtin_amdgcn_readlane(b, 0)) + __int_as_float(__builtin_amdgcn_readlane(b, 16))) +
+          __int_as_float(__builtin_amdgcn_readlane(b, 32))) +
+         __int_as_float(__builtin_amdgcn_readlane(b, 48));
+}
+
+// floor(sqrt(x)) for 0 <= x < 2^24: the hardware square root (within 1 ulp)
+// and one correction each way.
+__device__ __forceinline__ int isqrt_small(int x) {
+  int r = (int)__builtin_amdgcn_sqrtf((float)x);
+  if (r * r > x) --r;
+  if ((r + 1) * (r + 1) <= x) ++r;
+  return r;
+}
+
 // Maximum of w over the wave, returned to every lane.  Within each row of 16
 // lanes four DPP exchanges (xor 1, xor 2, the half-row and the row mirror:
 // after them every lane holds its row's maximum), then the four row maxima

@@ -724,13 +724,13 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
     const int p = t * 64 + lane;
     const bool valid = p < P;
     const int pc = valid ? p : P - 1;
-    int mp = (int)sqrtf((float)pc);
-    if (mp * mp > pc) --mp;
-    if ((mp + 1) * (mp + 1) <= pc) ++mp;
+    const int mp = mch::isqrt_small(pc);
     const int m = pc - mp * mp - mp;
-    int lstart = (int)sqrtf((float)(t * 64));
-    if (lstart * lstart > t * 64) --lstart;
-    if ((lstart + 1) * (lstart + 1) <= t * 64) ++lstart;
+    // mp = floor(sqrt(p)) rises with p: the task's smallest start degree is its
+    // first lane's, the largest among its valid lanes its last valid lane's
+    // (both wave-uniform)
+    const int lstart = __builtin_amdgcn_readfirstlane(mch::isqrt_small(t * 64));
+    const int mplast = __builtin_amdgcn_readfirstlane(mch::isqrt_small(min(t * 64 + 63, P - 1)));
     // seed at l0 = mp: d = (-1)^{mp-m} sqrt(C(2mp, mp+m)) c^{mp+m} s^{mp-m}
     const int a = mp + m, bb = mp - m;
     const float l2K = 0.5f * (lgf[2 * mp] - lgf[a] - lgf[2 * mp - a]);
@@ -815,11 +815,8 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
         }
       }
     };
-    // the largest start degree among the wave's valid lanes (wave-uniform)
-    int mpv = valid ? mp : lstart;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) mpv = max(mpv, __shfl_xor(mpv, off, 64));
-    const int l1 = min(__builtin_amdgcn_readfirstlane(mpv), Li);
+    // the largest start degree among the wave's valid lanes
+    const int l1 = min(mplast, Li);
     // sigma of shells l + 1 and l + 2 in flight while shell l runs (the offsets
     // are wave-uniform; reads past the particle's sigma return 0)
     unsigned soff = 4u * (unsigned)mch::pk_offset(lstart);
@@ -885,12 +882,14 @@ __device__ void march_eval(const float2* __restrict__ sp, unsigned sp_bytes, int
       }
     }
   }
+  // (the score-only evaluation's other sums are zero and not reduced)
+  constexpr int NSUM = SCORE_ONLY ? 1 : 10;
 #pragma unroll
-  for (int i = 0; i < 10; ++i) acc[i] = mch::wave_sum(acc[i]);
+  for (int i = 0; i < NSUM; ++i) acc[i] = mch::wave_sum_dpp(acc[i]);
   __syncthreads();
   if (lane == 0) {
 #pragma unroll
-    for (int i = 0; i < 10; ++i) red[i * kMarchWaves + wave] = acc[i];
+    for (int i = 0; i < 10; ++i) red[i * kMarchWaves + wave] = i < NSUM ? acc[i] : 0.f;
   }
   __syncthreads();
   if (tid == 0) {  // the callers read the sums in thread 0 only
