@@ -41,8 +41,8 @@ __device__ __forceinline__ float wave_sum(float v) {
 // its address arithmetic).  Every lane of the wave must be active.
 template <int CTRL>
 __device__ __forceinline__ float dpp_add(float v) {
-  return v + __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, 0xF, 0xF,
-                                                        false));
+  // a plain DPP move (no old value): the compiler folds it into the addition
+  return v + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float wave_sum_dpp(float v) {
   v = dpp_add<0xB1>(v);   // quad_perm [1, 0, 3, 2]
