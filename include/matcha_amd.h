@@ -157,6 +157,17 @@ int mch_fle_project(void* plan, const float* vols, long long ld_vol, int B, void
  * (src/matcha/utils/io_utils.py:113-178) without the patch copy. */
 int mch_fle_project_shifted(void* plan, const float* vols, long long ld_vol, int B, const float* shift_zyx,
                             void* coeffs, long long lda, hipStream_t stream);
+/* The coefficients of the four volumes of mch_shift_tangent (the patch at a
+ * continuous shift t (Bp, 3) and its three exact t-derivatives) into rows b,
+ * Bp + b, 2 Bp + b, 3 Bp + b of coeffs (4 Bp <= the plan's batch): what
+ * mch_fle_project of mch_shift_tangent's output gives, with the volumes formed
+ * per orbit image inside the operand build instead of written to and read
+ * from HBM.  FP32 and BF16x3 precision (FP16x3: MCH_ERR_UNSUPPORTED).  No
+ * reference counterpart (the reference differentiates the score in Euler
+ * angles only, src/matcha/core/WignerDGradients.py:247-302); it serves the
+ * differentiable score's shift derivative. */
+int mch_fle_project_tangents(void* plan, const float* vols, long long ld_vol, int Nv, int Bp, const float* t,
+                             void* coeffs, long long lda, hipStream_t stream);
 
 /* The projection GEMM alone on the operand of the last mch_fle_project
  * (timing aid; no reference counterpart). */

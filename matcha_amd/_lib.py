@@ -49,6 +49,8 @@ _SIGNATURES = [
     ("mch_fle_copy_basis", c_int, [c_void_p, c_void_p, c_void_p]),
     ("mch_fle_project", c_int, [c_void_p, c_void_p, c_ll, c_int, c_void_p, c_ll, c_void_p]),
     ("mch_fle_project_shifted", c_int, [c_void_p, c_void_p, c_ll, c_int, c_void_p, c_void_p, c_ll, c_void_p]),
+    ("mch_fle_project_tangents", c_int, [c_void_p, c_void_p, c_ll, c_int, c_int, c_void_p, c_void_p, c_ll,
+                                          c_void_p]),
     ("mch_fle_gemm_only", c_int, [c_void_p, c_int, c_void_p]),
     ("mch_fle_set_precision", c_int, [c_void_p, c_int]),
     ("mch_fle_tuned_depth", c_int, [c_void_p]),

@@ -433,6 +433,113 @@ __global__ __launch_bounds__(256) void fle_symmetrize_bf16_kernel(
   }
 }
 
+// The patch V_t of each particle at a continuous shift t (the Nv^3 volume at
+// a + Nv/2 - N/2 + t, a = (z, y, x) in the N^3 box, trilinear, zero outside
+// the volume) and its three t-derivatives, exact inside each cell (on a cell
+// face the one-sided derivative of the cell above), formed per orbit image and
+// symmetrised straight into operand rows j Bp + b of a batch of 4 Bp volumes
+// (j = 0: V_t; j = 1, 2, 3: d/dt_0, d/dt_1, d/dt_2; t_0 moves x, t_2 moves z):
+// the four volumes never go through HBM.  FP32 operand (BF16 = false, the
+// layout of fle_symmetrize_kernel) or BF16 hi/lo (that of
+// fle_symmetrize_bf16_kernel, whose K padding columns receive zeros); one
+// thread per (particle, column v).
+template <bool BF16>
+__global__ __launch_bounds__(256) void fle_symmetrize_tangent_kernel(
+    const float* __restrict__ vols, long long ld_vol, int Nv, int Bp, const int* __restrict__ orbit, int V0,
+    int N, int Kpad, int Mpad, const float* __restrict__ tsh, float* __restrict__ X, uint16_t* __restrict__ Xhi,
+    uint16_t* __restrict__ Xlo) {
+  const int KH = Kpad >> 1, Bv = 4 * Bp;
+  const int nv = BF16 ? KH : V0;
+  const long long tot = (long long)Bp * nv;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < tot;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int b = (int)(t / nv), v = (int)(t - (long long)b * nv);
+    float g[4][16];
+    if (v < V0) {
+      const float* f = vols + (long long)b * ld_vol;
+      const float base = (float)(Nv / 2 - N / 2);
+      const float t0 = tsh[b * 3 + 0], t1 = tsh[b * 3 + 1], t2 = tsh[b * 3 + 2];
+      for (int i = 0; i < 16; ++i) {
+        const int id = orbit[(long long)v * 16 + i];
+        if (id < 0) {  // an image outside the box (radius >= N/2)
+          g[0][i] = g[1][i] = g[2][i] = g[3][i] = 0.f;
+          continue;
+        }
+        const int z = id / (N * N), y = (id / N) % N, x = id % N;
+        const float px = (float)x + base + t0, py = (float)y + base + t1, pz = (float)z + base + t2;
+        const float fx0 = floorf(px), fy0 = floorf(py), fz0 = floorf(pz);
+        const int x0 = (int)fx0, y0 = (int)fy0, z0 = (int)fz0;
+        const float fx = px - fx0, fy = py - fy0, fz = pz - fz0;
+        float c[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int xi = x0 + (k & 1), yi = y0 + ((k >> 1) & 1), zi = z0 + (k >> 2);
+          const bool ok = (unsigned)xi < (unsigned)Nv && (unsigned)yi < (unsigned)Nv && (unsigned)zi < (unsigned)Nv;
+          c[k] = ok ? f[((long long)zi * Nv + yi) * Nv + xi] : 0.f;
+        }
+        const float gx = 1.f - fx, gy = 1.f - fy, gz = 1.f - fz;
+        // c[k]: k bit 0 = x, bit 1 = y, bit 2 = z
+        g[0][i] = gz * (gy * (gx * c[0] + fx * c[1]) + fy * (gx * c[2] + fx * c[3])) +
+                  fz * (gy * (gx * c[4] + fx * c[5]) + fy * (gx * c[6] + fx * c[7]));
+        g[1][i] = gz * (gy * (c[1] - c[0]) + fy * (c[3] - c[2])) + fz * (gy * (c[5] - c[4]) + fy * (c[7] - c[6]));
+        g[2][i] = gz * (gx * (c[2] - c[0]) + fx * (c[3] - c[1])) + fz * (gx * (c[6] - c[4]) + fx * (c[7] - c[5]));
+        g[3][i] = gy * (gx * (c[4] - c[0]) + fx * (c[5] - c[1])) + fy * (gx * (c[6] - c[2]) + fx * (c[7] - c[3]));
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) g[j][i] = 0.f;
+    }
+    auto store = [&](long long idx, float val) {
+      if constexpr (BF16) {
+        uint16_t h, l;
+        Split16<false>::split(val, h, l);
+        Xhi[idx] = h;
+        Xlo[idx] = l;
+      } else {
+        X[idx] = val;
+      }
+    };
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int vb = j * Bp + b;
+      // the class combinations of fle_symmetrize_kernel; g[j][p + 4q + 8t]
+#pragma unroll
+      for (int eps = 0; eps < 2; ++eps) {
+        float A[2][3], Bi[2][3];  // [t][rho]
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt) {
+          float s0 = 0.f, s2 = 0.f, s1r = 0.f, s1i = 0.f;
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const float sg = (eps && q) ? -1.f : 1.f;
+            const float* h = &g[j][4 * q + 8 * tt];
+            s0 += sg * (h[0] + h[1] + h[2] + h[3]);
+            s2 += sg * (h[0] - h[1] + h[2] - h[3]);
+            s1r += sg * (h[0] - h[2]);
+            s1i += sg * (h[3] - h[1]);
+          }
+          A[tt][0] = s0;  Bi[tt][0] = 0.f;
+          A[tt][1] = s1r; Bi[tt][1] = s1i;
+          A[tt][2] = s2;  Bi[tt][2] = 0.f;
+        }
+#pragma unroll
+        for (int rho = 0; rho < 3; ++rho) {  // rho = 3 reads the rho = 1 slab
+          const int cls = eps * 4 + rho;
+          const long long orow = ((long long)cls * Mpad + vb) * Kpad + v;
+          const long long oim = ((long long)cls * Mpad + Bv + vb) * Kpad + v;
+          const bool odd = rho & 1;
+          store(orow, A[0][rho] + A[1][rho]);
+          if (odd || straddle_row(vb, Bv)) store(orow + KH, Bi[0][rho] - Bi[1][rho]);
+          if (odd || straddle_row(Bv + vb, Bv)) store(oim, Bi[0][rho] + Bi[1][rho]);
+          store(oim + KH, A[1][rho] - A[0][rho]);
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // 2. grouped FP32 MFMA GEMM with split-K
 // ---------------------------------------------------------------------------
@@ -1680,6 +1787,47 @@ int mch_fle_project_shifted(void* plan, const float* vols, long long ld_vol, int
                      p->precision == kFp32 ? p->part : p->part16, splits, p->Mpad, p->Ntot, B,
                      p->chan_plus, p->chan_minus, p->chan_m, (float2*)coeffs, lda,
                      fp16 ? p->colscale : nullptr, fp16 ? p->rowmax : nullptr,
+                     make_int4(p->col_off[3], p->col_off[4], p->col_off[7], p->col_off[8]), skip_kchunk, *p);
+  return mch::hip_check(hipGetLastError(), "fle_finalize_kernel launch");
+}
+
+// Coefficients of the patches V_t and their t-derivatives (the four volumes
+// of fle_symmetrize_tangent_kernel) into rows b, Bp + b, 2 Bp + b, 3 Bp + b of
+// coeffs: one pass over the particles instead of four volumes written and
+// read back.  FP32 and BF16x3 precision.
+int mch_fle_project_tangents(void* plan, const float* vols, long long ld_vol, int Nv, int Bp, const float* t,
+                             void* coeffs, long long lda, hipStream_t stream) {
+  FlePlan* p = (FlePlan*)plan;
+  if (!p || !vols || !t || !coeffs) { mch::set_error("mch_fle_project_tangents: NULL argument"); return MCH_ERR_ARG; }
+  const int B = 4 * Bp;
+  if (Bp < 1 || B > p->max_batch || Nv < p->N || ld_vol < (long long)Nv * Nv * Nv || lda < p->ne) {
+    mch::set_error("mch_fle_project_tangents: bad sizes (4 x %d volumes, max %d; Nv=%d N=%d)", Bp, p->max_batch,
+                   Nv, p->N);
+    return MCH_ERR_SIZE;
+  }
+  if (p->precision == kFp16x3) {
+    mch::set_error("mch_fle_project_tangents: FP16x3 precision is not supported (use FP32 or BF16x3)");
+    return MCH_ERR_UNSUPPORTED;
+  }
+  const bool bf16 = p->precision == kBf16x3;
+  const long long tot = (long long)Bp * (bf16 ? p->Kpad / 2 : p->V0);
+  const int nb = (int)std::min<long long>((tot + 255) / 256, 16384);
+  if (bf16)
+    hipLaunchKernelGGL(fle_symmetrize_tangent_kernel<true>, dim3(nb), dim3(256), 0, stream, vols, ld_vol, Nv, Bp,
+                       p->orbit, p->V0, p->N, p->Kpad, p->Mpad, t, nullptr, p->Xhi, p->Xlo);
+  else
+    hipLaunchKernelGGL(fle_symmetrize_tangent_kernel<false>, dim3(nb), dim3(256), 0, stream, vols, ld_vol, Nv, Bp,
+                       p->orbit, p->V0, p->N, p->Kpad, p->Mpad, t, p->X, nullptr, nullptr);
+  int rc = mch::hip_check(hipGetLastError(), "fle_symmetrize_tangent_kernel launch");
+  if (rc) return rc;
+  int splits = 0, skip_kchunk = 0;
+  if ((rc = launch_gemm(p, B, stream, &splits, &skip_kchunk))) return rc;
+  const long long ftot = (long long)B * p->Ntot;
+  int fb = (int)((ftot + 255) / 256);
+  if (fb > 8192) fb = 8192;
+  hipLaunchKernelGGL(fle_finalize_kernel, dim3(fb), dim3(256), 0, stream,
+                     p->precision == kFp32 ? p->part : p->part16, splits, p->Mpad, p->Ntot, B, p->chan_plus,
+                     p->chan_minus, p->chan_m, (float2*)coeffs, lda, nullptr, nullptr,
                      make_int4(p->col_off[3], p->col_off[4], p->col_off[7], p->col_off[8]), skip_kchunk, *p);
   return mch::hip_check(hipGetLastError(), "fle_finalize_kernel launch");
 }

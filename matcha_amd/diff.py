@@ -58,6 +58,28 @@ class ScoreEngine:
         """Ball-harmonic coefficients (ne,) complex64 of a template volume."""
         return self.ccm.fle.evaluate_t(template.reshape(1, self.N, self.N, self.N).to(self.device))[0]
 
+    def tangent_coefficients(self, vols: torch.Tensor, t: torch.Tensor, fused: bool = None) -> torch.Tensor:
+        """(4B, ne) coefficients of the patches at shift t (B, 3) and of their three
+        t-derivatives.  fused (default: the FP32 and BF16x3 projections) forms
+        the four volumes inside the projection's operand build
+        (mch_fle_project_tangents); otherwise they go through HBM
+        (mch_shift_tangent, then the projection)."""
+        B, Nv, N = vols.shape[0], vols.shape[-1], self.N
+        fle = self.ccm.fle
+        stream = _lib.stream_handle(self.device)
+        if fused is None:
+            fused = fle.compute_precision in ("fp32", "bf16x3")
+        if fused:
+            coeffs = torch.empty((4 * B, self.ne), dtype=torch.complex64, device=self.device)
+            _lib.check(_lib.lib().mch_fle_project_tangents(fle._plan, vols.data_ptr(), vols.stride(0), Nv, B,
+                                                           t.data_ptr(), coeffs.data_ptr(), coeffs.stride(0),
+                                                           stream), "mch_fle_project_tangents")
+            return coeffs
+        tang = torch.empty((4 * B, N, N, N), dtype=torch.float32, device=self.device)
+        _lib.check(_lib.lib().mch_shift_tangent(vols.data_ptr(), vols.stride(0), Nv, B, N, t.data_ptr(),
+                                                tang.data_ptr(), tang.stride(0), stream), "mch_shift_tangent")
+        return fle.evaluate_t(tang)
+
     def _prepare(self, omega, shift, aT, particles):
         """Inputs of mch_score_gradients: (B, angles, jr, coeffs, sigma)."""
         B = omega.shape[0]
@@ -78,11 +100,7 @@ class ScoreEngine:
         angles = torch.empty((B, 3), dtype=torch.float64, device=self.device)
         jr = torch.empty((B, 9), dtype=torch.float64, device=self.device)
         _lib.check(lib.mch_score_pose(om.data_ptr(), B, angles.data_ptr(), jr.data_ptr(), stream), "mch_score_pose")
-        N = self.N
-        tang = torch.empty((4 * B, N, N, N), dtype=torch.float32, device=self.device)
-        _lib.check(lib.mch_shift_tangent(vols.data_ptr(), vols.stride(0), Nv, B, N, t.data_ptr(), tang.data_ptr(),
-                                         tang.stride(0), stream), "mch_shift_tangent")
-        coeffs = self.ccm.fle.evaluate_t(tang)
+        coeffs = self.tangent_coefficients(vols, t)
         dsz = self.ccm.sigma.shape[1]
         sigma = torch.empty((4 * B, dsz), dtype=torch.complex64, device=self.device)
         _lib.check(lib.mch_compute_sigma_degree(self.ccm._corr, a_t.data_ptr(), coeffs.data_ptr(),

@@ -191,3 +191,28 @@ def test_split_precision_projection_keeps_the_bars(gpu_device, precision):
         assert abs(out[b, 0] - s) < 1e-4 * abs(s), (b, out[b, 0], s)
         ref = pr.fd_gradient(omega[b], t[b])
         assert np.abs(out[b, 1:] - ref).max() < 1e-3 * np.abs(ref).max(), (b, out[b, 1:], ref)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("n, bl, radius, nv, bp", [(24, 8, 11.0, 24, 3), (24, 8, 13.0, 28, 70),
+                                                    (32, 10, 16.0, 36, 5)])
+def test_fused_tangent_projection_matches_volumes_through_hbm(gpu_device, precision, n, bl, radius, nv, bp):
+    # the patch and tangent volumes formed inside the operand build against
+    # the same four volumes written by mch_shift_tangent and projected: the
+    # operand values are the same trilinear formulas, so the coefficients
+    # agree to FP32 rounding of the class sums (70 particles: 280 volumes, so
+    # the real and imaginary operand rows span three 256-row tiles; radius 13
+    # in box 24: images beyond the box; nv > n: patches cut from larger volumes;
+    # shifts on cell faces and negative)
+    from matcha_amd.diff import ScoreEngine
+    rng = np.random.default_rng(7)
+    eng = ScoreEngine(n, bl, radius, 4, 4 * bp, gpu_device, compute_precision=precision)
+    vols = torch.tensor(rng.standard_normal((bp, nv, nv, nv)), dtype=torch.float32, device=gpu_device)
+    t = rng.uniform(-2.5, 2.5, (bp, 3))
+    t[0] = (1.0, -2.0, 0.0)
+    t = torch.tensor(t, dtype=torch.float32, device=gpu_device)
+    fused = eng.tangent_coefficients(vols, t, fused=True)
+    ref = eng.tangent_coefficients(vols, t, fused=False)
+    scale = ref.abs().amax(dim=1, keepdim=True).clamp_min(1e-30)
+    err = ((fused - ref).abs() / scale).max().item()
+    assert err < 2e-6, err
