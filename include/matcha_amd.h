@@ -158,7 +158,8 @@ int mch_fle_project(void* plan, const float* vols, long long ld_vol, int B, void
 int mch_fle_project_shifted(void* plan, const float* vols, long long ld_vol, int B, const float* shift_zyx,
                             void* coeffs, long long lda, hipStream_t stream);
 /* The coefficients of the four volumes of mch_shift_tangent (the patch at a
- * continuous shift t (Bp, 3) and its three exact t-derivatives) into rows b,
+ * continuous shift t (Bp, 3) and its three exact t-derivatives; volumes of
+ * Nv^3, N <= Nv <= 1024) into rows b,
  * Bp + b, 2 Bp + b, 3 Bp + b of coeffs (4 Bp <= the plan's batch): what
  * mch_fle_project of mch_shift_tangent's output gives, with the volumes formed
  * per orbit image inside the operand build instead of written to and read

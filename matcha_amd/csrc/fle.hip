@@ -456,15 +456,22 @@ __global__ __launch_bounds__(256) void fle_symmetrize_tangent_kernel(
     const int b = (int)(t / nv), v = (int)(t - (long long)b * nv);
     float g[4][16];
     if (v < V0) {
-      const float* f = vols + (long long)b * ld_vol;
+      // corners through the buffer range check (zero outside the volume),
+      // the orbit's 16 indices read at once and four images' corner loads in
+      // flight together
+      const __amdgpu_buffer_rsrc_t rv = make_rsrc(vols + (long long)b * ld_vol, (unsigned)Nv * Nv * Nv * 4u);
       const float base = (float)(Nv / 2 - N / 2);
       const float t0 = tsh[b * 3 + 0], t1 = tsh[b * 3 + 1], t2 = tsh[b * 3 + 2];
+      int ids[16];
+      const int4* o4 = reinterpret_cast<const int4*>(orbit + (long long)v * 16);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int4 q = o4[i];
+        ids[4 * i] = q.x; ids[4 * i + 1] = q.y; ids[4 * i + 2] = q.z; ids[4 * i + 3] = q.w;
+      }
+#pragma unroll 4
       for (int i = 0; i < 16; ++i) {
-        const int id = orbit[(long long)v * 16 + i];
-        if (id < 0) {  // an image outside the box (radius >= N/2)
-          g[0][i] = g[1][i] = g[2][i] = g[3][i] = 0.f;
-          continue;
-        }
+        const int id = ids[i];  // < 0: an image outside the box (radius >= N/2), zero
         const int z = id / (N * N), y = (id / N) % N, x = id % N;
         const float px = (float)x + base + t0, py = (float)y + base + t1, pz = (float)z + base + t2;
         const float fx0 = floorf(px), fy0 = floorf(py), fz0 = floorf(pz);
@@ -474,8 +481,9 @@ __global__ __launch_bounds__(256) void fle_symmetrize_tangent_kernel(
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
           const int xi = x0 + (k & 1), yi = y0 + ((k >> 1) & 1), zi = z0 + (k >> 2);
-          const bool ok = (unsigned)xi < (unsigned)Nv && (unsigned)yi < (unsigned)Nv && (unsigned)zi < (unsigned)Nv;
-          c[k] = ok ? f[((long long)zi * Nv + yi) * Nv + xi] : 0.f;
+          const bool ok = id >= 0 && (unsigned)xi < (unsigned)Nv && (unsigned)yi < (unsigned)Nv &&
+                          (unsigned)zi < (unsigned)Nv;
+          c[k] = load_f1(rv, ok ? (unsigned)((zi * Nv + yi) * Nv + xi) * 4u : kOutOfRange);
         }
         const float gx = 1.f - fx, gy = 1.f - fy, gz = 1.f - fz;
         // c[k]: k bit 0 = x, bit 1 = y, bit 2 = z
@@ -1800,7 +1808,7 @@ int mch_fle_project_tangents(void* plan, const float* vols, long long ld_vol, in
   FlePlan* p = (FlePlan*)plan;
   if (!p || !vols || !t || !coeffs) { mch::set_error("mch_fle_project_tangents: NULL argument"); return MCH_ERR_ARG; }
   const int B = 4 * Bp;
-  if (Bp < 1 || B > p->max_batch || Nv < p->N || ld_vol < (long long)Nv * Nv * Nv || lda < p->ne) {
+  if (Bp < 1 || B > p->max_batch || Nv < p->N || Nv > 1024 || ld_vol < (long long)Nv * Nv * Nv || lda < p->ne) {
     mch::set_error("mch_fle_project_tangents: bad sizes (4 x %d volumes, max %d; Nv=%d N=%d)", Bp, p->max_batch,
                    Nv, p->N);
     return MCH_ERR_SIZE;
