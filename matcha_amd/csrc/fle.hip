@@ -457,8 +457,8 @@ __global__ __launch_bounds__(256) void fle_symmetrize_tangent_kernel(
     float g[4][16];
     if (v < V0) {
       // corners through the buffer range check (zero outside the volume),
-      // the orbit's 16 indices read at once and four images' corner loads in
-      // flight together
+      // the orbit's 16 indices read at once; the image loop fully unrolled, so
+      // that g stays in registers
       const __amdgpu_buffer_rsrc_t rv = make_rsrc(vols + (long long)b * ld_vol, (unsigned)Nv * Nv * Nv * 4u);
       const float base = (float)(Nv / 2 - N / 2);
       const float t0 = tsh[b * 3 + 0], t1 = tsh[b * 3 + 1], t2 = tsh[b * 3 + 2];
@@ -469,7 +469,7 @@ __global__ __launch_bounds__(256) void fle_symmetrize_tangent_kernel(
         const int4 q = o4[i];
         ids[4 * i] = q.x; ids[4 * i + 1] = q.y; ids[4 * i + 2] = q.z; ids[4 * i + 3] = q.w;
       }
-#pragma unroll 4
+#pragma unroll
       for (int i = 0; i < 16; ++i) {
         const int id = ids[i];  // < 0: an image outside the box (radius >= N/2), zero
         const int z = id / (N * N), y = (id / N) % N, x = id % N;
