@@ -89,18 +89,29 @@ struct CorrPlan {
 // is negated back in the finalize.  Slabs 3 and 7 are never written.
 __host__ __device__ __forceinline__ int x_slab(int cls) { return (cls & 3) == 3 ? cls - 2 : cls; }
 
+// Class of padded column c: the count of class starts at or below it, summed
+// with constant indices (a loop that indexes the by-value offsets with a
+// running class would copy them to scratch memory in every thread).
+__device__ __forceinline__ int column_class(const int (&col_off)[kClasses + 1], int c) {
+  int cls = 0;
+#pragma unroll
+  for (int i = 1; i < kClasses; ++i) cls += c >= col_off[i] ? 1 : 0;
+  return cls;
+}
+
 // K range a GEMM row tile [row0, row0 + rows) of class cls needs: the classes
 // m = 0, 2 mod 4 (cls even) have zero sin-weighted operands in their real
 // rows (< B) and zero cos-weighted ones in their imaginary rows, so a tile
 // wholly of one kind reads one half of K; other tiles read all of it.
 __host__ __device__ __forceinline__ void k_range(int cls, int row0, int rows, int B, int Kpad, int& klo,
                                                  int& khi) {
-  klo = 0;
-  khi = Kpad;
-  if ((cls & 1) == 0) {
-    if (row0 + rows <= B) khi = Kpad / 2;
-    else if (row0 >= B) klo = Kpad / 2;
-  }
+  // selects, not branches: branchy stores to the two outputs were turned into
+  // a store through a computed address into scratch memory
+  const bool half = (cls & 1) == 0;
+  const bool real_only = half && row0 + rows <= B;
+  const bool imag_only = half && !real_only && row0 >= B;
+  klo = imag_only ? Kpad / 2 : 0;
+  khi = real_only ? Kpad / 2 : Kpad;
 }
 
 // Whether operand row `row` lies in the one GEMM row tile that holds both real
@@ -561,8 +572,7 @@ __global__ __launch_bounds__(kGemmThreads) void fle_gemm_kernel(
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int col0 = blockIdx.x * BN;
   const int row0 = blockIdx.y * BM;
-  int cls = 0;
-  while (cls < kClasses - 1 && col0 >= plan_cols.col_off[cls + 1]) ++cls;
+  const int cls = column_class(plan_cols.col_off, col0);
   const float* A = X + (long long)x_slab(cls) * Mpad * Kpad;
   int klo, khi;
   k_range(cls, row0, BM, B, Kpad, klo, khi);
@@ -702,8 +712,7 @@ __global__ __launch_bounds__(64 * NW) void fle_gemm16_kernel(
   const int mt = lb % nmt, rest = lb / nmt;
   const int tn = rest % ntn, ks = rest / ntn;
   const int col0 = tn * BN16, row0 = mt * BM16 * RT;
-  int cls = 0;
-  while (cls < kClasses - 1 && col0 >= cols.col_off[cls + 1]) ++cls;
+  const int cls = column_class(cols.col_off, col0);
   int klo, khi;
   k_range(cls, row0, BM16 * RT, B, Kpad, klo, khi);
   const int kbeg = max(ks * kchunk, klo);
@@ -1059,8 +1068,7 @@ __global__ void fle_finalize_kernel(const float* __restrict__ part, int splits, 
     if (ip < 0) continue;
     float re = 0.f, im = 0.f;
     if (skip_kchunk > 0) {
-      int cls = 0;
-      while (cls < kClasses - 1 && c >= cols.col_off[cls + 1]) ++cls;
+      const int cls = column_class(cols.col_off, c);
       int klo_r, khi_r, klo_i, khi_i;
       k_range(cls, (b / 256) * 256, 256, B, cols.Kpad, klo_r, khi_r);
       k_range(cls, ((B + b) / 256) * 256, 256, B, cols.Kpad, klo_i, khi_i);
